@@ -1,0 +1,128 @@
+/*
+ * xtbeig: dense symmetric(-definite) eigensolver for one AMD MI355X (gfx950), a drop-in for the eigensolve in
+ * xTB's SCC cycle. Plain C ABI: pointers and sizes only.
+ *
+ * Two groups of entry points:
+ *  1. Host-memory, Fortran-by-reference entry points (every argument a pointer, 32-bit integers, column-major, no
+ *     hidden string lengths), with the names and argument meaning of the library the reference builds into libxtb
+ *     in place of xtb/src/mctc/lapack/sytrd.c (reference: Sources/Diagonalization/Exports.swift:1-138), plus
+ *     LAPACK-convention sytrd / syevd / sygvd under the xtbeig_ prefix (workspace queries with lwork = -1). Each
+ *     call copies its inputs to the GPU, runs there and copies the results back; a per-thread handle holds the
+ *     device workspace between calls.
+ *  2. Device-memory entry points on an explicit handle and HIP stream (xtbeig_dev_*), for callers that keep their
+ *     matrices resident on the GPU. These enqueue work and return; xtbeig_synchronize waits and reports status.
+ *
+ * Matrices are column-major with leading dimensions; "upper" means the upper triangle is referenced.
+ */
+#ifndef XTBEIG_H
+#define XTBEIG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1a. The sytrd.c symbol set (reference: Sources/Diagonalization/Exports.swift) -------------------------- */
+
+/* dstedc('I') semantics: eigenvalues of the symmetric tridiagonal (d, e) ascending in d, eigenvectors in z (n x n,
+ * ldz). e is destroyed. info = 0 on success, > 0 when a secular equation fails to converge. work / iwork are not
+ * used (the GPU holds its own workspace) but lwork / liwork must be at least 1 + 4n + n^2 / 3 + 5n as LAPACK's, or
+ * -1 for a query answered in work[0] / iwork[0]. Replaces Exports.swift:88-101 (xtb_dstedc). */
+void xtb_dstedc(const int *n, double *d, double *e, double *z, const int *ldz, double *work, const int *lwork,
+                int *iwork, const int *liwork, int *info);
+/* FP32 interface of the same (Exports.swift:103-115, xtb_sstedc); the tridiagonal problem is solved in FP64 on the
+ * GPU and rounded to FP32. */
+void xtb_sstedc(const int *n, float *d, float *e, float *z, const int *ldz, float *work, const int *lwork,
+                int *iwork, const int *liwork, int *info);
+/* dst(1:n, 1:n) = the symmetric matrix whose upper triangle is src's (src's lower triangle is not read)
+ * (Exports.swift:117-138, xtb_dsymmetrize_pad). */
+void xtb_dsymmetrize_pad(const int *n, const double *src, const int *lds, double *dst, const int *ldd);
+
+/* ---- 1b. LAPACK-convention entry points ------------------------------------------------------------------------ */
+
+/* dsyevd(jobz, uplo, ...): all eigenvalues (w, ascending) and, for jobz = 'V', eigenvectors in a. lwork >=
+ * 1 + 6n + 2n^2, liwork >= 3 + 5n ('V'); lwork = -1 or liwork = -1 is a workspace query. The reference's
+ * eigensolve (tridiagonalize/sbrReduce + stedc + back-transformation, Sources/EigenBench/main.swift:291-420). */
+void xtbeig_dsyevd(const char *jobz, const char *uplo, const int *n, double *a, const int *lda, double *w,
+                   double *work, const int *lwork, int *iwork, const int *liwork, int *info);
+void xtbeig_ssyevd(const char *jobz, const char *uplo, const int *n, float *a, const int *lda, float *w, float *work,
+                   const int *lwork, int *iwork, const int *liwork, int *info);
+/* dsygvd(itype = 1, jobz, uplo, ...): A x = lambda B x with B symmetric positive definite. On exit a holds the
+ * B-orthonormal eigenvectors ('V') and b the Cholesky factor U (B = U^T U, upper) as LAPACK leaves it. info > n:
+ * B's leading minor of order info - n is not positive definite. The xTB fact_solve generalized solve (reference
+ * spec: scripts/kernels/eigbench.f90:101-168). */
+void xtbeig_dsygvd(const int *itype, const char *jobz, const char *uplo, const int *n, double *a, const int *lda,
+                   double *b, const int *ldb, double *w, double *work, const int *lwork, int *iwork,
+                   const int *liwork, int *info);
+/* As xtbeig_dsygvd, but computed in FP32 on the matrix cores and refined in FP64 (Ogita-Aishima, the reference's
+ * fp32_refine scheme, scripts/kernels/fp32_error.c:95-158); same arguments and results as the FP64 call. */
+void xtbeig_dsygvd_mixed(const int *itype, const char *jobz, const char *uplo, const int *n, double *a,
+                         const int *lda, double *b, const int *ldb, double *w, double *work, const int *lwork,
+                         int *iwork, const int *liwork, int *info);
+
+/* ---- 2. Device-memory API ------------------------------------------------------------------------------------- */
+
+typedef struct xtbeig_handle *xtbeig_handle_t;
+
+/* Precision of a generalized solve. */
+enum { XTBEIG_FP64 = 0, XTBEIG_FP32_REFINE = 1 };
+
+/* Create a handle on a HIP device (its own non-blocking stream). Returns 0 on success. */
+int xtbeig_create(xtbeig_handle_t *handle, int device);
+int xtbeig_destroy(xtbeig_handle_t handle);
+/* Enqueue on the caller's HIP stream (hipStream_t passed as void*) instead of the handle's own. */
+int xtbeig_set_stream(xtbeig_handle_t handle, void *stream);
+void *xtbeig_get_stream(xtbeig_handle_t handle);
+/* Band width of the two-stage reduction (0: the built-in table per n) and the refinement iterations. */
+int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations);
+/* Wait for the handle's stream; status[0..2] = Cholesky info, bulge-chase stall flag, divide-and-conquer info
+ * (status may be NULL). Returns 0 when all three are zero. Clears them. */
+int xtbeig_synchronize(xtbeig_handle_t handle, int *status);
+/* Message of the last failed call on this thread. */
+const char *xtbeig_last_error(void);
+
+/* Generalized solve of device-resident H, S (n x n, full symmetric storage, not modified): eigenvalues w (n,
+ * ascending) and S-orthonormal eigenvectors C (n x n, ldc). precision: XTBEIG_FP64 or XTBEIG_FP32_REFINE. */
+int xtbeig_dev_sygvd(xtbeig_handle_t handle, int precision, int n, const double *h, int ldh, const double *s,
+                     int lds, double *w, double *c, int ldc);
+/* Standard solve of a device-resident symmetric A (full storage, not modified). */
+int xtbeig_dev_dsyevd(xtbeig_handle_t handle, int n, const double *a, int lda, double *w, double *z, int ldz);
+int xtbeig_dev_ssyevd(xtbeig_handle_t handle, int n, const float *a, int lda, double *w, float *z, int ldz);
+/* Tridiagonal (d, e) -> eigenvalues ascending in d, eigenvectors in z; e destroyed (FP64). */
+int xtbeig_dev_dstedc(xtbeig_handle_t handle, int n, double *d, double *e, double *z, int ldz);
+
+/* ---- 2b. Stage entry points (device memory), for callers composing their own pipelines and for testing ------ */
+
+/* C = alpha op(A) op(B) + beta C on the matrix cores (op: 'N' or 'T'). */
+int xtbeig_dev_dgemm(xtbeig_handle_t handle, char transa, char transb, int m, int n, int k, double alpha,
+                     const double *a, int lda, const double *b, int ldb, double beta, double *c, int ldc);
+int xtbeig_dev_sgemm(xtbeig_handle_t handle, char transa, char transb, int m, int n, int k, float alpha,
+                     const float *a, int lda, const float *b, int ldb, float beta, float *c, int ldc);
+/* S = U^T U and X = U^{-1} (upper; zero below the diagonal). s (full storage) is destroyed. */
+int xtbeig_dev_potrf_inverse(xtbeig_handle_t handle, int n, double *s, int lds, double *u, int ldu, double *x,
+                             int ldx);
+/* Dense -> band (reference: Stages/BandForm.swift:28-142): a (full symmetric, destroyed) -> lower band of width b
+ * in band storage ab (element (i, j), 0 <= i - j < 2b, at i + (2b - 1) j; xtbeig_band_elems(n, b) elements), the
+ * panels' Householder vectors v1 (n x n) and T factors t1 (b x n). */
+long long xtbeig_band_elems(int n, int b);
+int xtbeig_dev_dreduce_to_band(xtbeig_handle_t handle, int n, int b, double *a, int lda, double *v1, double *t1,
+                               double *ab);
+int xtbeig_dev_sreduce_to_band(xtbeig_handle_t handle, int n, int b, float *a, int lda, float *v1, float *t1,
+                               float *ab);
+/* Band -> tridiagonal by bulge chasing (reference: Stages/BulgeChasing.swift:123-321): ab (destroyed) -> d, e;
+ * reflectors w (H = I - w w^T) in v2 (n x n, sweep s in column s). */
+int xtbeig_dev_dchase(xtbeig_handle_t handle, int n, int b, double *ab, double *v2, double *d, double *e);
+int xtbeig_dev_schase(xtbeig_handle_t handle, int n, int b, float *ab, float *v2, float *d, float *e);
+/* z = Q2 z (bulge reflectors) and z = Q1 z (band panels) for m columns (reference:
+ * Stages/BulgeBackTransformation.swift:11-30, Stages/BandFormBackTransformation.swift:10-21). */
+int xtbeig_dev_dback_bulges(xtbeig_handle_t handle, int n, int b, const double *v2, double *z, int ldz, int m);
+int xtbeig_dev_sback_bulges(xtbeig_handle_t handle, int n, int b, const float *v2, float *z, int ldz, int m);
+int xtbeig_dev_dback_band(xtbeig_handle_t handle, int n, int b, const double *v1, const double *t1, double *z,
+                          int ldz, int m);
+int xtbeig_dev_sback_band(xtbeig_handle_t handle, int n, int b, const float *v1, const float *t1, float *z, int ldz,
+                          int m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XTBEIG_H */

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every function include/xtbeig.h declares (no GPU needed)."""
+import os
+import re
+
+import xtbeig
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "xtbeig.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(xtb[a-z0-9_]*)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_the_reference_symbol_set():
+    names = declared()
+    for s in ["xtb_dstedc", "xtb_sstedc", "xtb_dsymmetrize_pad", "xtbeig_dsyevd", "xtbeig_dsygvd"]:
+        assert s in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = xtbeig.lib()
+    missing = [n for n in declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_the_header():
+    assert set(declared()) <= set(xtbeig.SIGNATURES)
+
+
+def test_workspace_queries_without_gpu():
+    """lwork = -1 queries are answered on the host (no device touched)."""
+    import ctypes
+    import numpy as np
+    n = 100
+    work = np.zeros(1)
+    iwork = np.zeros(1, dtype=np.int32)
+    info = ctypes.c_int(7)
+    xtbeig.lib().xtbeig_dsygvd(xtbeig._i(1), b"V", b"U", xtbeig._i(n), None, xtbeig._i(n), None, xtbeig._i(n), None,
+                               xtbeig._ptr(work), xtbeig._i(-1), xtbeig._ptr(iwork), xtbeig._i(-1), ctypes.byref(info))
+    assert info.value == 0 and work[0] == 1 + 6 * n + 2 * n * n and iwork[0] == 3 + 5 * n
+    xtbeig.lib().xtb_dstedc(xtbeig._i(n), None, None, None, xtbeig._i(n), xtbeig._ptr(work), xtbeig._i(-1),
+                            xtbeig._ptr(iwork), xtbeig._i(-1), ctypes.byref(info))
+    assert info.value == 0 and work[0] == 1 + 4 * n + n * n and iwork[0] == 3 + 5 * n
+    # illegal argument: itype 2 is rejected with info = -1
+    xtbeig.lib().xtbeig_dsygvd(xtbeig._i(2), b"V", b"U", xtbeig._i(n), None, xtbeig._i(n), None, xtbeig._i(n), None,
+                               xtbeig._ptr(work), xtbeig._i(-1), xtbeig._ptr(iwork), xtbeig._i(-1), ctypes.byref(info))
+    assert info.value == -1

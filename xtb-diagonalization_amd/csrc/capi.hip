@@ -1,0 +1,433 @@
+// The C ABI (include/xtbeig.h). Host-memory entry points run on a per-thread handle (no process-global mutable state:
+// two threads of one process never share a handle or workspace); device-memory entry points on the caller's handle.
+// Every entry point catches internal errors and reports them (return code, LAPACK info, xtbeig_last_error).
+#include "../../include/xtbeig.h"
+#include "solver.hpp"
+
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+using xtbeig::Handle;
+
+struct xtbeig_handle {
+  Handle h;
+  explicit xtbeig_handle(int dev) : h(dev) {}
+};
+
+namespace {
+
+thread_local std::string tl_error;
+thread_local std::unique_ptr<xtbeig_handle> tl_handle;
+
+int set_error(const char* what) {
+  tl_error = what;
+  return 1;
+}
+
+Handle& thread_handle() {
+  if (!tl_handle) {
+    int dev = 0;
+    if (const char* e = std::getenv("XTBEIG_DEVICE")) dev = std::atoi(e);
+    tl_handle.reset(new xtbeig_handle(dev));
+  }
+  return tl_handle->h;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    return set_error(e.what());
+  } catch (...) {
+    return set_error("unknown error");
+  }
+}
+
+// Device status words -> host, cleared.
+void read_status(Handle& h, int* st) {
+  XTB_HIP(hipStreamSynchronize(h.stream));
+  XTB_HIP(hipMemcpy(st, h.dinfo, 3 * sizeof(int), hipMemcpyDeviceToHost));
+  XTB_HIP(hipMemsetAsync(h.dinfo, 0, 3 * sizeof(int), h.stream));
+  XTB_HIP(hipStreamSynchronize(h.stream));
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  explicit DevBuf(size_t bytes) { XTB_HIP(hipMalloc(&p, bytes > 0 ? bytes : 1)); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  template <typename T> T* as() { return static_cast<T*>(p); }
+};
+
+void h2d(Handle& h, void* dst, const void* src, size_t bytes) {
+  XTB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h.stream));
+}
+void d2h(Handle& h, void* dst, const void* src, size_t bytes) {
+  XTB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h.stream));
+}
+// column-major sub-matrix copies with leading dimensions
+template <typename T>
+void h2d_mat(Handle& h, T* dst, int ldd, const T* src, int lds, int m, int n) {
+  XTB_HIP(hipMemcpy2DAsync(dst, sizeof(T) * ldd, src, sizeof(T) * lds, sizeof(T) * m, n, hipMemcpyHostToDevice,
+                           h.stream));
+}
+template <typename T>
+void d2h_mat(Handle& h, T* dst, int ldd, const T* src, int lds, int m, int n) {
+  XTB_HIP(hipMemcpy2DAsync(dst, sizeof(T) * ldd, src, sizeof(T) * lds, sizeof(T) * m, n, hipMemcpyDeviceToHost,
+                           h.stream));
+}
+
+bool is_char(const char* c, char u) { return c && (*c == u || *c == (char)(u + 32)); }
+
+}  // namespace
+
+template <typename T>
+int reduce_to_band_impl(xtbeig_handle_t handle, int n, int b, T* a, int lda, T* v1, T* t1, T* ab) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve(xtbeig::band_workspace<T>(n, b) * sizeof(T) + 4096);
+    xtbeig::reduce_to_band<T>(h.stream, n, b, a, lda, v1, t1, ab, static_cast<T*>(h.arena));
+  });
+}
+
+template <typename T>
+int back_band_impl(xtbeig_handle_t handle, int n, int b, const T* v1, const T* t1, T* z, int ldz, int m) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve(2 * (size_t)b * m * sizeof(T) + 4096);
+    xtbeig::back_transform_band<T>(h.stream, n, b, v1, t1, z, ldz, m, static_cast<T*>(h.arena));
+  });
+}
+
+template <typename T>
+void syevd_host(const char* jobz, const char* uplo, const int* n_, T* a, const int* lda_, T* w, T* work,
+                       const int* lwork, int* iwork, const int* liwork, int* info) {
+  const int n = *n_, lda = *lda_;
+  const bool wantz = is_char(jobz, 'V');
+  const long long lmin = n <= 1 ? 1 : (wantz ? 1 + 6LL * n + 2LL * n * n : 2LL * n + 1);
+  const long long limin = (n <= 1 || !wantz) ? 1 : 3 + 5LL * n;
+  *info = 0;
+  if (!wantz && !is_char(jobz, 'N')) *info = -1;
+  else if (!is_char(uplo, 'U') && !is_char(uplo, 'L')) *info = -2;
+  else if (n < 0) *info = -3;
+  else if (lda < (n > 1 ? n : 1)) *info = -5;
+  if (*info != 0) return;
+  if (*lwork == -1 || *liwork == -1) {
+    work[0] = (T)lmin;
+    iwork[0] = (int)limin;
+    return;
+  }
+  if (*lwork < lmin) { *info = -8; return; }
+  if (*liwork < limin) { *info = -10; return; }
+  if (n == 0) return;
+  int st[3] = {0, 0, 0};
+  const int rc = guarded([&] {
+    Handle& h = thread_handle();
+    DevBuf da(sizeof(T) * (size_t)n * n), dz(sizeof(T) * (size_t)n * n), dw(sizeof(double) * n),
+        dwt(sizeof(T) * n);
+    h2d_mat<T>(h, da.as<T>(), n, a, lda, n, n);
+    xtbeig::symmetrize<T>(h.stream, is_char(uplo, 'U') ? 'U' : 'L', n, da.as<T>(), n);
+    xtbeig::syevd_device<T>(h, n, da.as<T>(), n, dw.as<double>(), dz.as<T>(), n, h.band_override);
+    xtbeig::copy_matrix<double, T>(h.stream, n, 1, dw.as<double>(), n, dwt.as<T>(), n);
+    d2h(h, w, dwt.as<T>(), sizeof(T) * n);
+    if (wantz) d2h_mat<T>(h, a, lda, dz.as<T>(), n, n, n);
+    read_status(h, st);
+  });
+  if (rc) *info = n + 1;
+  else if (st[1] || st[2]) *info = st[2] ? st[2] : n + 1;
+}
+
+extern "C" {
+
+const char* xtbeig_last_error(void) { return tl_error.c_str(); }
+
+int xtbeig_create(xtbeig_handle_t* handle, int device) {
+  return guarded([&] { *handle = new xtbeig_handle(device); });
+}
+
+int xtbeig_destroy(xtbeig_handle_t handle) {
+  return guarded([&] { delete handle; });
+}
+
+int xtbeig_set_stream(xtbeig_handle_t handle, void* stream) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    XTB_HIP(hipStreamSynchronize(h.stream));
+    if (h.own_stream) XTB_HIP(hipStreamDestroy(h.stream));
+    h.own_stream = false;
+    h.stream = static_cast<hipStream_t>(stream);
+  });
+}
+
+void* xtbeig_get_stream(xtbeig_handle_t handle) { return handle ? (void*)handle->h.stream : nullptr; }
+
+int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations) {
+  if (band < 0 || band > 64 || refine_iterations < 0 || refine_iterations > 8)
+    return set_error("xtbeig_set_tuning: band must be in [0, 64] and refine_iterations in [0, 8]");
+  handle->h.band_override = band;
+  handle->h.refine_iters = refine_iterations;
+  return 0;
+}
+
+int xtbeig_synchronize(xtbeig_handle_t handle, int* status) {
+  int st[3] = {0, 0, 0};
+  int rc = guarded([&] { read_status(handle->h, st); });
+  if (rc) return rc;
+  if (status) { status[0] = st[0]; status[1] = st[1]; status[2] = st[2]; }
+  return (st[0] || st[1] || st[2]) ? 2 : 0;
+}
+
+int xtbeig_dev_sygvd(xtbeig_handle_t handle, int precision, int n, const double* hm, int ldh, const double* sm,
+                     int lds, double* w, double* c, int ldc) {
+  if (n < 0 || ldh < (n > 1 ? n : 1) || lds < (n > 1 ? n : 1) || ldc < (n > 1 ? n : 1))
+    return set_error("xtbeig_dev_sygvd: bad dimensions");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_dev_sygvd: unknown precision");
+  return guarded([&] {
+    Handle& h = handle->h;
+    xtbeig::sygvd_device(h, n, hm, ldh, sm, lds, w, c, ldc, precision == XTBEIG_FP32_REFINE, h.band_override);
+  });
+}
+
+int xtbeig_dev_dsyevd(xtbeig_handle_t handle, int n, const double* a, int lda, double* w, double* z, int ldz) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    xtbeig::syevd_device<double>(h, n, a, lda, w, z, ldz, h.band_override);
+  });
+}
+
+int xtbeig_dev_ssyevd(xtbeig_handle_t handle, int n, const float* a, int lda, double* w, float* z, int ldz) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    xtbeig::syevd_device<float>(h, n, a, lda, w, z, ldz, h.band_override);
+  });
+}
+
+int xtbeig_dev_dstedc(xtbeig_handle_t handle, int n, double* d, double* e, double* z, int ldz) {
+  return guarded([&] { xtbeig::stedc_device(handle->h, n, d, e, z, ldz); });
+}
+
+int xtbeig_dev_dgemm(xtbeig_handle_t handle, char ta, char tb, int m, int n, int k, double alpha, const double* a,
+                     int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  return guarded([&] { xtbeig::gemm<double>(handle->h.stream, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
+}
+
+int xtbeig_dev_sgemm(xtbeig_handle_t handle, char ta, char tb, int m, int n, int k, float alpha, const float* a,
+                     int lda, const float* b, int ldb, float beta, float* c, int ldc) {
+  return guarded([&] { xtbeig::gemm<float>(handle->h.stream, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc); });
+}
+
+int xtbeig_dev_potrf_inverse(xtbeig_handle_t handle, int n, double* s, int lds, double* u, int ldu, double* x,
+                             int ldx) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve((size_t)n * 64 * sizeof(double) + 4096);
+    xtbeig::potrf_inverse(h.stream, n, s, lds, u, ldu, x, ldx, static_cast<double*>(h.arena), h.dinfo);
+  });
+}
+
+long long xtbeig_band_elems(int n, int b) { return (long long)xtbeig::band_elems(n, b); }
+
+int xtbeig_dev_dreduce_to_band(xtbeig_handle_t handle, int n, int b, double* a, int lda, double* v1, double* t1,
+                               double* ab) {
+  return reduce_to_band_impl<double>(handle, n, b, a, lda, v1, t1, ab);
+}
+int xtbeig_dev_sreduce_to_band(xtbeig_handle_t handle, int n, int b, float* a, int lda, float* v1, float* t1,
+                               float* ab) {
+  return reduce_to_band_impl<float>(handle, n, b, a, lda, v1, t1, ab);
+}
+
+int xtbeig_dev_dchase(xtbeig_handle_t handle, int n, int b, double* ab, double* v2, double* d, double* e) {
+  return guarded([&] { xtbeig::chase_bulges<double>(handle->h.stream, n, b, ab, v2, d, e, handle->h.dinfo + 1); });
+}
+int xtbeig_dev_schase(xtbeig_handle_t handle, int n, int b, float* ab, float* v2, float* d, float* e) {
+  return guarded([&] { xtbeig::chase_bulges<float>(handle->h.stream, n, b, ab, v2, d, e, handle->h.dinfo + 1); });
+}
+int xtbeig_dev_dback_bulges(xtbeig_handle_t handle, int n, int b, const double* v2, double* z, int ldz, int m) {
+  return guarded([&] { xtbeig::back_transform_bulges<double>(handle->h.stream, n, b, v2, z, ldz, m); });
+}
+int xtbeig_dev_sback_bulges(xtbeig_handle_t handle, int n, int b, const float* v2, float* z, int ldz, int m) {
+  return guarded([&] { xtbeig::back_transform_bulges<float>(handle->h.stream, n, b, v2, z, ldz, m); });
+}
+int xtbeig_dev_dback_band(xtbeig_handle_t handle, int n, int b, const double* v1, const double* t1, double* z,
+                          int ldz, int m) {
+  return back_band_impl<double>(handle, n, b, v1, t1, z, ldz, m);
+}
+int xtbeig_dev_sback_band(xtbeig_handle_t handle, int n, int b, const float* v1, const float* t1, float* z, int ldz,
+                          int m) {
+  return back_band_impl<float>(handle, n, b, v1, t1, z, ldz, m);
+}
+
+// ---- host-memory entry points -----------------------------------------------------------------------------------
+
+void xtb_dstedc(const int* n_, double* d, double* e, double* z, const int* ldz_, double* work, const int* lwork,
+                int* iwork, const int* liwork, int* info) {
+  const int n = *n_, ldz = *ldz_;
+  const long long lmin = n <= 1 ? 1 : 1 + 4LL * n + (long long)n * n, limin = n <= 1 ? 1 : 3 + 5LL * n;
+  if (*lwork == -1 || *liwork == -1) {
+    work[0] = (double)lmin;
+    iwork[0] = (int)limin;
+    *info = 0;
+    return;
+  }
+  if (n < 0) { *info = -2; return; }
+  if (ldz < (n > 1 ? n : 1)) { *info = -5; return; }
+  if (*lwork < lmin) { *info = -7; return; }
+  if (*liwork < limin) { *info = -9; return; }
+  *info = 0;
+  if (n == 0) return;
+  int st[3] = {0, 0, 0};
+  const int rc = guarded([&] {
+    Handle& h = thread_handle();
+    DevBuf dd(sizeof(double) * n), de(sizeof(double) * n), dz(sizeof(double) * (size_t)n * n);
+    h2d(h, dd.as<double>(), d, sizeof(double) * n);
+    if (n > 1) h2d(h, de.as<double>(), e, sizeof(double) * (n - 1));
+    xtbeig::stedc_device(h, n, dd.as<double>(), de.as<double>(), dz.as<double>(), n);
+    d2h(h, d, dd.as<double>(), sizeof(double) * n);
+    d2h_mat<double>(h, z, ldz, dz.as<double>(), n, n, n);
+    read_status(h, st);
+  });
+  if (rc) *info = n + 1;
+  else if (st[2]) *info = st[2];
+}
+
+void xtb_sstedc(const int* n_, float* d, float* e, float* z, const int* ldz_, float* work, const int* lwork,
+                int* iwork, const int* liwork, int* info) {
+  const int n = *n_, ldz = *ldz_;
+  const long long lmin = n <= 1 ? 1 : 1 + 4LL * n + (long long)n * n, limin = n <= 1 ? 1 : 3 + 5LL * n;
+  if (*lwork == -1 || *liwork == -1) {
+    work[0] = (float)lmin;
+    iwork[0] = (int)limin;
+    *info = 0;
+    return;
+  }
+  if (n < 0) { *info = -2; return; }
+  if (ldz < (n > 1 ? n : 1)) { *info = -5; return; }
+  if (*lwork < lmin) { *info = -7; return; }
+  if (*liwork < limin) { *info = -9; return; }
+  *info = 0;
+  if (n == 0) return;
+  int st[3] = {0, 0, 0};
+  const int rc = guarded([&] {
+    Handle& h = thread_handle();
+    DevBuf df(sizeof(float) * n * 2), dd(sizeof(double) * n), de(sizeof(double) * n),
+        dz(sizeof(double) * (size_t)n * n), dzf(sizeof(float) * (size_t)n * n);
+    float* fd = df.as<float>();
+    float* fe = fd + n;
+    h2d(h, fd, d, sizeof(float) * n);
+    if (n > 1) h2d(h, fe, e, sizeof(float) * (n - 1));
+    xtbeig::copy_matrix<float, double>(h.stream, n, 1, fd, n, dd.as<double>(), n);
+    if (n > 1) xtbeig::copy_matrix<float, double>(h.stream, n - 1, 1, fe, n, de.as<double>(), n);
+    xtbeig::stedc_device(h, n, dd.as<double>(), de.as<double>(), dz.as<double>(), n);
+    xtbeig::copy_matrix<double, float>(h.stream, n, 1, dd.as<double>(), n, fd, n);
+    xtbeig::copy_matrix<double, float>(h.stream, n, n, dz.as<double>(), n, dzf.as<float>(), n);
+    d2h(h, d, fd, sizeof(float) * n);
+    d2h_mat<float>(h, z, ldz, dzf.as<float>(), n, n, n);
+    read_status(h, st);
+  });
+  if (rc) *info = n + 1;
+  else if (st[2]) *info = st[2];
+}
+
+void xtb_dsymmetrize_pad(const int* n_, const double* src, const int* lds_, double* dst, const int* ldd_) {
+  const int n = *n_, lds = *lds_, ldd = *ldd_;
+  if (n <= 0) return;
+  guarded([&] {
+    Handle& h = thread_handle();
+    DevBuf ds(sizeof(double) * (size_t)n * n), dt(sizeof(double) * (size_t)n * n);
+    h2d_mat<double>(h, ds.as<double>(), n, src, lds, n, n);
+    xtbeig::symmetrize_pad(h.stream, n, ds.as<double>(), n, dt.as<double>(), n);
+    d2h_mat<double>(h, dst, ldd, dt.as<double>(), n, n, n);
+    XTB_HIP(hipStreamSynchronize(h.stream));
+  });
+}
+
+
+void xtbeig_dsyevd(const char* jobz, const char* uplo, const int* n, double* a, const int* lda, double* w,
+                   double* work, const int* lwork, int* iwork, const int* liwork, int* info) {
+  syevd_host<double>(jobz, uplo, n, a, lda, w, work, lwork, iwork, liwork, info);
+}
+void xtbeig_ssyevd(const char* jobz, const char* uplo, const int* n, float* a, const int* lda, float* w, float* work,
+                   const int* lwork, int* iwork, const int* liwork, int* info) {
+  syevd_host<float>(jobz, uplo, n, a, lda, w, work, lwork, iwork, liwork, info);
+}
+
+static void sygvd_host(bool mixed, const int* itype, const char* jobz, const char* uplo, const int* n_, double* a,
+                       const int* lda_, double* b, const int* ldb_, double* w, double* work, const int* lwork,
+                       int* iwork, const int* liwork, int* info) {
+  const int n = *n_, lda = *lda_, ldb = *ldb_;
+  const bool wantz = is_char(jobz, 'V');
+  const long long lmin = n <= 1 ? 1 : (wantz ? 1 + 6LL * n + 2LL * n * n : 2LL * n + 1);
+  const long long limin = (n <= 1 || !wantz) ? 1 : 3 + 5LL * n;
+  *info = 0;
+  if (*itype != 1) *info = -1;  // itype 2 / 3 (A B x, B A x) are not part of xTB's path
+  else if (!wantz && !is_char(jobz, 'N')) *info = -2;
+  else if (!is_char(uplo, 'U') && !is_char(uplo, 'L')) *info = -3;
+  else if (n < 0) *info = -4;
+  else if (lda < (n > 1 ? n : 1)) *info = -6;
+  else if (ldb < (n > 1 ? n : 1)) *info = -8;
+  if (*info != 0) return;
+  if (*lwork == -1 || *liwork == -1) {
+    work[0] = (double)lmin;
+    iwork[0] = (int)limin;
+    return;
+  }
+  if (*lwork < lmin) { *info = -11; return; }
+  if (*liwork < limin) { *info = -13; return; }
+  if (n == 0) return;
+  int st[3] = {0, 0, 0};
+  const int rc = guarded([&] {
+    Handle& h = thread_handle();
+    const size_t nn = (size_t)n * n;
+    DevBuf dh(sizeof(double) * nn), ds(sizeof(double) * nn), dc(sizeof(double) * nn), dw(sizeof(double) * n),
+        du(sizeof(double) * nn), dx(sizeof(double) * nn), ds2(sizeof(double) * nn);
+    const char ul = is_char(uplo, 'U') ? 'U' : 'L';
+    h2d_mat<double>(h, dh.as<double>(), n, a, lda, n, n);
+    h2d_mat<double>(h, ds.as<double>(), n, b, ldb, n, n);
+    xtbeig::symmetrize<double>(h.stream, ul, n, dh.as<double>(), n);
+    xtbeig::symmetrize<double>(h.stream, ul, n, ds.as<double>(), n);
+    xtbeig::sygvd_device(h, n, dh.as<double>(), n, ds.as<double>(), n, dw.as<double>(), dc.as<double>(), n, mixed,
+                         h.band_override);
+    d2h(h, w, dw.as<double>(), sizeof(double) * n);
+    if (wantz) d2h_mat<double>(h, a, lda, dc.as<double>(), n, n, n);
+    read_status(h, st);
+    if (st[0] == 0) {
+      // b on exit: the Cholesky factor in the referenced triangle, as LAPACK's dsygvd leaves it
+      xtbeig::copy_matrix<double, double>(h.stream, n, n, ds.as<double>(), n, ds2.as<double>(), n);
+      XTB_HIP(hipStreamSynchronize(h.stream));
+      h.reserve((size_t)n * 64 * sizeof(double) + 4096);
+      xtbeig::potrf_inverse(h.stream, n, ds2.as<double>(), n, du.as<double>(), n, dx.as<double>(), n,
+                            static_cast<double*>(h.arena), h.dinfo);
+      if (ul == 'U') {
+        d2h_mat<double>(h, b, ldb, du.as<double>(), n, n, n);
+      } else {
+        // L = U^T into the lower triangle
+        xtbeig::copy_matrix<double, double>(h.stream, n, n, du.as<double>(), n, dx.as<double>(), n);
+        XTB_HIP(hipStreamSynchronize(h.stream));
+        xtbeig::symmetrize<double>(h.stream, 'U', n, dx.as<double>(), n);
+        d2h_mat<double>(h, b, ldb, dx.as<double>(), n, n, n);
+      }
+      int st2[3];
+      read_status(h, st2);
+    }
+  });
+  if (rc) *info = 2 * n + 1;
+  else if (st[0]) *info = n + st[0];
+  else if (st[1] || st[2]) *info = st[2] ? st[2] : n;
+}
+
+void xtbeig_dsygvd(const int* itype, const char* jobz, const char* uplo, const int* n, double* a, const int* lda,
+                   double* b, const int* ldb, double* w, double* work, const int* lwork, int* iwork,
+                   const int* liwork, int* info) {
+  sygvd_host(false, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
+}
+
+void xtbeig_dsygvd_mixed(const int* itype, const char* jobz, const char* uplo, const int* n, double* a,
+                         const int* lda, double* b, const int* ldb, double* w, double* work, const int* lwork,
+                         int* iwork, const int* liwork, int* info) {
+  sygvd_host(true, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
+}
+
+}  // extern "C"

@@ -1,0 +1,566 @@
+// Divide and conquer for the symmetric tridiagonal eigenproblem, FP64, all eigenpairs (dstedc('I') semantics:
+// eigenvalues ascending, eigenvectors of T). The reference tears at n1 = n / 2 (a rank-one change, as dlaed0), solves
+// both halves and merges them as dlaed1: deflation by LAPACK's dlaed2, a port of dlaed3 (one dlaed4 per root,
+// Gu-Eisenstat z, merge GEMMs), dlamrg and a column sort (Stages/ParallelDivideAndConquer.swift:14-211).
+//
+// Here the same tear is applied recursively down to 1 x 1 leaves, so that every subproblem is a merge and each level
+// of the tree is one batch of GPU launches over all its merges:
+//   deflate_kernel   one workgroup per merge: z from the halves' boundary rows, dlaed2 (sort, small-z and close-pole
+//                    deflation with Givens rotations, the four column groups, Q2) staged in LDS
+//   secular_kernel   one lane per secular-equation root (dlaed4 / dlaed5), outputs (origin pole, shift)
+//   gu_kernel        Gu-Eisenstat z_i from the products over all roots (dlaed3's order)
+//   vectors_kernel   one wavefront per root: the normalized eigenvector of the rank-one problem, rows permuted
+//   merge_gemm       Q(0:n1, 0:K) = Q2a S(0:n12), Q(n1:n, 0:K) = Q2b S(c0:c0+n23), MFMA FP64 with device-side sizes
+//   order_kernel     dlamrg of the K new and n - K deflated eigenvalues, then gather / copy-back of the columns
+// Deflation bookkeeping (K, the index arrays, column types) follows dlaed2 step for step and is compared with the host
+// LAPACK's dlaed2 on the same inputs by the tests.
+#include "common.hpp"
+#include "gemm_tile.hpp"
+#include "kernels.hpp"
+#include "secular.hpp"
+
+#include <vector>
+
+namespace xtbeig {
+
+namespace {
+
+constexpr double EPS = 1.1102230246251565e-16;
+
+using Node = DcNode;
+// per merge results: K, ctot[4]
+struct NodeOut {
+  int k, ctot[4], pad[3];
+};
+
+__device__ inline double lapy2(double x, double y) {
+  const double xa = fabs(x), ya = fabs(y);
+  const double w = fmax(xa, ya), z = fmin(xa, ya);
+  if (z == 0 || w > 1.79e308) return w;
+  const double r = z / w;
+  return w * sqrt(1 + r * r);
+}
+
+// dlamrg: index (1-based values) merging a[0:n1] (stride s1) with a[n1:n1+n2] (stride s2) into ascending order.
+__device__ inline void lamrg(int n1, int n2, const double* a, int s1, int s2, int* index) {
+  int n1sv = n1, n2sv = n2;
+  int ind1 = s1 > 0 ? 1 : n1;
+  int ind2 = s2 > 0 ? 1 + n1 : n1 + n2;
+  int i = 0;
+  while (n1sv > 0 && n2sv > 0) {
+    if (a[ind1 - 1] <= a[ind2 - 1]) { index[i++] = ind1; ind1 += s1; --n1sv; }
+    else { index[i++] = ind2; ind2 += s2; --n2sv; }
+  }
+  if (n1sv == 0) { for (int q = 0; q < n2sv; ++q) { index[i++] = ind2; ind2 += s2; } }
+  else { for (int q = 0; q < n1sv; ++q) { index[i++] = ind1; ind1 += s1; } }
+}
+
+// Scale by the max norm, apply every tear (pre-order, parent before children), Q = I.
+__global__ void prepare_kernel(int n, double* d, double* e, const Node* nodes_pre, int nnodes, double* rho,
+                               double* scale) {
+  __shared__ double red[16];
+  const int t = threadIdx.x;
+  double m = 0;
+  for (int i = t; i < n; i += blockDim.x) m = fmax(m, fabs(d[i]));
+  for (int i = t; i < n - 1; i += blockDim.x) m = fmax(m, fabs(e[i]));
+  m = wave_max(m);
+  if ((t & 63) == 0) red[t >> 6] = m;
+  __syncthreads();
+  double nrm = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) nrm = fmax(nrm, red[w]);
+  if (nrm == 0) nrm = 1;
+  if (t == 0) *scale = nrm;
+  for (int i = t; i < n; i += blockDim.x) d[i] /= nrm;
+  for (int i = t; i < n - 1; i += blockDim.x) e[i] /= nrm;
+  __syncthreads();
+  if (t == 0) {
+    for (int p = 0; p < nnodes; ++p) {
+      const Node nd = nodes_pre[p];
+      const int m1 = nd.off + nd.n1;
+      const double r = e[m1 - 1];
+      rho[p] = r;
+      d[m1 - 1] -= fabs(r);
+      d[m1] -= fabs(r);
+    }
+  }
+}
+
+__global__ void unscale_kernel(int n, double* d, const double* scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] *= *scale;
+}
+
+// dlaed2 for one merge (one workgroup). LDS: d, z, dlamda, w (doubles) and indxq, indx, indxc, indxp, coltyp (ints).
+__global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const int* node_pre_index, int n_all,
+                                                      double* __restrict__ d_all, double* __restrict__ q_all, int ldq,
+                                                      const double* __restrict__ rho_pre, double* __restrict__ rho_lvl,
+                                                      double* __restrict__ dlamda_all, double* __restrict__ w_all,
+                                                      double* __restrict__ q2_all, int* __restrict__ indxc_all,
+                                                      NodeOut* __restrict__ out) {
+  extern __shared__ char smem[];
+  const Node nd = nodes[blockIdx.x];
+  const int n = nd.sz, n1 = nd.n1, n2 = n - n1, off = nd.off;
+  double* D = reinterpret_cast<double*>(smem);
+  double* Z = D + n;
+  double* DL = Z + n;
+  double* W = DL + n;
+  int* INDXQ = reinterpret_cast<int*>(W + n);
+  int* INDX = INDXQ + n;
+  int* INDXC = INDX + n;
+  int* INDXP = INDXC + n;
+  int* COLTYP = INDXP + n;
+  int* ROT_P = COLTYP + n;  // Givens rotations recorded by the scan (at most n)
+  int* ROT_N = ROT_P + n;
+  double* ROT_C = reinterpret_cast<double*>(ROT_N + n + (n & 1));
+  double* ROT_S = ROT_C + n;
+  __shared__ int sh_k, sh_nrot, sh_ctot[4], sh_early;
+  __shared__ double sh_rho;
+  const int t = threadIdx.x;
+  double* q = q_all + idx(off, off, ldq);
+  double* q2 = q2_all + nd.q2off;
+  for (int i = t; i < n; i += blockDim.x) {
+    D[i] = d_all[off + i];
+    Z[i] = (i < n1) ? q[idx(n1 - 1, i, ldq)] : q[idx(n1, i, ldq)];
+    INDXQ[i] = (i < n1) ? i + 1 : i - n1 + 1;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double rho = rho_pre[node_pre_index[blockIdx.x]];
+    if (rho < 0) for (int i = n1; i < n; ++i) Z[i] = -Z[i];
+    const double tt = 1.0 / sqrt(2.0);
+    for (int i = 0; i < n; ++i) Z[i] *= tt;
+    rho = fabs(2 * rho);
+    for (int i = n1; i < n; ++i) INDXQ[i] += n1;
+    for (int i = 0; i < n; ++i) DL[i] = D[INDXQ[i] - 1];
+    lamrg(n1, n2, DL, 1, 1, INDXC);
+    for (int i = 0; i < n; ++i) INDX[i] = INDXQ[INDXC[i] - 1];
+    int imax = 0, jmax = 0;
+    for (int i = 1; i < n; ++i) {
+      if (fabs(Z[i]) > fabs(Z[imax])) imax = i;
+      if (fabs(D[i]) > fabs(D[jmax])) jmax = i;
+    }
+    const double tol = 8 * EPS * fmax(fabs(D[jmax]), fabs(Z[imax]));
+    int k = 0, nrot = 0;
+    sh_early = 0;
+    if (rho * fabs(Z[imax]) <= tol) {
+      sh_early = 1;
+    } else {
+      for (int i = 0; i < n1; ++i) COLTYP[i] = 1;
+      for (int i = n1; i < n; ++i) COLTYP[i] = 3;
+      int k2 = n + 1, j = 1, pj = 0;
+      // first non-deflated column (one exists: z[imax] is not small)
+      for (; j <= n; ++j) {
+        const int nj = INDX[j - 1];
+        if (rho * fabs(Z[nj - 1]) <= tol) {
+          --k2; COLTYP[nj - 1] = 4; INDXP[k2 - 1] = nj;
+        } else {
+          pj = nj;
+          break;
+        }
+      }
+      if (pj == 0) pj = INDX[n - 1];  // unreachable: z[imax] is not small
+      for (++j; j <= n; ++j) {
+        const int nj = INDX[j - 1];
+        if (rho * fabs(Z[nj - 1]) <= tol) {
+          --k2; COLTYP[nj - 1] = 4; INDXP[k2 - 1] = nj;
+          continue;
+        }
+        double s = Z[pj - 1], c = Z[nj - 1];
+        const double tau = lapy2(c, s);
+        double tdiff = D[nj - 1] - D[pj - 1];
+        c = c / tau;
+        s = -s / tau;
+        if (fabs(tdiff * c * s) <= tol) {
+          Z[nj - 1] = tau;
+          Z[pj - 1] = 0;
+          if (COLTYP[nj - 1] != COLTYP[pj - 1]) COLTYP[nj - 1] = 2;
+          COLTYP[pj - 1] = 4;
+          ROT_P[nrot] = pj - 1; ROT_N[nrot] = nj - 1; ROT_C[nrot] = c; ROT_S[nrot] = s; ++nrot;
+          const double tp = D[pj - 1] * c * c + D[nj - 1] * s * s;
+          D[nj - 1] = D[pj - 1] * s * s + D[nj - 1] * c * c;
+          D[pj - 1] = tp;
+          --k2;
+          int i = 1;
+          while (true) {
+            if (k2 + i <= n) {
+              if (D[pj - 1] < D[INDXP[k2 + i - 1] - 1]) {
+                INDXP[k2 + i - 2] = INDXP[k2 + i - 1];
+                INDXP[k2 + i - 1] = pj;
+                ++i;
+                continue;
+              }
+              INDXP[k2 + i - 2] = pj;
+            } else {
+              INDXP[k2 + i - 2] = pj;
+            }
+            break;
+          }
+          pj = nj;
+        } else {
+          ++k;
+          DL[k - 1] = D[pj - 1];
+          W[k - 1] = Z[pj - 1];
+          INDXP[k - 1] = pj;
+          pj = nj;
+        }
+      }
+      ++k;
+      DL[k - 1] = D[pj - 1];
+      W[k - 1] = Z[pj - 1];
+      INDXP[k - 1] = pj;
+      int ctot[4] = {0, 0, 0, 0};
+      for (int jj = 0; jj < n; ++jj) ctot[COLTYP[jj] - 1]++;
+      int psm[4];
+      psm[0] = 1; psm[1] = 1 + ctot[0]; psm[2] = psm[1] + ctot[1]; psm[3] = psm[2] + ctot[2];
+      k = n - ctot[3];
+      for (int jj = 1; jj <= n; ++jj) {
+        const int js = INDXP[jj - 1], ct = COLTYP[js - 1];
+        INDX[psm[ct - 1] - 1] = js;
+        INDXC[psm[ct - 1] - 1] = jj;
+        psm[ct - 1]++;
+      }
+      for (int c = 0; c < 4; ++c) sh_ctot[c] = ctot[c];
+    }
+    sh_k = k;
+    sh_nrot = nrot;
+    sh_rho = rho;
+  }
+  __syncthreads();
+  if (sh_early) {
+    // K = 0: Q columns in sorted order, D sorted
+    for (int j = 0; j < n; ++j) {
+      const int src = INDX[j] - 1;
+      for (int i = t; i < n; i += blockDim.x) q2[idx(i, j, n)] = q[idx(i, src, ldq)];
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j)
+      for (int i = t; i < n; i += blockDim.x) q[idx(i, j, ldq)] = q2[idx(i, j, n)];
+    for (int j = t; j < n; j += blockDim.x) d_all[off + j] = D[INDX[j] - 1];
+    if (t == 0) {
+      out[blockIdx.x].k = 0;
+      for (int c = 0; c < 4; ++c) out[blockIdx.x].ctot[c] = 0;
+      rho_lvl[blockIdx.x] = sh_rho;
+    }
+    return;
+  }
+  // Givens rotations on Q's columns, in order (each thread owns its rows)
+  for (int r = 0; r < sh_nrot; ++r) {
+    const int cp = ROT_P[r], cn = ROT_N[r];
+    const double c = ROT_C[r], s = ROT_S[r];
+    for (int i = t; i < n; i += blockDim.x) {
+      const double x = q[idx(i, cp, ldq)], y = q[idx(i, cn, ldq)];
+      q[idx(i, cp, ldq)] = c * x + s * y;
+      q[idx(i, cn, ldq)] = c * y - s * x;
+    }
+  }
+  __syncthreads();
+  const int k = sh_k, c0 = sh_ctot[0], c1 = sh_ctot[1], c2 = sh_ctot[2], c3 = sh_ctot[3];
+  const int n12 = c0 + c1, n23 = c1 + c2;
+  // Q2 = [ n1 x n12 | n2 x n23 | n x c3 ]
+  double* q2a = q2;
+  double* q2b = q2 + (size_t)n1 * n12;
+  double* q2c = q2b + (size_t)n2 * n23;
+  for (int pos = 0; pos < n; ++pos) {
+    const int js = INDX[pos] - 1;
+    if (pos < c0) {
+      for (int i = t; i < n1; i += blockDim.x) q2a[idx(i, pos, n1)] = q[idx(i, js, ldq)];
+    } else if (pos < c0 + c1) {
+      for (int i = t; i < n1; i += blockDim.x) q2a[idx(i, pos, n1)] = q[idx(i, js, ldq)];
+      for (int i = t; i < n2; i += blockDim.x) q2b[idx(i, pos - c0, n2)] = q[idx(n1 + i, js, ldq)];
+    } else if (pos < c0 + c1 + c2) {
+      for (int i = t; i < n2; i += blockDim.x) q2b[idx(i, pos - c0, n2)] = q[idx(n1 + i, js, ldq)];
+    } else {
+      for (int i = t; i < n; i += blockDim.x) q2c[idx(i, pos - k, n)] = q[idx(i, js, ldq)];
+    }
+  }
+  __syncthreads();
+  // deflated eigenpairs into the last n - K slots
+  for (int j = 0; j < c3; ++j)
+    for (int i = t; i < n; i += blockDim.x) q[idx(i, k + j, ldq)] = q2c[idx(i, j, n)];
+  for (int j = k + t; j < n; j += blockDim.x) d_all[off + j] = D[INDX[j] - 1];
+  for (int j = t; j < k; j += blockDim.x) {
+    dlamda_all[off + j] = DL[j];
+    w_all[off + j] = W[j];
+  }
+  for (int j = t; j < n; j += blockDim.x) indxc_all[off + j] = INDXC[j];
+  if (t == 0) {
+    out[blockIdx.x].k = k;
+    for (int c = 0; c < 4; ++c) out[blockIdx.x].ctot[c] = sh_ctot[c];
+    rho_lvl[blockIdx.x] = sh_rho;
+  }
+}
+
+// One lane per root. S (the K x K eigenvector matrix of the rank-one problem, ld sz) is written here only for K <= 2.
+__global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const NodeOut* out, const double* rho_lvl,
+                                                     const double* __restrict__ dlamda_all,
+                                                     const double* __restrict__ w_all, const int* __restrict__ indxc_all,
+                                                     double* __restrict__ d_all, int* __restrict__ orig_all,
+                                                     double* __restrict__ tau_all, double* __restrict__ s_all,
+                                                     int* info) {
+  const Node nd = nodes[blockIdx.y];
+  const int k = out[blockIdx.y].k;
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= k) return;
+  const double* dl = dlamda_all + nd.off;
+  const double* w = w_all + nd.off;
+  const double rho = rho_lvl[blockIdx.y];
+  double* S = s_all + nd.q2off;
+  if (k == 1) {
+    d_all[nd.off] = dl[0] + rho * w[0] * w[0];
+    S[0] = 1;
+    return;
+  }
+  if (k == 2) {
+    double v[2], lam;
+    const double dd[2] = {dl[0], dl[1]}, zz[2] = {w[0], w[1]};
+    secular::laed5(j, dd, zz, rho, v, lam);
+    d_all[nd.off + j] = lam;
+    const int* ix = indxc_all + nd.off;
+    S[idx(0, j, nd.sz)] = v[ix[0] - 1];
+    S[idx(1, j, nd.sz)] = v[ix[1] - 1];
+    return;
+  }
+  int orig = 0;
+  double tau = 0;
+  const int bad = secular::laed4(k, j, dl, w, rho, orig, tau);
+  if (bad) atomicCAS(info, 0, nd.off + j + 1);
+  orig_all[nd.off + j] = orig;
+  tau_all[nd.off + j] = tau;
+  d_all[nd.off + j] = dl[orig] + tau;
+}
+
+__device__ inline double delta_of(const double* dl, int i, int orig, double tau) { return (dl[i] - dl[orig]) - tau; }
+
+// Gu-Eisenstat: w_i = sign(z_i) sqrt(-delta(i, i) prod_{j != i} delta(i, j) / (d_i - d_j)), same product order as
+// dlaed3 (j ascending).
+__global__ void gu_kernel(const Node* nodes, const NodeOut* out, const double* __restrict__ dlamda_all,
+                          const double* __restrict__ w_all, const int* __restrict__ orig_all,
+                          const double* __restrict__ tau_all, double* __restrict__ wnew_all) {
+  const Node nd = nodes[blockIdx.y];
+  const int k = out[blockIdx.y].k;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= 2 || i >= k) return;
+  const double* dl = dlamda_all + nd.off;
+  const int* og = orig_all + nd.off;
+  const double* ta = tau_all + nd.off;
+  double prod = delta_of(dl, i, og[i], ta[i]);
+  for (int j = 0; j < k; ++j) {
+    if (j == i) continue;
+    prod *= delta_of(dl, i, og[j], ta[j]) / (dl[i] - dl[j]);
+  }
+  const double mag = sqrt(-prod);
+  wnew_all[nd.off + i] = (w_all[nd.off + i] < 0) ? -mag : mag;
+}
+
+// Column j of S: s_i = w_i / delta(i, j), normalized, rows permuted by indxc. One wavefront per column.
+__global__ __launch_bounds__(256) void vectors_kernel(const Node* nodes, const NodeOut* out,
+                                                      const double* __restrict__ dlamda_all,
+                                                      const int* __restrict__ orig_all,
+                                                      const double* __restrict__ tau_all,
+                                                      const double* __restrict__ wnew_all,
+                                                      const int* __restrict__ indxc_all, double* __restrict__ s_all) {
+  const Node nd = nodes[blockIdx.y];
+  const int k = out[blockIdx.y].k;
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (k <= 2 || j >= k) return;
+  const double* dl = dlamda_all + nd.off;
+  const double* wn = wnew_all + nd.off;
+  const int* ix = indxc_all + nd.off;
+  const int oj = orig_all[nd.off + j];
+  const double tj = tau_all[nd.off + j];
+  double sum = 0;
+  for (int i = lane; i < k; i += 64) {
+    const double s = wn[i] / delta_of(dl, i, oj, tj);
+    sum += s * s;
+  }
+  sum = wave_sum(sum);
+  const double scale = 1 / sqrt(sum);
+  double* S = s_all + nd.q2off;
+  for (int i = lane; i < k; i += 64) {
+    const int src = ix[i] - 1;
+    S[idx(i, j, nd.sz)] = wn[src] / delta_of(dl, src, oj, tj) * scale;
+  }
+}
+
+// Merge products with device-side sizes: blockIdx.z = 2 * merge + half.
+__global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, const NodeOut* out,
+                                                         const double* __restrict__ q2_all,
+                                                         const double* __restrict__ s_all, double* __restrict__ q_all,
+                                                         int ldq) {
+  using namespace gemm_detail;
+  __shared__ double As[BK][BM + PAD];
+  __shared__ double Bs[BK][BN + PAD];
+  const int node = blockIdx.z >> 1, half = blockIdx.z & 1;
+  const Node nd = nodes[node];
+  const NodeOut o = out[node];
+  const int k = o.k;
+  if (k == 0) return;
+  const int n1 = nd.n1, n2 = nd.sz - nd.n1;
+  const int n12 = o.ctot[0] + o.ctot[1], n23 = o.ctot[1] + o.ctot[2];
+  const int rows = half ? n2 : n1, inner = half ? n23 : n12;
+  const int m0 = blockIdx.x * BM, c0 = blockIdx.y * BN;
+  if (m0 >= rows || c0 >= k) return;
+  const double* A = q2_all + nd.q2off + (half ? (size_t)n1 * n12 : 0);
+  const double* B = s_all + nd.q2off + (half ? o.ctot[0] : 0);
+  double* C = q_all + idx(nd.off + (half ? n1 : 0), nd.off, ldq);
+  gemm_tile<double, false, false>(rows, k, inner, 1.0, A, rows, B, nd.sz, 0.0, C, ldq, m0, c0, As, Bs);
+}
+
+// dlamrg of the K merged (ascending) and n - K deflated (descending) eigenvalues.
+__global__ void lamrg_kernel(int count, const Node* nodes, const NodeOut* out, const double* __restrict__ d_all,
+                             int* __restrict__ perm_all) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= count) return;
+  const Node nd = nodes[node];
+  const int k = out[node].k;
+  int* perm = perm_all + nd.off;
+  if (k == 0) {
+    for (int i = 0; i < nd.sz; ++i) perm[i] = i + 1;
+  } else {
+    lamrg(k, nd.sz - k, d_all + nd.off, 1, -1, perm);
+  }
+}
+
+// temp(:, i) = Q(:, perm[i]) within each merge's block; dtemp(i) = d(perm[i]).
+__global__ void gather_kernel(const Node* nodes, const int* __restrict__ perm_all, const double* __restrict__ q_all,
+                              int ldq, const double* __restrict__ d_all, double* __restrict__ tmp_all,
+                              double* __restrict__ dtmp_all) {
+  const Node nd = nodes[blockIdx.z];
+  const int col = blockIdx.y;
+  if (col >= nd.sz) return;
+  const int src = perm_all[nd.off + col] - 1;
+  double* tmp = tmp_all + nd.q2off;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd.sz; i += gridDim.x * blockDim.x)
+    tmp[idx(i, col, nd.sz)] = q_all[idx(nd.off + i, nd.off + src, ldq)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) dtmp_all[nd.off + col] = d_all[nd.off + src];
+}
+
+__global__ void scatter_kernel(const Node* nodes, const double* __restrict__ tmp_all,
+                               const double* __restrict__ dtmp_all, double* __restrict__ q_all, int ldq,
+                               double* __restrict__ d_all) {
+  const Node nd = nodes[blockIdx.z];
+  const int col = blockIdx.y;
+  if (col >= nd.sz) return;
+  const double* tmp = tmp_all + nd.q2off;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd.sz; i += gridDim.x * blockDim.x)
+    q_all[idx(nd.off + i, nd.off + col, ldq)] = tmp[idx(i, col, nd.sz)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) d_all[nd.off + col] = dtmp_all[nd.off + col];
+}
+
+void build(std::vector<Node>& pre, std::vector<std::vector<int>>& levels, int off, int sz, int depth) {
+  if (sz <= 1) return;
+  const int n1 = sz / 2;
+  const int id = (int)pre.size();
+  pre.push_back(Node{off, sz, n1, 0});
+  if ((int)levels.size() <= depth) levels.resize(depth + 1);
+  levels[depth].push_back(id);
+  build(pre, levels, off, n1, depth + 1);
+  build(pre, levels, off + n1, sz - n1, depth + 1);
+}
+
+}  // namespace
+
+DcPlan::DcPlan(int n_) : n(n_) {
+  if (n <= 1) return;
+  std::vector<Node> pre;
+  std::vector<std::vector<int>> levels;
+  build(pre, levels, 0, n, 0);
+  nmerge = (int)pre.size();
+  // device node array: [pre-order (for the tears)] then every level's merges with their Q2 / S offsets
+  std::vector<Node> nodes(pre);
+  std::vector<int> pre_index;
+  for (size_t L = 0; L < levels.size(); ++L) {
+    Level lv;
+    lv.start = (int)nodes.size();
+    lv.count = (int)levels[L].size();
+    lv.pre_start = (int)pre_index.size();
+    int q2off = 0;
+    for (int id : levels[L]) {
+      Node nd = pre[id];
+      nd.q2off = q2off;
+      q2off += nd.sz * nd.sz;
+      lv.maxsz = std::max(lv.maxsz, nd.sz);
+      nodes.push_back(nd);
+      pre_index.push_back(id);
+    }
+    lvls.push_back(lv);
+  }
+  XTB_HIP(hipMalloc(&nodes_dev, sizeof(Node) * nodes.size()));
+  XTB_HIP(hipMalloc(&pre_index_dev, sizeof(int) * pre_index.size()));
+  XTB_HIP(hipMemcpy(nodes_dev, nodes.data(), sizeof(Node) * nodes.size(), hipMemcpyHostToDevice));
+  XTB_HIP(hipMemcpy(pre_index_dev, pre_index.data(), sizeof(int) * pre_index.size(), hipMemcpyHostToDevice));
+}
+
+DcPlan::~DcPlan() {
+  if (nodes_dev) (void)hipFree(nodes_dev);
+  if (pre_index_dev) (void)hipFree(pre_index_dev);
+}
+
+// Workspace layout (doubles): dlamda, w, tau, wnew, dtmp (n each, dtmp 2n), rho_pre (n), rho_lvl (n), scale (16),
+// Q2 (n^2), S (n^2). Ints: orig, indxc, perm (n each), out (8 n).
+size_t dc_workspace_doubles(int n) { return 8 * (size_t)n + 16 + 2 * (size_t)n * n; }
+size_t dc_workspace_ints(int n) { return 3 * (size_t)n + 8 * (size_t)n + 16; }
+
+void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, int ldz, double* work, int* iwork,
+           int* info) {
+  const int n = plan.n;
+  if (n <= 0) return;
+  if (n == 1) {
+    set_identity(s, 1, z, ldz);
+    return;
+  }
+  double* dlamda = work;
+  double* w = dlamda + n;
+  double* tau = w + n;
+  double* wnew = tau + n;
+  double* dtmp = wnew + n;
+  double* rho_pre = dtmp + 2 * (size_t)n;
+  double* rho_lvl = rho_pre + n;
+  double* scale = rho_lvl + n;
+  double* q2 = scale + 16;
+  double* sbuf = q2 + (size_t)n * n;
+  int* orig = iwork;
+  int* indxc = orig + n;
+  int* perm = indxc + n;
+  NodeOut* outs = reinterpret_cast<NodeOut*>(perm + n);
+  const Node* nodes = plan.nodes_dev;
+  set_identity(s, n, z, ldz);
+  hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(256), 0, s, n, d, e, nodes, plan.nmerge, rho_pre, scale);
+  XTB_LAUNCH_CHECK();
+  for (int L = (int)plan.lvls.size() - 1; L >= 0; --L) {
+    const DcPlan::Level& lv = plan.lvls[L];
+    const int cnt = lv.count, maxsz = lv.maxsz;
+    const Node* lvl = nodes + lv.start;
+    const int* lvl_pre = plan.pre_index_dev + lv.pre_start;
+    const size_t lds = (size_t)maxsz * (6 * sizeof(double) + 7 * sizeof(int)) + 64;
+    if (lds > 160 * 1024) throw std::runtime_error("stedc: n too large for the LDS-staged deflation");
+    allow_lds(deflate_kernel, lds);
+    hipLaunchKernelGGL(deflate_kernel, dim3(cnt), dim3(256), lds, s, lvl, lvl_pre, n, d, z, ldz, rho_pre, rho_lvl,
+                       dlamda, w, q2, indxc, outs);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 0, s, lvl, outs, rho_lvl, dlamda, w,
+                       indxc, d, orig, tau, sbuf, info);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 0, s, lvl, outs, dlamda, w, orig, tau, wnew);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(maxsz, 4), cnt), dim3(256), 0, s, lvl, outs, dlamda, orig, tau,
+                       wnew, indxc, sbuf);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), 2 * cnt), dim3(256), 0, s, lvl,
+                       outs, q2, sbuf, z, ldz);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(lamrg_kernel, dim3(cdiv(cnt, 256)), dim3(256), 0, s, cnt, lvl, outs, d, perm);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gather_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, perm, z, ldz, d,
+                       sbuf, dtmp);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, sbuf, dtmp, z, ldz,
+                       d);
+    XTB_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(unscale_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, scale);
+  XTB_LAUNCH_CHECK();
+}
+
+}  // namespace xtbeig

@@ -1,0 +1,123 @@
+// FP64 refinement of approximate generalized eigenpairs (H C = S C Lambda, C^T S C = I), Ogita-Aishima style with
+// the S inner product, and the eigenpair sort. The reference refines FP32 eigenpairs in FP64 (scripts/kernels/
+// fp32_error.c:95-158; the S-inner-product variant in scripts/kernels/refine.f90:254-325):
+//     R = I - C^T S C,  A = C^T H C,  lambda_i = A_ii / (1 - R_ii),
+//     E_ij = (A_ij + lambda_j R_ij) / (lambda_j - lambda_i)  for |lambda_j - lambda_i| > delta,  E_ij = R_ij / 2 else,
+//     C <- C + C E.
+// The four products and the update run as FP64 GEMMs on the matrix cores. delta follows the size of the current
+// residual: pairs closer than 64 max|R| max|lambda| are treated as a cluster (their subspace converges; the basis
+// inside it is not refined), so near-degenerate pairs separate as the residual shrinks.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace xtbeig {
+
+namespace {
+
+// R <- I - R (R = C^T S C on entry), lambda_i = A_ii / (C^T S C)_ii; stats[0] = max |R_ij|, stats[1] = max |lambda|.
+__global__ void oa_prepare_kernel(int n, double* __restrict__ r, int ldr, const double* __restrict__ a, int lda,
+                                  double* __restrict__ lam, unsigned long long* __restrict__ stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  double m = 0;
+  if (i < n) {
+    const double v = r[idx(i, j, ldr)];
+    if (i == j) lam[i] = a[idx(i, i, lda)] / v;
+    const double rv = (i == j ? 1.0 : 0.0) - v;
+    r[idx(i, j, ldr)] = rv;
+    m = fabs(rv);
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) atomicMax(&stats[0], __double_as_longlong(m));  // non-negative doubles order as ints
+  if (i < n && i == j) {
+    const double l = fabs(a[idx(i, i, lda)] / (1.0 - r[idx(i, j, ldr)]));
+    atomicMax(&stats[1], __double_as_longlong(l));
+  }
+}
+
+__global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, const double* __restrict__ a, int lda,
+                                 const double* __restrict__ lam, const unsigned long long* __restrict__ stats,
+                                 double* __restrict__ e, int lde) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= n) return;
+  const double rmax = __longlong_as_double(stats[0]), lmax = __longlong_as_double(stats[1]);
+  const double delta = 64.0 * rmax * fmax(lmax, 1e-300);
+  const double rij = r[idx(i, j, ldr)];
+  const double gap = lam[j] - lam[i];
+  double v;
+  if (i != j && fabs(gap) > delta) v = (a[idx(i, j, lda)] + lam[j] * rij) / gap;
+  else v = 0.5 * rij;
+  e[idx(i, j, lde)] = v;
+}
+
+__global__ void zero_stats_kernel(unsigned long long* stats) {
+  if (threadIdx.x < 2) stats[threadIdx.x] = 0;
+}
+
+// rank of lambda_i in ascending order (ties by index)
+__global__ void rank_kernel(int n, const double* __restrict__ lam, int* __restrict__ rank) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double li = lam[i];
+  int r = 0;
+  for (int j = 0; j < n; ++j) {
+    const double lj = lam[j];
+    r += (lj < li) || (lj == li && j < i);
+  }
+  rank[i] = r;
+}
+
+// dst(:, rank[j]) = src(:, j), w[rank[j]] = lam[j]
+template <typename T>
+__global__ void scatter_columns_kernel(int n, int m, const int* __restrict__ rank, const T* __restrict__ src, int lds,
+                                       T* __restrict__ dst, int ldd, const double* __restrict__ lam,
+                                       double* __restrict__ w) {
+  const int j = blockIdx.y;
+  const int r = rank[j];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[idx(i, r, ldd)] = src[idx(i, j, lds)];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && lam != nullptr) w[r] = lam[j];
+  (void)m;
+}
+
+}  // namespace
+
+size_t refine_workspace(int n) { return 6 * (size_t)n * n + 2 * (size_t)n + 16; }
+
+// One or more Ogita-Aishima steps on C (n x n, ldc), then eigenvalues w (ascending) and C's columns sorted to match.
+void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm, int lds, double* c, int ldc,
+               double* w, int iterations, double* work, int* iwork) {
+  const size_t nn = (size_t)n * n;
+  double* sc = work;
+  double* hc = sc + nn;
+  double* r = hc + nn;
+  double* a = r + nn;
+  double* e = a + nn;
+  double* cnew = e + nn;
+  double* lam = cnew + nn;
+  unsigned long long* stats = reinterpret_cast<unsigned long long*>(lam + n + n);
+  const dim3 eg(cdiv(n, 256), n);
+  for (int it = 0; it < iterations; ++it) {
+    gemm<double>(s, 'N', 'N', n, n, n, 1.0, sm, lds, c, ldc, 0.0, sc, n);   // S C
+    gemm<double>(s, 'N', 'N', n, n, n, 1.0, h, ldh, c, ldc, 0.0, hc, n);    // H C
+    gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, sc, n, 0.0, r, n);      // C^T S C
+    gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, hc, n, 0.0, a, n);      // C^T H C
+    hipLaunchKernelGGL(zero_stats_kernel, dim3(1), dim3(64), 0, s, stats);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, stats);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, stats, e, n);
+    XTB_LAUNCH_CHECK();
+    copy_matrix<double, double>(s, n, n, c, ldc, cnew, n);
+    gemm<double>(s, 'N', 'N', n, n, n, 1.0, c, ldc, e, n, 1.0, cnew, n);    // C + C E
+    copy_matrix<double, double>(s, n, n, cnew, n, c, ldc);
+  }
+  // sort
+  hipLaunchKernelGGL(rank_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, lam, iwork);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, n, iwork, c, ldc,
+                     cnew, n, lam, w);
+  XTB_LAUNCH_CHECK();
+  copy_matrix<double, double>(s, n, n, cnew, n, c, ldc);
+}
+
+}  // namespace xtbeig

@@ -1,0 +1,248 @@
+// The eigensolver drivers: per-handle device workspace, the standard solve (two-stage reduction, divide and conquer,
+// both back-transformations) and the generalized solve in FP64 or in FP32 with FP64 refinement.
+//
+// Generalized solve H C = S C Lambda (itype 1), as xTB's fact_solve path (the reference: scripts/kernels/
+// eigbench.f90:101-168, scripts/kernels/congruence.c:1-52):
+//   S = U^T U and X = U^{-1}                          FP64 always (chol.hip)
+//   H' = X^T H X                                      two GEMMs, FP64 or FP32
+//   H' Z = Z Lambda                                    syevd_core: dense -> band (MFMA), bulge chase, FP64 D&C,
+//                                                      bulge and band back-transforms, FP64 or FP32
+//   C = X Z                                           GEMM, FP64 or FP32
+//   FP32 only: FP64 Ogita-Aishima refinement of (Lambda, C) against (H, S), then the eigenpair sort.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace xtbeig {
+
+int default_band(int n, bool fp32) {
+  (void)fp32;
+  if (n <= 3) return 1;
+  int b = 32;
+  if (n < 100) b = 16;
+  return std::min(b, std::max(2, n - 1));
+}
+
+Handle::Handle(int dev) : device(dev) {
+  XTB_HIP(hipSetDevice(device));
+  XTB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  own_stream = true;
+  XTB_HIP(hipMalloc(&dinfo, 64 * sizeof(int)));
+  XTB_HIP(hipMemset(dinfo, 0, 64 * sizeof(int)));
+}
+
+Handle::~Handle() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (arena) (void)hipFree(arena);
+  if (dinfo) (void)hipFree(dinfo);
+  if (pinned) (void)hipHostFree(pinned);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Handle::reserve(size_t bytes) {
+  if (bytes <= arena_bytes) return;
+  XTB_HIP(hipSetDevice(device));
+  if (arena) {
+    XTB_HIP(hipStreamSynchronize(stream));
+    XTB_HIP(hipFree(arena));
+    arena = nullptr;
+    arena_bytes = 0;
+  }
+  XTB_HIP(hipMalloc(&arena, bytes));
+  arena_bytes = bytes;
+}
+
+void* Handle::host_staging(size_t bytes) {
+  if (bytes > pinned_bytes) {
+    if (pinned) XTB_HIP(hipHostFree(pinned));
+    pinned = nullptr;
+    XTB_HIP(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
+    pinned_bytes = bytes;
+  }
+  return pinned;
+}
+
+const DcPlan& Handle::plan(int n) {
+  auto it = plans.find(n);
+  if (it != plans.end()) return *it->second;
+  auto p = std::make_unique<DcPlan>(n);
+  const DcPlan& ref = *p;
+  plans.emplace(n, std::move(p));
+  return ref;
+}
+
+// ---- workspace carving --------------------------------------------------------------------------------------------
+
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+template <typename T>
+struct StdBuffers {
+  T *v1, *t1, *ab, *bwork, *v2, *dT, *eT, *btwork;
+  double *d64, *e64, *z64, *dcwork;
+  int* dciwork;
+};
+
+template <typename T>
+StdBuffers<T> carve_std(Carver& c, int n, int b) {
+  StdBuffers<T> s;
+  const size_t nn = (size_t)n * n;
+  s.v1 = c.take<T>(nn);
+  s.t1 = c.take<T>((size_t)b * n);
+  s.ab = c.take<T>(band_elems(n, b));
+  s.bwork = c.take<T>(band_workspace<T>(n, b));
+  s.v2 = c.take<T>(nn);
+  s.dT = c.take<T>(n);
+  s.eT = c.take<T>(n);
+  s.btwork = c.take<T>(2 * (size_t)b * n);
+  s.d64 = c.take<double>(n);
+  s.e64 = c.take<double>(n);
+  s.z64 = std::is_same<T, double>::value ? nullptr : c.take<double>(nn);
+  s.dcwork = c.take<double>(dc_workspace_doubles(n));
+  s.dciwork = c.take<int>(dc_workspace_ints(n));
+  return s;
+}
+
+// Eigenpairs of the symmetric a (n x n, full storage, destroyed): w ascending (FP64), z (n x n).
+template <typename T>
+void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, double* w, T* z, int ldz) {
+  hipStream_t st = h.stream;
+  if (n == 1) {
+    copy_matrix<T, double>(st, 1, 1, a, lda, w, 1);
+    set_identity(st, 1, z, ldz);
+    return;
+  }
+  reduce_to_band<T>(st, n, b, a, lda, s.v1, s.t1, s.ab, s.bwork);
+  chase_bulges<T>(st, n, b, s.ab, s.v2, s.dT, s.eT, h.dinfo + 1);
+  copy_matrix<T, double>(st, n, 1, s.dT, n, s.d64, n);
+  copy_matrix<T, double>(st, n - 1, 1, s.eT, n, s.e64, n);
+  double* z64 = std::is_same<T, double>::value ? reinterpret_cast<double*>(z) : s.z64;
+  const int ldz64 = std::is_same<T, double>::value ? ldz : n;
+  stedc(st, h.plan(n), s.d64, s.e64, z64, ldz64, s.dcwork, s.dciwork, h.dinfo + 2);
+  if (!std::is_same<T, double>::value) copy_matrix<double, T>(st, n, n, z64, n, z, ldz);
+  copy_matrix<double, double>(st, n, 1, s.d64, n, w, n);
+  back_transform_bulges<T>(st, n, b, s.v2, z, ldz, n);
+  back_transform_band<T>(st, n, b, s.v1, s.t1, z, ldz, n, s.btwork);
+}
+
+template <typename T>
+size_t syevd_bytes(int n, int b) {
+  Carver c(nullptr);
+  c.take<T>((size_t)n * n);  // working copy
+  carve_std<T>(c, n, b);
+  return c.off + 4096;
+}
+
+template <typename T>
+void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b) {
+  if (n <= 0) return;
+  if (b <= 0) b = default_band(n, sizeof(T) == 4);
+  b = std::max(1, std::min(b, std::max(1, n - 1)));
+  h.reserve(syevd_bytes<T>(n, b));
+  Carver c(h.arena);
+  T* work = c.take<T>((size_t)n * n);
+  StdBuffers<T> s = carve_std<T>(c, n, b);
+  copy_matrix<T, T>(h.stream, n, n, a, lda, work, n);
+  syevd_core<T>(h, s, n, b, work, n, w, z, ldz);
+}
+
+size_t sygvd_bytes(int n, int b, bool mixed) {
+  Carver c(nullptr);
+  const size_t nn = (size_t)n * n;
+  c.take<double>(nn);  // S copy
+  c.take<double>(nn);  // U
+  c.take<double>(nn);  // X
+  c.take<double>((size_t)n * 64);
+  if (!mixed) {
+    c.take<double>(nn);  // H X
+    c.take<double>(nn);  // H'
+    c.take<double>(nn);  // Z
+    carve_std<double>(c, n, b);
+  } else {
+    c.take<float>(nn);  // X32
+    c.take<float>(nn);  // H32
+    c.take<float>(nn);  // H X
+    c.take<float>(nn);  // H'
+    c.take<float>(nn);  // Z32
+    c.take<float>(nn);  // C32
+    carve_std<float>(c, n, b);
+    c.take<double>(refine_workspace(n));
+    c.take<int>(n);
+  }
+  return c.off + 4096;
+}
+
+void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
+                  int ldc, bool mixed, int b) {
+  if (n <= 0) return;
+  hipStream_t st = h.stream;
+  if (b <= 0) b = default_band(n, mixed);
+  b = std::max(1, std::min(b, std::max(1, n - 1)));
+  h.reserve(sygvd_bytes(n, b, mixed));
+  Carver c(h.arena);
+  const size_t nn = (size_t)n * n;
+  double* scopy = c.take<double>(nn);
+  double* u = c.take<double>(nn);
+  double* x = c.take<double>(nn);
+  double* tmp = c.take<double>((size_t)n * 64);
+  copy_matrix<double, double>(st, n, n, sm, lds, scopy, n);
+  potrf_inverse(st, n, scopy, n, u, n, x, n, tmp, h.dinfo + 0);
+  if (!mixed) {
+    double* hx = c.take<double>(nn);
+    double* hp = c.take<double>(nn);
+    double* z = c.take<double>(nn);
+    StdBuffers<double> s = carve_std<double>(c, n, b);
+    gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, n, 0.0, hx, n);
+    gemm<double>(st, 'T', 'N', n, n, n, 1.0, x, n, hx, n, 0.0, hp, n);
+    symmetrize<double>(st, 'L', n, hp, n);
+    syevd_core<double>(h, s, n, b, hp, n, w, z, n);
+    gemm<double>(st, 'N', 'N', n, n, n, 1.0, x, n, z, n, 0.0, cm, ldc);
+  } else {
+    float* x32 = c.take<float>(nn);
+    float* h32 = c.take<float>(nn);
+    float* hx = c.take<float>(nn);
+    float* hp = c.take<float>(nn);
+    float* z = c.take<float>(nn);
+    float* c32 = c.take<float>(nn);
+    StdBuffers<float> s = carve_std<float>(c, n, b);
+    double* rwork = c.take<double>(refine_workspace(n));
+    int* riwork = c.take<int>(n);
+    copy_matrix<double, float>(st, n, n, x, n, x32, n);
+    copy_matrix<double, float>(st, n, n, hm, ldh, h32, n);
+    gemm<float>(st, 'N', 'N', n, n, n, 1.0f, h32, n, x32, n, 0.0f, hx, n);
+    gemm<float>(st, 'T', 'N', n, n, n, 1.0f, x32, n, hx, n, 0.0f, hp, n);
+    symmetrize<float>(st, 'L', n, hp, n);
+    syevd_core<float>(h, s, n, b, hp, n, w, z, n);
+    gemm<float>(st, 'N', 'N', n, n, n, 1.0f, x32, n, z, n, 0.0f, c32, n);
+    copy_matrix<float, double>(st, n, n, c32, n, cm, ldc);
+    refine_oa(st, n, hm, ldh, sm, lds, cm, ldc, w, h.refine_iters, rwork, riwork);
+  }
+}
+
+void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz) {
+  if (n <= 0) return;
+  Carver c0(nullptr);
+  c0.take<double>(dc_workspace_doubles(n));
+  c0.take<int>(dc_workspace_ints(n));
+  h.reserve(c0.off + 4096);
+  Carver c(h.arena);
+  double* work = c.take<double>(dc_workspace_doubles(n));
+  int* iwork = c.take<int>(dc_workspace_ints(n));
+  stedc(h.stream, h.plan(n), d, e, z, ldz, work, iwork, h.dinfo + 2);
+}
+
+template void syevd_device<double>(Handle&, int, const double*, int, double*, double*, int, int);
+template void syevd_device<float>(Handle&, int, const float*, int, double*, float*, int, int);
+
+}  // namespace xtbeig

@@ -1,0 +1,119 @@
+// Elementwise matrix kernels: copies with precision conversion, symmetrization, identity, and the 32 x 32-tiled
+// upper -> full copy behind xtb_dsymmetrize_pad (Exports.swift:117-138 of the reference).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace xtbeig {
+
+namespace {
+
+template <typename S, typename D>
+__global__ void copy_kernel(int m, int n, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int j = blockIdx.y; j < n; j += gridDim.y)
+    if (i < m) dst[idx(i, j, ldd)] = D(src[idx(i, j, lds)]);
+}
+
+// 32 x 32 tiles through LDS: tile (bi, bj) with bi <= bj reads the referenced triangle and writes both.
+template <typename T>
+__global__ void symmetrize_kernel(int n, T* a, int lda, bool upper) {
+  __shared__ T tile[32][33];
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi > bj) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  // Source tile: upper -> rows bi, cols bj; lower -> rows bj, cols bi (transposed role).
+  for (int r = ty; r < 32; r += 8) {
+    const int i = (upper ? bi : bj) * 32 + tx, j = (upper ? bj : bi) * 32 + r;
+    if (i < n && j < n) tile[r][tx] = a[idx(i, j, lda)];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    // write the mirror: element (i, j) of the source goes to (j, i)
+    const int i = (upper ? bj : bi) * 32 + tx, j = (upper ? bi : bj) * 32 + r;
+    // dst (i, j) = src (j, i) = tile[i - ...][j - ...]
+    if (i < n && j < n) {
+      const bool ref = upper ? (j <= i) : (j >= i);  // (j, i) lies in the referenced triangle
+      if (ref) a[idx(i, j, lda)] = tile[tx][r];
+    }
+  }
+}
+
+template <typename T>
+__global__ void fill_kernel(int m, int n, T* a, int lda, T diag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int j = blockIdx.y; j < n; j += gridDim.y)
+    if (i < m) a[idx(i, j, lda)] = (i == j) ? diag : T(0);
+}
+
+__global__ void symmetrize_pad_kernel(int n, const double* __restrict__ src, int lds, double* __restrict__ dst,
+                                      int ldd) {
+  __shared__ double tile[32][33];
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi > bj) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bi * 32 + tx, j = bj * 32 + r;
+    if (i < n && j < n && i <= j) {
+      const double v = src[idx(i, j, lds)];
+      tile[r][tx] = v;
+      dst[idx(i, j, ldd)] = v;
+    }
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = bj * 32 + tx, j = bi * 32 + r;  // (i, j) in the lower triangle, mirror of (j, i)
+    if (i < n && j < n && j <= i) dst[idx(i, j, ldd)] = tile[tx][r];
+  }
+}
+
+}  // namespace
+
+template <typename S, typename D>
+void copy_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL((copy_kernel<S, D>), dim3(cdiv(m, 256), n < 65535 ? n : 65535), dim3(256), 0, s, m, n, src,
+                     lds, dst, ldd);
+  XTB_LAUNCH_CHECK();
+}
+
+template <typename T>
+void symmetrize(hipStream_t s, char uplo, int n, T* a, int lda) {
+  if (n <= 1) return;
+  const int nt = cdiv(n, 32);
+  hipLaunchKernelGGL(symmetrize_kernel<T>, dim3(nt, nt), dim3(32, 8), 0, s, n, a, lda, uplo == 'U' || uplo == 'u');
+  XTB_LAUNCH_CHECK();
+}
+
+template <typename T>
+void set_zero(hipStream_t s, int m, int n, T* a, int lda) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(cdiv(m, 256), n < 65535 ? n : 65535), dim3(256), 0, s, m, n, a, lda, T(0));
+  XTB_LAUNCH_CHECK();
+}
+
+template <typename T>
+void set_identity(hipStream_t s, int n, T* a, int lda) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(cdiv(n, 256), n < 65535 ? n : 65535), dim3(256), 0, s, n, n, a, lda, T(1));
+  XTB_LAUNCH_CHECK();
+}
+
+void symmetrize_pad(hipStream_t s, int n, const double* src, int lds, double* dst, int ldd) {
+  if (n <= 0) return;
+  const int nt = cdiv(n, 32);
+  hipLaunchKernelGGL(symmetrize_pad_kernel, dim3(nt, nt), dim3(32, 8), 0, s, n, src, lds, dst, ldd);
+  XTB_LAUNCH_CHECK();
+}
+
+template void copy_matrix<double, double>(hipStream_t, int, int, const double*, int, double*, int);
+template void copy_matrix<double, float>(hipStream_t, int, int, const double*, int, float*, int);
+template void copy_matrix<float, double>(hipStream_t, int, int, const float*, int, double*, int);
+template void copy_matrix<float, float>(hipStream_t, int, int, const float*, int, float*, int);
+template void symmetrize<double>(hipStream_t, char, int, double*, int);
+template void symmetrize<float>(hipStream_t, char, int, float*, int);
+template void set_zero<double>(hipStream_t, int, int, double*, int);
+template void set_zero<float>(hipStream_t, int, int, float*, int);
+template void set_identity<double>(hipStream_t, int, double*, int);
+template void set_identity<float>(hipStream_t, int, float*, int);
+
+}  // namespace xtbeig

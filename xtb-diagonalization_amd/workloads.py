@@ -1,0 +1,111 @@
+"""Synthetic eigenproblems of the benchmark configurations and the accuracy gates.
+
+Inputs (all column-major float64, symmetric):
+  * spectrum(n): 45 % uniform in [-0.9, -0.25], 10 % in [-0.25, -0.15], the rest in [0.0, 0.6]; then 6 exactly
+    repeated pairs and 4 pairs split by 1e-9 (entries overwritten in place).
+  * standard_matrix(n): H = Q diag(lambda) Q^T with Q = H_1 ... H_n, H_i = I - 2 v v^T / (v^T v), v ~ N(0, 1)^n.
+  * overlap_matrix(n): S = I/2 + G/2, G_ij = exp(-|r_i - r_j|^2 / (2 * 1.2^2)) for n points uniform in a cube of
+    side (n / 0.8)^(1/3). (The construction gives cond(S) of about 20 at these n: the I/2 shift bounds the smallest
+    eigenvalue by 1/2.)
+Seeds are fixed per configuration: 1000 + configuration index.
+
+Gates (the reference's accuracy quantities, Sources/EigenBench/main.swift:233-251 and scripts/kernels/
+eigbench.f90:189-204, with the thresholds of the generalized-solve contract):
+  * eigenvalue error  max |w - w_ref| <= 1e-12 ||H||_2
+  * residual          ||H C - S C diag(w)||_F / (n ||H||_2) <= 1e-13
+  * orthogonality     max |C^T S C - I| <= 1e-12 n
+"""
+import numpy as np
+
+SEED_BASE = 1000
+
+
+def spectrum(n, rng):
+    n1 = int(round(0.45 * n))
+    n2 = int(round(0.10 * n))
+    lam = np.concatenate([
+        rng.uniform(-0.9, -0.25, n1),
+        rng.uniform(-0.25, -0.15, n2),
+        rng.uniform(0.0, 0.6, n - n1 - n2),
+    ])
+    if n >= 20:
+        pick = rng.permutation(n)[:20]
+        for p in range(6):
+            lam[pick[2 * p + 1]] = lam[pick[2 * p]]
+        for p in range(6, 10):
+            lam[pick[2 * p + 1]] = lam[pick[2 * p]] + 1e-9
+    return lam
+
+
+def householder_product(n, rng, block=32):
+    """Q = H_1 H_2 ... H_n with H_i = I - 2 v_i v_i^T / (v_i^T v_i), v_i ~ N(0, 1)^n, applied block-wise
+    (compact WY, Q = I - Y T Y^T per block of reflectors) so that it costs GEMMs."""
+    q = np.eye(n)
+    i = 0
+    while i < n:
+        k = min(block, n - i)
+        v = rng.standard_normal((k, n)).T  # one N(0, 1)^n vector per reflector, in order
+        tau = 2.0 / np.einsum("ij,ij->j", v, v)
+        # T for H_i ... H_{i+k-1} = I - V T V^T (forward, columnwise)
+        t = np.zeros((k, k))
+        g = v.T @ v
+        for j in range(k):
+            t[j, j] = tau[j]
+            if j:
+                t[:j, j] = -tau[j] * (t[:j, :j] @ g[:j, j])
+        # Q <- Q (I - V T V^T)
+        q = q - (q @ v) @ t @ v.T
+        i += k
+    return q
+
+
+def standard_matrix(n, seed):
+    rng = np.random.default_rng(seed)
+    lam = spectrum(n, rng)
+    q = householder_product(n, rng)
+    h = (q * lam) @ q.T
+    h = 0.5 * (h + h.T)
+    return np.asfortranarray(h), np.sort(lam)
+
+
+def overlap_matrix(n, rng):
+    side = (n / 0.8) ** (1.0 / 3.0)
+    r = rng.uniform(0.0, side, (n, 3))
+    sq = np.einsum("ij,ij->i", r, r)
+    d2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * (r @ r.T), 0.0)
+    s = 0.5 * np.eye(n) + 0.5 * np.exp(-d2 / (2.0 * 1.2 ** 2))
+    return np.asfortranarray(0.5 * (s + s.T))
+
+
+def generalized_pair(n, seed):
+    """(H, S) of the generalized configurations: H as standard_matrix, S = overlap_matrix, one seed."""
+    rng = np.random.default_rng(seed)
+    lam = spectrum(n, rng)
+    q = householder_product(n, rng)
+    h = (q * lam) @ q.T
+    h = np.asfortranarray(0.5 * (h + h.T))
+    s = overlap_matrix(n, rng)
+    return h, s
+
+
+def gates(h, s, w, c, w_ref):
+    """The three contract quantities, normalized so that each passes at <= 1."""
+    n = h.shape[0]
+    nh = np.linalg.norm(h, 2)
+    eig = np.max(np.abs(np.asarray(w) - np.asarray(w_ref))) / nh
+    if s is None:
+        r = h @ c - c * w
+        o = np.max(np.abs(c.T @ c - np.eye(n)))
+    else:
+        r = h @ c - (s @ c) * w
+        o = np.max(np.abs(c.T @ s @ c - np.eye(n)))
+    res = np.linalg.norm(r) / (n * nh)
+    return {
+        "eig_err": float(eig), "eig_gate": float(eig / 1e-12),
+        "residual": float(res), "residual_gate": float(res / 1e-13),
+        "orth": float(o), "orth_gate": float(o / (1e-12 * n)),
+    }
+
+
+def passes(g):
+    return g["eig_gate"] <= 1 and g["residual_gate"] <= 1 and g["orth_gate"] <= 1

@@ -1,0 +1,271 @@
+"""Host-side interface to libxtbeig.so (the HIP eigensolver), through its C ABI (include/xtbeig.h).
+
+Two layers, mirroring the library:
+  * LAPACK-style host calls on numpy arrays: dsygvd / dsygvd_mixed / dsyevd / ssyevd / dstedc / sstedc /
+    dsymmetrize_pad, with the same argument meaning and info convention as the Fortran-ABI entry points xTB links.
+  * Handle: device-resident calls on torch tensors (torch is used only for device memory and the stream).
+
+The library must be built (xtb-diagonalization_amd/build.py); loading fails loudly otherwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxtbeig.so")
+
+FP64 = 0
+FP32_REFINE = 1
+
+_lib = None
+
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_dbl_p = ctypes.POINTER(ctypes.c_double)
+c_flt_p = ctypes.POINTER(ctypes.c_float)
+vp = ctypes.c_void_p
+ci = ctypes.c_int
+
+# symbol -> (restype, argtypes)
+SIGNATURES = {
+    "xtb_dstedc": (None, [c_int_p, vp, vp, vp, c_int_p, vp, c_int_p, vp, c_int_p, c_int_p]),
+    "xtb_sstedc": (None, [c_int_p, vp, vp, vp, c_int_p, vp, c_int_p, vp, c_int_p, c_int_p]),
+    "xtb_dsymmetrize_pad": (None, [c_int_p, vp, c_int_p, vp, c_int_p]),
+    "xtbeig_dsyevd": (None, [ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, vp, c_int_p, vp, c_int_p,
+                             c_int_p]),
+    "xtbeig_ssyevd": (None, [ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, vp, c_int_p, vp, c_int_p,
+                             c_int_p]),
+    "xtbeig_dsygvd": (None, [c_int_p, ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, c_int_p, vp, vp,
+                             c_int_p, vp, c_int_p, c_int_p]),
+    "xtbeig_dsygvd_mixed": (None, [c_int_p, ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, c_int_p, vp,
+                                   vp, c_int_p, vp, c_int_p, c_int_p]),
+    "xtbeig_create": (ci, [ctypes.POINTER(vp), ci]),
+    "xtbeig_destroy": (ci, [vp]),
+    "xtbeig_set_stream": (ci, [vp, vp]),
+    "xtbeig_get_stream": (vp, [vp]),
+    "xtbeig_set_tuning": (ci, [vp, ci, ci]),
+    "xtbeig_synchronize": (ci, [vp, c_int_p]),
+    "xtbeig_last_error": (ctypes.c_char_p, []),
+    "xtbeig_dev_sygvd": (ci, [vp, ci, ci, vp, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_dev_dsyevd": (ci, [vp, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_dev_ssyevd": (ci, [vp, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_dev_dstedc": (ci, [vp, ci, vp, vp, vp, ci]),
+    "xtbeig_dev_dgemm": (ci, [vp, ctypes.c_char, ctypes.c_char, ci, ci, ci, ctypes.c_double, vp, ci, vp, ci,
+                              ctypes.c_double, vp, ci]),
+    "xtbeig_dev_sgemm": (ci, [vp, ctypes.c_char, ctypes.c_char, ci, ci, ci, ctypes.c_float, vp, ci, vp, ci,
+                              ctypes.c_float, vp, ci]),
+    "xtbeig_dev_potrf_inverse": (ci, [vp, ci, vp, ci, vp, ci, vp, ci]),
+    "xtbeig_band_elems": (ctypes.c_longlong, [ci, ci]),
+    "xtbeig_dev_dreduce_to_band": (ci, [vp, ci, ci, vp, ci, vp, vp, vp]),
+    "xtbeig_dev_sreduce_to_band": (ci, [vp, ci, ci, vp, ci, vp, vp, vp]),
+    "xtbeig_dev_dchase": (ci, [vp, ci, ci, vp, vp, vp, vp]),
+    "xtbeig_dev_schase": (ci, [vp, ci, ci, vp, vp, vp, vp]),
+    "xtbeig_dev_dback_bulges": (ci, [vp, ci, ci, vp, vp, ci, ci]),
+    "xtbeig_dev_sback_bulges": (ci, [vp, ci, ci, vp, vp, ci, ci]),
+    "xtbeig_dev_dback_band": (ci, [vp, ci, ci, vp, vp, vp, ci, ci]),
+    "xtbeig_dev_sback_band": (ci, [vp, ci, ci, vp, vp, vp, ci, ci]),
+}
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run xtb-diagonalization_amd/build.py (no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(vp)
+
+
+def last_error():
+    return lib().xtbeig_last_error().decode()
+
+
+# ---- LAPACK-style host calls -----------------------------------------------------------------------------------
+
+def _f64(a):
+    return np.array(a, dtype=np.float64, order="F", copy=True)
+
+
+def dsygvd(h, s, mixed=False, uplo="U"):
+    """Generalized eigenpairs of (h, s) through xtbeig_dsygvd[_mixed]. Returns (w, c, u, info): c has S-orthonormal
+    columns, u is s's Cholesky factor as LAPACK leaves it in b."""
+    a = _f64(h)
+    b = _f64(s)
+    n = a.shape[0]
+    w = np.zeros(n)
+    lw = 1 + 6 * n + 2 * n * n
+    liw = 3 + 5 * n
+    work = np.zeros(max(lw, 1))
+    iwork = np.zeros(max(liw, 1), dtype=np.int32)
+    info = ctypes.c_int(0)
+    f = lib().xtbeig_dsygvd_mixed if mixed else lib().xtbeig_dsygvd
+    f(_i(1), b"V", uplo.encode(), _i(n), _ptr(a), _i(n), _ptr(b), _i(n), _ptr(w), _ptr(work), _i(lw), _ptr(iwork),
+      _i(liw), ctypes.byref(info))
+    return w, a, b, info.value
+
+
+def syevd(a_in, jobz="V", uplo="U", dtype=np.float64):
+    a = np.array(a_in, dtype=dtype, order="F", copy=True)
+    n = a.shape[0]
+    w = np.zeros(n, dtype=dtype)
+    lw = 1 + 6 * n + 2 * n * n
+    liw = 3 + 5 * n
+    work = np.zeros(max(lw, 1), dtype=dtype)
+    iwork = np.zeros(max(liw, 1), dtype=np.int32)
+    info = ctypes.c_int(0)
+    f = lib().xtbeig_dsyevd if dtype == np.float64 else lib().xtbeig_ssyevd
+    f(jobz.encode(), uplo.encode(), _i(n), _ptr(a), _i(n), _ptr(w), _ptr(work), _i(lw), _ptr(iwork), _i(liw),
+      ctypes.byref(info))
+    return w, a, info.value
+
+
+def stedc(d_in, e_in, dtype=np.float64):
+    d = np.array(d_in, dtype=dtype, copy=True)
+    n = d.shape[0]
+    e = np.zeros(max(n, 1), dtype=dtype)
+    e[: max(n - 1, 0)] = e_in
+    z = np.zeros((n, n), dtype=dtype, order="F")
+    lw = 1 + 4 * n + n * n
+    liw = 3 + 5 * n
+    work = np.zeros(max(lw, 1), dtype=dtype)
+    iwork = np.zeros(max(liw, 1), dtype=np.int32)
+    info = ctypes.c_int(0)
+    f = lib().xtb_dstedc if dtype == np.float64 else lib().xtb_sstedc
+    f(_i(n), _ptr(d), _ptr(e), _ptr(z), _i(max(n, 1)), _ptr(work), _i(lw), _ptr(iwork), _i(liw), ctypes.byref(info))
+    return d, z, info.value
+
+
+def dsymmetrize_pad(src, ldd=None):
+    src = np.asfortranarray(src, dtype=np.float64)
+    n = src.shape[0]
+    ldd = ldd or n
+    dst = np.zeros((ldd, n), order="F")
+    lib().xtb_dsymmetrize_pad(_i(n), _ptr(src), _i(src.shape[0]), _ptr(dst), _i(ldd))
+    return dst[:n, :n]
+
+
+# ---- device-resident calls ---------------------------------------------------------------------------------------
+
+class Handle:
+    """A device handle; run on the given torch stream (default: the handle's own stream)."""
+
+    def __init__(self, device=0, stream=None):
+        self._h = vp()
+        if lib().xtbeig_create(ctypes.byref(self._h), device) != 0:
+            raise RuntimeError(last_error())
+        if stream is not None:
+            self.set_stream(stream)
+
+    def close(self):
+        if self._h:
+            lib().xtbeig_destroy(self._h)
+            self._h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(last_error())
+
+    def set_stream(self, stream):
+        raw = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+        self._check(lib().xtbeig_set_stream(self._h, vp(raw)))
+
+    def stream_ptr(self):
+        return lib().xtbeig_get_stream(self._h)
+
+    def set_tuning(self, band=0, refine_iterations=2):
+        self._check(lib().xtbeig_set_tuning(self._h, band, refine_iterations))
+
+    def synchronize(self):
+        st = (ctypes.c_int * 3)()
+        rc = lib().xtbeig_synchronize(self._h, st)
+        if rc == 1:
+            raise RuntimeError(last_error())
+        return list(st)
+
+    def sygvd(self, h, s, w, c, precision=FP64):
+        n = h.shape[0]
+        self._check(lib().xtbeig_dev_sygvd(self._h, precision, n, vp(h.data_ptr()), h.stride(0),
+                                           vp(s.data_ptr()), s.stride(0), vp(w.data_ptr()), vp(c.data_ptr()),
+                                           c.stride(0)))
+
+    def dsyevd(self, a, w, z):
+        n = a.shape[0]
+        self._check(lib().xtbeig_dev_dsyevd(self._h, n, vp(a.data_ptr()), n, vp(w.data_ptr()), vp(z.data_ptr()), n))
+
+    def ssyevd(self, a, w, z):
+        n = a.shape[0]
+        self._check(lib().xtbeig_dev_ssyevd(self._h, n, vp(a.data_ptr()), n, vp(w.data_ptr()), vp(z.data_ptr()), n))
+
+    def dstedc(self, d, e, z):
+        n = d.shape[0]
+        self._check(lib().xtbeig_dev_dstedc(self._h, n, vp(d.data_ptr()), vp(e.data_ptr()), vp(z.data_ptr()), n))
+
+    def gemm(self, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+        import torch
+        f = lib().xtbeig_dev_dgemm if c.dtype == torch.float64 else lib().xtbeig_dev_sgemm
+        self._check(f(self._h, ta.encode(), tb.encode(), m, n, k, alpha, vp(a.data_ptr()), lda, vp(b.data_ptr()),
+                      ldb, beta, vp(c.data_ptr()), ldc))
+
+    def potrf_inverse(self, s, u, x):
+        n = s.shape[0]
+        self._check(lib().xtbeig_dev_potrf_inverse(self._h, n, vp(s.data_ptr()), n, vp(u.data_ptr()), n,
+                                                   vp(x.data_ptr()), n))
+
+    def reduce_to_band(self, a, b, v1, t1, ab):
+        import torch
+        n = a.shape[0]
+        f = lib().xtbeig_dev_dreduce_to_band if a.dtype == torch.float64 else lib().xtbeig_dev_sreduce_to_band
+        self._check(f(self._h, n, b, vp(a.data_ptr()), n, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(ab.data_ptr())))
+
+    def chase(self, n, b, ab, v2, d, e):
+        import torch
+        f = lib().xtbeig_dev_dchase if ab.dtype == torch.float64 else lib().xtbeig_dev_schase
+        self._check(f(self._h, n, b, vp(ab.data_ptr()), vp(v2.data_ptr()), vp(d.data_ptr()), vp(e.data_ptr())))
+
+    def back_bulges(self, n, b, v2, z, m):
+        import torch
+        f = lib().xtbeig_dev_dback_bulges if z.dtype == torch.float64 else lib().xtbeig_dev_sback_bulges
+        self._check(f(self._h, n, b, vp(v2.data_ptr()), vp(z.data_ptr()), n, m))
+
+    def back_band(self, n, b, v1, t1, z, m):
+        import torch
+        f = lib().xtbeig_dev_dback_band if z.dtype == torch.float64 else lib().xtbeig_dev_sback_band
+        self._check(f(self._h, n, b, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(z.data_ptr()), n, m))
+
+
+def band_elems(n, b):
+    return int(lib().xtbeig_band_elems(n, b))
+
+
+def band_to_dense(ab, n, b):
+    """Band storage (element (i, j), 0 <= i - j < 2b, at i + (2b - 1) j) -> dense symmetric numpy matrix."""
+    ld = 2 * b - 1 if b > 1 else 1
+    a = np.zeros((n, n))
+    flat = np.asarray(ab).ravel()
+    for j in range(n):
+        for dd in range(min(ld + 1, n - j)):
+            i = j + dd
+            v = flat[i + ld * j]
+            a[i, j] = v
+            a[j, i] = v
+    return a
