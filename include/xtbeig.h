@@ -121,6 +121,15 @@ int xtbeig_dev_dback_band(xtbeig_handle_t handle, int n, int b, const double *v1
 int xtbeig_dev_sback_band(xtbeig_handle_t handle, int n, int b, const float *v1, const float *t1, float *z, int ldz,
                           int m);
 
+/* One divide-and-conquer merge (dlaed1 semantics; reference: Stages/ParallelDivideAndConquer.swift:77-124) of two
+ * solved halves: d (n) ascending within rows 0 ..< n1 and n1 ..< n, q (n x n, ldq) block diagonal with their
+ * eigenvectors, rho the coupling. On return d and q hold the merged eigensystem (ascending). dlaed2's outputs are
+ * exposed for checking: k_ctot = K and the four column-type counts, d_deflated = d after deflation, q2 (n^2),
+ * dlamda (K), w (K), indxc (n, 1-based), rho_out = |2 rho|. */
+int xtbeig_dev_dc_merge(xtbeig_handle_t handle, int n, int n1, double rho, double *d, double *q, int ldq,
+                        double *d_deflated, double *q2, int *k_ctot, double *dlamda, double *w, int *indxc,
+                        double *rho_out);
+
 #ifdef __cplusplus
 }
 #endif

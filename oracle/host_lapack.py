@@ -86,7 +86,7 @@ def dlaed2(n1, d, q, rho, z, indxq):
     indx = np.zeros(n, dtype=np.int32)
     indxc = np.zeros(n, dtype=np.int32)
     indxp = np.zeros(n, dtype=np.int32)
-    coltyp = np.zeros(n, dtype=np.int32)
+    coltyp = np.zeros(max(n, 4), dtype=np.int32)  # dlaed2 returns the four counts in COLTYP(1:4)
     k = ctypes.c_int(0)
     r = ctypes.c_double(rho)
     info = ctypes.c_int(0)

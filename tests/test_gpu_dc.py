@@ -1,0 +1,88 @@
+"""Divide-and-conquer merge parity: the GPU's deflation bookkeeping equals LAPACK dlaed2's on the same inputs
+(K, the column-type counts, the INDXC permutation, the secular poles and weights, the deflated eigenvalues), and the
+merged eigensystem is the eigendecomposition of the rank-one-modified matrix."""
+import numpy as np
+import pytest
+from scipy.stats import ortho_group
+
+import host_lapack as hl
+import xtbeig
+
+pytestmark = pytest.mark.gpu
+
+
+def make_merge(n, n1, kind, seed):
+    rng = np.random.default_rng(seed)
+    n2 = n - n1
+    d1 = np.sort(rng.uniform(-1, 1, n1))
+    d2 = np.sort(rng.uniform(-1, 1, n2))
+    if kind == "repeated":
+        # some eigenvalues shared exactly across the halves -> Givens deflation
+        m = min(n1, n2) // 3
+        d2[:m] = d1[:m]
+        d2 = np.sort(d2)
+    q1 = ortho_group.rvs(n1, random_state=seed) if n1 > 1 else np.ones((1, 1))
+    q2 = ortho_group.rvs(n2, random_state=seed + 1) if n2 > 1 else np.ones((1, 1))
+    if kind == "smallz":
+        # last row of Q1 / first row of Q2 nearly axis-aligned -> many tiny z components
+        q1 = np.eye(n1)[:, ::-1].copy()
+        q1[:, :2] = ortho_group.rvs(n1, random_state=seed)[:, :2] if n1 > 2 else q1[:, :2]
+        q1, _ = np.linalg.qr(q1)
+        q2 = np.eye(n2)
+    q = np.zeros((n, n))
+    q[:n1, :n1] = q1
+    q[n1:, n1:] = q2
+    d = np.concatenate([d1, d2])
+    rho = rng.uniform(0.2, 0.8) * (1 if seed % 2 else -1)
+    return d, np.asfortranarray(q), rho
+
+
+@pytest.mark.parametrize("kind", ["random", "repeated", "smallz"])
+@pytest.mark.parametrize("n,n1", [(2, 1), (3, 1), (9, 4), (64, 32), (301, 150), (1204, 602)])
+def test_merge_matches_dlaed2(handle, torch_cuda, kind, n, n1):
+    torch = torch_cuda
+    d, q, rho = make_merge(n, n1, kind, n + 7)
+    z = np.concatenate([q[n1 - 1, :n1], q[n1, n1:]])
+    indxq = np.concatenate([np.arange(1, n1 + 1), np.arange(1, n - n1 + 1)]).astype(np.int32)
+    ref = hl.dlaed2(n1, d, q, rho, z, indxq)
+    D = torch.from_numpy(d.copy()).cuda()
+    Q = torch.from_numpy(np.ascontiguousarray(q.T)).cuda()
+    out = xtbeig.dc_merge(handle, D, Q, n1, rho)
+    assert out["status"] == [0, 0, 0]
+    k = ref["k"]
+    assert out["k"] == k
+    assert list(out["ctot"]) == list(ref["ctot"])
+    assert out["rho"] == pytest.approx(ref["rho"], rel=1e-15)
+    if k > 0:
+        assert np.array_equal(out["indxc"], ref["indxc"])
+        # values agree to rounding (Givens-deflated poles are recombined in floating point)
+        assert np.max(np.abs(out["dlamda"] - ref["dlamda"][:k])) <= 8e-16 * np.max(np.abs(d))
+        assert np.max(np.abs(out["w"] - ref["w"][:k])) <= 8e-16
+    assert np.max(np.abs(out["d_deflated"][k:] - ref["d"][k:]), initial=0) <= 8e-16 * np.max(np.abs(d))
+    # merged eigensystem of the torn matrix M = Q diag(d) Q^T + |rho| u u^T, u = e_{n1 - 1} + sign(rho) e_{n1}
+    # (off-diagonal coupling rho, the |rho| taken off both diagonal entries by the tear)
+    v = np.zeros(n)
+    v[n1 - 1] = 1
+    v[n1] = np.sign(rho)
+    m = (q * d) @ q.T + abs(rho) * np.outer(v, v)
+    w, qm = out["d"], out["q"]
+    assert np.all(np.diff(w) >= 0)
+    assert np.max(np.abs(w - np.linalg.eigvalsh(m))) < 1e-14 * n
+    assert np.linalg.norm(m @ qm - qm * w) < 1e-14 * n
+    assert np.max(np.abs(qm.T @ qm - np.eye(n))) < 1e-14 * n
+
+
+@pytest.mark.parametrize("n", [3, 20, 300])
+def test_secular_roots_match_dlaed4(handle, torch_cuda, n):
+    """With no deflation the merged eigenvalues are exactly the secular roots: compare each with host dlaed4."""
+    torch = torch_cuda
+    d, q, rho = make_merge(n, n // 2, "random", 3 * n)
+    z = np.concatenate([q[n // 2 - 1, : n // 2], q[n // 2, n // 2:]])
+    indxq = np.concatenate([np.arange(1, n // 2 + 1), np.arange(1, n - n // 2 + 1)]).astype(np.int32)
+    ref = hl.dlaed2(n // 2, d, q, rho, z, indxq)
+    k = ref["k"]
+    roots = np.array([hl.dlaed4(ref["dlamda"][:k], ref["w"][:k], i, ref["rho"])[0] for i in range(k)])
+    out = xtbeig.dc_merge(handle, torch.from_numpy(d.copy()).cuda(),
+                          torch.from_numpy(np.ascontiguousarray(q.T)).cuda(), n // 2, rho)
+    allw = np.sort(np.concatenate([roots, ref["d"][k:]]))
+    assert np.max(np.abs(out["d"] - allw)) <= 2e-15 * max(1.0, np.max(np.abs(allw)))

@@ -260,6 +260,20 @@ int xtbeig_dev_sback_band(xtbeig_handle_t handle, int n, int b, const float* v1,
   return back_band_impl<float>(handle, n, b, v1, t1, z, ldz, m);
 }
 
+int xtbeig_dev_dc_merge(xtbeig_handle_t handle, int n, int n1, double rho, double* d, double* q, int ldq,
+                        double* d_deflated, double* q2, int* k_ctot, double* dlamda, double* w, int* indxc,
+                        double* rho_out) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve((xtbeig::dc_workspace_doubles(n) + 64) * sizeof(double) + (xtbeig::dc_workspace_ints(n) + 64) * 4 +
+              8192);
+    double* work = static_cast<double*>(h.arena);
+    int* iwork = reinterpret_cast<int*>(work + xtbeig::dc_workspace_doubles(n) + 64);
+    xtbeig::dc_merge_probe(h.stream, n, n1, rho, d, q, ldq, work, iwork, h.dinfo + 2, d_deflated, q2, k_ctot, dlamda,
+                           w, indxc, rho_out);
+  });
+}
+
 // ---- host-memory entry points -----------------------------------------------------------------------------------
 
 void xtb_dstedc(const int* n_, double* d, double* e, double* z, const int* ldz_, double* work, const int* lwork,

@@ -97,6 +97,7 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
                                                       double* __restrict__ dlamda_all, double* __restrict__ w_all,
                                                       double* __restrict__ q2_all, int* __restrict__ indxc_all,
                                                       NodeOut* __restrict__ out) {
+#pragma clang fp contract(off)  // dlaed2's scalar recurrences rounded as written (no fused multiply-add)
   extern __shared__ char smem[];
   const Node nd = nodes[blockIdx.x];
   const int n = nd.sz, n1 = nd.n1, n2 = n - n1, off = nd.off;
@@ -560,6 +561,62 @@ void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, i
     XTB_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(unscale_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, scale);
+  XTB_LAUNCH_CHECK();
+}
+
+}  // namespace xtbeig
+
+namespace xtbeig {
+
+// One merge (dlaed1 semantics) of two solved halves on the device, exposing dlaed2's outputs for comparison with the
+// host LAPACK: d (n) ascending within each half, q (n x n, ldq) block diagonal with the halves' eigenvectors.
+// kctot_out: K, ctot[0..3]; d_defl_out: d after deflation (the deflated eigenvalues in K ..< n); q2_out: Q2 (n^2).
+void dc_merge_probe(hipStream_t s, int n, int n1, double rho, double* d, double* q, int ldq, double* work,
+                    int* iwork, int* info, double* d_defl_out, double* q2_out, int* kctot_out, double* dlamda_out,
+                    double* w_out, int* indxc_out, double* rho_out) {
+  if (n < 2 || n1 < 1 || n1 >= n) throw std::runtime_error("dc_merge_probe: need 1 <= n1 < n");
+  double* dlamda = work;
+  double* w = dlamda + n;
+  double* tau = w + n;
+  double* wnew = tau + n;
+  double* dtmp = wnew + n;
+  double* rho_pre = dtmp + 2 * (size_t)n;
+  double* rho_lvl = rho_pre + n;
+  double* q2 = rho_lvl + n + 16;
+  double* sbuf = q2 + (size_t)n * n;
+  int* orig = iwork;
+  int* indxc = orig + n;
+  int* perm = indxc + n;
+  NodeOut* outs = reinterpret_cast<NodeOut*>(perm + n);
+  int* pre_index = reinterpret_cast<int*>(outs + 1);
+  Node* node = reinterpret_cast<Node*>(pre_index + 4);
+  const Node host_node{0, n, n1, 0};
+  const int zero = 0;
+  XTB_HIP(hipMemcpy(node, &host_node, sizeof(Node), hipMemcpyHostToDevice));
+  XTB_HIP(hipMemcpy(pre_index, &zero, sizeof(int), hipMemcpyHostToDevice));
+  XTB_HIP(hipMemcpy(rho_pre, &rho, sizeof(double), hipMemcpyHostToDevice));
+  const size_t lds = (size_t)n * (6 * sizeof(double) + 7 * sizeof(int)) + 64;
+  allow_lds(deflate_kernel, lds);
+  hipLaunchKernelGGL(deflate_kernel, dim3(1), dim3(256), lds, s, node, pre_index, n, d, q, ldq, rho_pre, rho_lvl,
+                     dlamda, w, q2, indxc, outs);
+  XTB_LAUNCH_CHECK();
+  XTB_HIP(hipMemcpyAsync(d_defl_out, d, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  XTB_HIP(hipMemcpyAsync(q2_out, q2, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  XTB_HIP(hipMemcpyAsync(kctot_out, outs, sizeof(int) * 5, hipMemcpyDeviceToDevice, s));
+  XTB_HIP(hipMemcpyAsync(dlamda_out, dlamda, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  XTB_HIP(hipMemcpyAsync(w_out, w, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  XTB_HIP(hipMemcpyAsync(indxc_out, indxc, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+  XTB_HIP(hipMemcpyAsync(rho_out, rho_lvl, sizeof(double), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(secular_kernel, dim3(cdiv(n, 64), 1), dim3(64), 0, s, node, outs, rho_lvl, dlamda, w, indxc, d,
+                     orig, tau, sbuf, info);
+  hipLaunchKernelGGL(gu_kernel, dim3(cdiv(n, 64), 1), dim3(64), 0, s, node, outs, dlamda, w, orig, tau, wnew);
+  hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(n, 4), 1), dim3(256), 0, s, node, outs, dlamda, orig, tau, wnew,
+                     indxc, sbuf);
+  hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(n, 64), cdiv(n, 64), 2), dim3(256), 0, s, node, outs, q2, sbuf, q,
+                     ldq);
+  hipLaunchKernelGGL(lamrg_kernel, dim3(1), dim3(64), 0, s, 1, node, outs, d, perm);
+  hipLaunchKernelGGL(gather_kernel, dim3(cdiv(n, 256), n, 1), dim3(256), 0, s, node, perm, q, ldq, d, sbuf, dtmp);
+  hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(n, 256), n, 1), dim3(256), 0, s, node, sbuf, dtmp, q, ldq, d);
   XTB_LAUNCH_CHECK();
 }
 

@@ -1,8 +1,8 @@
 // The MFMA GEMM tile shared by every matrix-product kernel (gemm.hip, and the divide-and-conquer merge products
 // whose sizes are only known on the device). 64 x 64 tile of C per workgroup of 256 threads: 4 wavefronts in a 2 x 2
 // grid, 32 x 32 each = 2 x 2 MFMA blocks of 16 x 16; K staged through LDS 16 at a time with the next slab prefetched
-// into registers. MFMA operand layout (16x16x4): lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16);
-// accumulator register r of lane l is C(4 (l / 16) + r, l % 16).
+// into registers. MFMA operand layout (16x16x4): lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16); the
+// accumulator layout differs between FP32 and FP64 (Mfma<T>::row).
 #pragma once
 #include "common.hpp"
 
@@ -12,17 +12,21 @@ namespace gemm_detail {
 constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
 
 template <typename T> struct Mfma;
+// Accumulator layout: register r of lane l holds C(row(l, r), l % 16). The FP32 16x16x4 instruction places rows
+// 4 (l / 16) + r; the FP64 one places rows l / 16 + 4 r (each of its four 64-bit accumulators is a 4 x 16 slab).
 template <> struct Mfma<double> {
   typedef double acc_t __attribute__((ext_vector_type(4)));
   static __device__ inline acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct Mfma<float> {
   typedef float acc_t __attribute__((ext_vector_type(4)));
   static __device__ inline acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 // Loads of one K slab: 4 elements of op(A) (64 x 16) and 4 of op(B) (16 x 64) per thread.
@@ -109,7 +113,7 @@ __device__ inline void gemm_tile(int M, int N, int K, T alpha, const T* __restri
       if (col >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+        const int row = m0 + wm * 32 + bi * 16 + Mfma<T>::row(lane, r);
         if (row < M) {
           T* p = C + idx(row, col, ldc);
           const T v = alpha * acc[bi][bj][r];

@@ -78,6 +78,9 @@ size_t dc_workspace_ints(int n);
 // info: device int, set to a positive value when a secular equation fails to converge.
 void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, int ldz, double* work, int* iwork,
            int* info);
+void dc_merge_probe(hipStream_t s, int n, int n1, double rho, double* d, double* q, int ldq, double* work,
+                    int* iwork, int* info, double* d_defl_out, double* q2_out, int* kctot_out, double* dlamda_out,
+                    double* w_out, int* indxc_out, double* rho_out);
 
 }  // namespace xtbeig
 

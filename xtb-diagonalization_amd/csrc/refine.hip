@@ -2,11 +2,12 @@
 // the S inner product, and the eigenpair sort. The reference refines FP32 eigenpairs in FP64 (scripts/kernels/
 // fp32_error.c:95-158; the S-inner-product variant in scripts/kernels/refine.f90:254-325):
 //     R = I - C^T S C,  A = C^T H C,  lambda_i = A_ii / (1 - R_ii),
-//     E_ij = (A_ij + lambda_j R_ij) / (lambda_j - lambda_i)  for |lambda_j - lambda_i| > delta,  E_ij = R_ij / 2 else,
+//     E_ij = (A_ij + lambda_j R_ij) / (lambda_j - lambda_i) for resolved pairs i != j,  E_ii = R_ii / 2,
 //     C <- C + C E.
-// The four products and the update run as FP64 GEMMs on the matrix cores. delta follows the size of the current
-// residual: pairs closer than 64 max|R| max|lambda| are treated as a cluster (their subspace converges; the basis
-// inside it is not refined), so near-degenerate pairs separate as the residual shrinks.
+// The four products and the update run as FP64 GEMMs on the matrix cores. A pair is treated as a cluster (E_ij =
+// R_ij / 2: its subspace converges, the basis inside it is not rotated) when its Newton correction would not be
+// small, |E_ij| > 1/4, which covers exactly and nearly degenerate pairs (the gap below what the current accuracy
+// resolves); as the residual shrinks, near-degenerate pairs separate.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -39,13 +40,11 @@ __global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, c
                                  double* __restrict__ e, int lde) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= n) return;
-  const double rmax = __longlong_as_double(stats[0]), lmax = __longlong_as_double(stats[1]);
-  const double delta = 64.0 * rmax * fmax(lmax, 1e-300);
+  (void)stats;
   const double rij = r[idx(i, j, ldr)];
   const double gap = lam[j] - lam[i];
-  double v;
-  if (i != j && fabs(gap) > delta) v = (a[idx(i, j, lda)] + lam[j] * rij) / gap;
-  else v = 0.5 * rij;
+  double v = (a[idx(i, j, lda)] + lam[j] * rij) / gap;
+  if (i == j || !(fabs(v) <= 0.25)) v = 0.5 * rij;
   e[idx(i, j, lde)] = v;
 }
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "xtbeig_dev_schase": (ci, [vp, ci, ci, vp, vp, vp, vp]),
     "xtbeig_dev_dback_bulges": (ci, [vp, ci, ci, vp, vp, ci, ci]),
     "xtbeig_dev_sback_bulges": (ci, [vp, ci, ci, vp, vp, ci, ci]),
+    "xtbeig_dev_dc_merge": (ci, [vp, ci, ci, ctypes.c_double, vp, vp, ci, vp, vp, vp, vp, vp, vp, vp]),
     "xtbeig_dev_dback_band": (ci, [vp, ci, ci, vp, vp, vp, ci, ci]),
     "xtbeig_dev_sback_band": (ci, [vp, ci, ci, vp, vp, vp, ci, ci]),
 }
@@ -251,6 +252,29 @@ class Handle:
         import torch
         f = lib().xtbeig_dev_dback_band if z.dtype == torch.float64 else lib().xtbeig_dev_sback_band
         self._check(f(self._h, n, b, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(z.data_ptr()), n, m))
+
+
+def dc_merge(handle, d, q, n1, rho):
+    """One device merge on torch tensors d (n), q (n x n column-major as (n, n) row-major of q^T). Returns dlaed2's
+    outputs and the merged (d, q) as numpy arrays."""
+    import torch
+    n = d.shape[0]
+    dd = torch.empty(n, dtype=torch.float64, device=d.device)
+    q2 = torch.zeros(n * n, dtype=torch.float64, device=d.device)
+    kc = torch.zeros(5, dtype=torch.int32, device=d.device)
+    dl = torch.zeros(n, dtype=torch.float64, device=d.device)
+    w = torch.zeros(n, dtype=torch.float64, device=d.device)
+    ix = torch.zeros(n, dtype=torch.int32, device=d.device)
+    ro = torch.zeros(1, dtype=torch.float64, device=d.device)
+    handle._check(lib().xtbeig_dev_dc_merge(handle._h, n, n1, float(rho), vp(d.data_ptr()), vp(q.data_ptr()), n,
+                                            vp(dd.data_ptr()), vp(q2.data_ptr()), vp(kc.data_ptr()),
+                                            vp(dl.data_ptr()), vp(w.data_ptr()), vp(ix.data_ptr()),
+                                            vp(ro.data_ptr())))
+    st = handle.synchronize()
+    k = int(kc[0])
+    return dict(k=k, ctot=kc[1:].cpu().numpy(), d_deflated=dd.cpu().numpy(), q2=q2.cpu().numpy(),
+                dlamda=dl.cpu().numpy()[:k], w=w.cpu().numpy()[:k], indxc=ix.cpu().numpy(), rho=float(ro[0]),
+                d=d.cpu().numpy(), q=q.cpu().numpy().T, status=st)
 
 
 def band_elems(n, b):
