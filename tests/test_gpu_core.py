@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 import scipy.linalg as sl
 
+import sbr
 import workloads
 import xtbeig
 
@@ -75,6 +76,16 @@ def test_reduce_to_band(handle, torch_cuda, prec, b, n):
     wb = np.linalg.eigvalsh(band)
     tol = 1e-13 if prec == "d" else 3e-6
     assert np.max(np.abs(wb - lam)) < tol
+    # same algorithm as the oracle's restatement of BandForm.swift: same band to rounding
+    oband, _, _ = sbr.reduce_to_band(a, b)
+    assert np.max(np.abs(band - oband)) < (1e-12 if prec == "d" else 2e-5) * n
+    # back-transformation through the panels: Q1 = (back_band of I), Q1^T A Q1 = band
+    Z = torch.eye(n, dtype=dt, device="cuda")
+    handle.back_band(n, b, V1, T1, Z, n)
+    handle.synchronize()
+    q1 = host(Z).astype(np.float64)
+    assert np.max(np.abs(q1.T @ q1 - np.eye(n))) < (1e-13 if prec == "d" else 3e-6) * n
+    assert np.max(np.abs(q1.T @ a @ q1 - band)) < (1e-13 if prec == "d" else 3e-6) * n
 
 
 @pytest.mark.parametrize("prec,b,n", [("d", 32, 260), ("d", 8, 131), ("s", 16, 200), ("d", 3, 40), ("d", 64, 300)])
@@ -98,6 +109,9 @@ def test_chase(handle, torch_cuda, prec, b, n):
     handle.chase(n, b, AB, V2, D, E)
     assert handle.synchronize() == [0, 0, 0]
     d, e = D.double().cpu().numpy(), E.double().cpu().numpy()
+    od, oe, _ = sbr.chase_bulges(full, b)
+    assert np.max(np.abs(d - od)) < (1e-13 if prec == "d" else 1e-4) * n
+    assert np.max(np.abs(np.abs(e[: n - 1]) - np.abs(oe))) < (1e-13 if prec == "d" else 1e-4) * n
     wt = sl.eigvalsh_tridiagonal(d, e[: n - 1])
     wref = np.linalg.eigvalsh(full)
     tol = 1e-12 if prec == "d" else 2e-4

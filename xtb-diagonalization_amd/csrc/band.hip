@@ -1,15 +1,14 @@
 // Stage 1 of the two-stage reduction: dense symmetric -> lower band of width b (the reference's reduceToBand with the
-// combined sy2sb update, Stages/BandForm.swift:28-142), and its back-transformation (BandFormBackTransformation.swift
-// :10-21).
+// combined sy2sb update, Stages/BandForm.swift:28-142); the back-transformation through its panels is in wy.hip.
 //
 // Per panel j (columns j ..< j + b, rows r = j + b ..< n, m = n - r, k = min(b, m)):
-//   1. panel_qr_kernel: Householder QR of the m x b panel by one workgroup of 1024 threads. The panel is processed
+//   1. panel_qr_kernel: Householder QR of the m x b panel by one workgroup of 256 threads. The panel is processed
 //      in sub-panels of SB columns held in registers (R rows per thread), earlier reflectors applied left-looking.
 //      Writes R into the band (zeros below), V explicitly (unit diagonal, zeros above) into the back-transform store
 //      v1 and into the update buffer [V | . | V], and tau.
-//   2. G = V^T V (GEMM), then larft_kernel builds the forward columnwise T (I - V T V^T = H_1 ... H_k).
-//   3. VT = V T; X = A2 VT; Y = VT^T X; W = X - 1/2 V Y (GEMMs); A2 -= [V W] [W V]^T, one rank-2k GEMM on the
-//      matrix cores (full storage: both triangles of the trailing block stay current).
+//   2. panel_t_kernel: G = V^T V, the forward columnwise T (I - V T V^T = H_1 ... H_k) and VT = V T.
+//   3. X = A2 VT (GEMM); panel_finish_kernel: Y = VT^T X, W = X - 1/2 V Y; A2 -= [V W] [W V]^T, one rank-2k GEMM
+//      on the matrix cores (full storage: both triangles of the trailing block stay current).
 // After the last panel the lower band is copied into the chase's band storage (element (i, j), 0 <= i - j <= 2b - 1,
 // at i + (2b - 1) j; see chase.hip).
 #include "common.hpp"
@@ -19,33 +18,33 @@ namespace xtbeig {
 
 namespace {
 
-constexpr int QR_THREADS = 1024;
+constexpr int QR_THREADS = 256;
 
-// Block-wide sums of NV values per thread; the totals land in tot[0 .. NV). red: 16 * NV scratch.
+// Block-wide sums of NV values per thread (4 wavefronts); the totals land in tot[0 .. NV). red: 4 * NV scratch.
 template <typename T, int NV>
 __device__ inline void block_sums(const T (&v)[NV], T* red, T* tot) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < NV; ++c) {
     const T s = wave_sum(v[c]);
     if (lane == 0) red[wid * NV + c] = s;
   }
   __syncthreads();
-  if (threadIdx.x < NV) {
-    T s = 0;
-    for (int w = 0; w < nw; ++w) s += red[w * NV + threadIdx.x];
-    tot[threadIdx.x] = s;
-  }
+  if (threadIdx.x < NV) tot[threadIdx.x] = (red[threadIdx.x] + red[NV + threadIdx.x]) +
+                                           (red[2 * NV + threadIdx.x] + red[3 * NV + threadIdx.x]);
   __syncthreads();
 }
 
-// Householder QR of the panel p (m x nb, ld lda; nb <= b). R overwrites the upper triangle of p's first min(m, nb)
-// rows, zeros below. V (explicit) goes to v1 (ld ldv1) and to buf (ld m) at column offsets 0 and 2k.
+// Householder QR of the panel p (m x nb, ld lda; nb <= 64) by one workgroup of 256 threads: row i = t + 256 r of
+// the current sub-panel of SB columns lives in registers of thread t (R rows per thread, m <= 256 R). Earlier
+// sub-panels' reflectors are applied left-looking (re-read from v1). R overwrites the upper triangle of p's first
+// min(m, nb) rows, zeros below. V (explicit: unit diagonal, zeros above) goes to v1 (ld ldv1) and to buf (ld m) at
+// column offsets 0 and 2k; tau to tau.
 template <typename T, int SB, int R>
 __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* __restrict__ p, int lda,
                                                               T* __restrict__ v1, int ldv1, T* __restrict__ buf,
                                                               T* __restrict__ tau) {
-  __shared__ T red[16 * SB];
+  __shared__ T red[4 * SB];
   __shared__ T tot[SB];
   __shared__ T sh_alpha;
   const int t = threadIdx.x;
@@ -53,7 +52,6 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
   T x[R][SB];
   for (int c0 = 0; c0 < nb; c0 += SB) {
     const int sb = min(SB, nb - c0);
-    // load the sub-panel (all nb - c0 columns up to SB; columns >= k still receive the reflectors)
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -61,7 +59,6 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
         const int i = t + r * QR_THREADS;
         x[r][c] = (i < m && c < sb) ? p[idx(i, c0 + c, lda)] : T(0);
       }
-    // apply reflectors 0 .. c0 - 1 (stored in v1) left-looking
     for (int q = 0; q < min(c0, k); ++q) {
       const T tq = tau[q];
       T vq[R];
@@ -86,12 +83,10 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
         for (int r = 0; r < R; ++r) x[r][c] -= vq[r] * f;
       }
     }
-    // factor the sub-panel column by column
 #pragma unroll
     for (int c = 0; c < SB; ++c) {
       const int col = c0 + c;
       if (col < k) {
-        // alpha = x(col, c); xnorm^2 = sum over rows > col
         T part1[1];
         part1[0] = 0;
 #pragma unroll
@@ -110,17 +105,15 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
           tq = (beta - alpha) / beta;
           scal = T(1) / (alpha - beta);
         }
-        // v in column c: 1 at col, scaled below, R entries above kept
         T v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int i = t + r * QR_THREADS;
           v[r] = (i == col) ? T(1) : ((i > col && i < m) ? x[r][c] * scal : T(0));
           if (i == col) x[r][c] = beta;
-          else if (i > col) x[r][c] = 0;  // R below the diagonal
+          else if (i > col) x[r][c] = 0;
         }
         if (t == 0) tau[col] = tq;
-        // store V column explicitly
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int i = t + r * QR_THREADS;
@@ -130,29 +123,29 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
             buf[idx(i, 2 * k + col, m)] = v[r];
           }
         }
-        // apply H = I - tau v v^T to the remaining sub-panel columns
-        T part[SB];
+        if (c + 1 < SB) {
+          T part[SB];
 #pragma unroll
-        for (int cc = 0; cc < SB; ++cc) {
-          T s = 0;
-          if (cc > c) {
+          for (int cc = 0; cc < SB; ++cc) {
+            T s = 0;
+            if (cc > c) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) s += v[r] * x[r][cc];
+              for (int r = 0; r < R; ++r) s += v[r] * x[r][cc];
+            }
+            part[cc] = s;
           }
-          part[cc] = s;
-        }
-        block_sums<T, SB>(part, red, tot);
+          block_sums<T, SB>(part, red, tot);
 #pragma unroll
-        for (int cc = 0; cc < SB; ++cc) {
-          if (cc > c) {
-            const T f = tq * tot[cc];
+          for (int cc = 0; cc < SB; ++cc) {
+            if (cc > c) {
+              const T f = tq * tot[cc];
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[r][cc] -= v[r] * f;
+              for (int r = 0; r < R; ++r) x[r][cc] -= v[r] * f;
+            }
           }
         }
       }
     }
-    // write back the sub-panel (R part and zeros; columns beyond k keep their reflected values)
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -160,31 +153,72 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
         const int i = t + r * QR_THREADS;
         if (i < m && c < sb) p[idx(i, c0 + c, lda)] = x[r][c];
       }
-    __syncthreads();  // v1 columns of this sub-panel visible to the next one
+    __syncthreads();  // v1 and tau of this sub-panel visible to the next one
   }
 }
 
-// T (k x k, upper, ld ldt) of H_1 ... H_k = I - V T V^T from G = V^T V (k x k, ld k) and tau.
+// After the panel QR, one workgroup: G = V^T V, the forward columnwise T (I - V T V^T = H_1 ... H_k, stored in tt, ld
+// ldt), and VT = V T (m x k, ld m). v: m x k explicit (ld m), tau (k).
 template <typename T>
-__global__ void larft_kernel(int k, const T* __restrict__ g, const T* __restrict__ tau, T* __restrict__ tt,
-                             int ldt) {
+__global__ __launch_bounds__(256) void panel_t_kernel(int m, int k, const T* __restrict__ v, const T* __restrict__ tau,
+                                                      T* __restrict__ tt, int ldt, T* __restrict__ vt) {
+  __shared__ T G[64][65];
   __shared__ T Ts[64][65];
-  __shared__ T col[64];
-  const int t = threadIdx.x;  // 64 threads
-  for (int i = 0; i < k; ++i) {
-    // col[0:i] = -tau_i G(0:i, i); then T(0:i, i) = T(0:i, 0:i) col
-    if (t < i) col[t] = -tau[i] * g[idx(t, i, k)];
-    __syncthreads();
-    if (t < i) {
-      T s = 0;
-      for (int l = t; l < i; ++l) s += Ts[t][l] * col[l];
-      Ts[t][i] = s;
+  const int t = threadIdx.x;
+  for (int e = t; e < k * k; e += blockDim.x) {
+    const int r = e % k, c = e / k;
+    if (r < c) {
+      const T* vr = v + (int64_t)r * m;
+      const T* vc = v + (int64_t)c * m;
+      T sum = 0;
+      for (int i = 0; i < m; ++i) sum += vr[i] * vc[i];
+      G[r][c] = sum;
     }
-    if (t == 0) Ts[i][i] = tau[i];
+  }
+  __syncthreads();
+  for (int c = 0; c < k; ++c) {
+    if (t < c) {
+      T sum = 0;
+      for (int l = t; l < c; ++l) sum += Ts[t][l] * G[l][c];
+      Ts[t][c] = -tau[c] * sum;
+    }
+    if (t == 0) Ts[c][c] = tau[c];
     __syncthreads();
   }
-  for (int j = 0; j < k; ++j)
-    if (t < k) tt[idx(t, j, ldt)] = (t <= j) ? Ts[t][j] : T(0);
+  for (int e = t; e < k * k; e += blockDim.x) {
+    const int r = e % k, c = e / k;
+    tt[r + (int64_t)c * ldt] = (r <= c) ? Ts[r][c] : T(0);
+  }
+  // VT = V T
+  for (int e = t; e < m * k; e += blockDim.x) {
+    const int i = e % m, c = e / m;
+    T sum = 0;
+    for (int l = 0; l <= c; ++l) sum += v[i + (int64_t)l * m] * Ts[l][c];
+    vt[e] = sum;
+  }
+}
+
+// Y = VT^T X (k x k), then W = X - V Y / 2 in place of X (m x k). One workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_finish_kernel(int m, int k, const T* __restrict__ v,
+                                                           const T* __restrict__ vt, T* __restrict__ x) {
+  __shared__ T Y[64][65];
+  const int t = threadIdx.x;
+  for (int e = t; e < k * k; e += blockDim.x) {
+    const int r = e % k, c = e / k;
+    const T* a = vt + (int64_t)r * m;
+    const T* bb = x + (int64_t)c * m;
+    T sum = 0;
+    for (int i = 0; i < m; ++i) sum += a[i] * bb[i];
+    Y[r][c] = sum;
+  }
+  __syncthreads();
+  for (int e = t; e < m * k; e += blockDim.x) {
+    const int i = e % m, c = e / m;
+    T sum = 0;
+    for (int l = 0; l < k; ++l) sum += v[i + (int64_t)l * m] * Y[l][c];
+    x[e] -= T(0.5) * sum;
+  }
 }
 
 // Lower band of a (diagonals 0 .. b) into band storage ab, diagonals b + 1 .. 2b - 1 zero.
@@ -197,20 +231,26 @@ __global__ void extract_band_kernel(int n, int b, const T* __restrict__ a, int l
   }
 }
 
+template <typename T, int SB, int R>
+void launch_qr(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, T* buf, T* tau) {
+  hipLaunchKernelGGL((panel_qr_kernel<T, SB, R>), dim3(1), dim3(QR_THREADS), 0, s, m, nb, p, lda, v1, ldv1, buf, tau);
+}
+
 template <typename T>
 void launch_panel_qr(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, T* buf, T* tau) {
-  constexpr int SB = sizeof(T) == 8 ? 16 : 32;
-  if (m <= QR_THREADS)
-    hipLaunchKernelGGL((panel_qr_kernel<T, SB, 1>), dim3(1), dim3(QR_THREADS), 0, s, m, nb, p, lda, v1, ldv1, buf,
-                       tau);
-  else if (m <= 2 * QR_THREADS)
-    hipLaunchKernelGGL((panel_qr_kernel<T, SB, 2>), dim3(1), dim3(QR_THREADS), 0, s, m, nb, p, lda, v1, ldv1, buf,
-                       tau);
-  else if (m <= 4 * QR_THREADS)
-    hipLaunchKernelGGL((panel_qr_kernel<T, SB, 4>), dim3(1), dim3(QR_THREADS), 0, s, m, nb, p, lda, v1, ldv1, buf,
-                       tau);
-  else
-    throw std::runtime_error("reduce_to_band: n > 4096 + b is not supported");
+  constexpr int SB = 16;  // wider sub-panels exceed the unroller's budget and spill the register array
+  switch ((m + QR_THREADS - 1) / QR_THREADS) {
+    case 0: case 1: launch_qr<T, SB, 1>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    case 2: launch_qr<T, SB, 2>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    case 3: launch_qr<T, SB, 3>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    case 4: launch_qr<T, SB, 4>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    case 5: launch_qr<T, SB, 5>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    case 6: launch_qr<T, SB, 6>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    // FP64 with seven or eight rows per thread: 8-column sub-panels keep the register array out of scratch
+    case 7: launch_qr<T, sizeof(T) == 8 ? 8 : SB, 7>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    case 8: launch_qr<T, sizeof(T) == 8 ? 8 : SB, 8>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    default: throw std::runtime_error("reduce_to_band: n > 2048 + b is not supported");
+  }
   XTB_LAUNCH_CHECK();
 }
 
@@ -227,54 +267,28 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
   if (b > 64 || b < 1) throw std::runtime_error("reduce_to_band: band width must be in [1, 64]");
   T* tau = work;
   T* buf = tau + b;
-  T* g = buf + 3 * (size_t)n * b;
-  T* y = g + (size_t)b * b;
-  T* vt = y + (size_t)b * b;
+  T* vt = buf + 3 * (size_t)n * b;
   for (int j = 0; j < n - b - 1; j += b) {
     const int r = j + b, m = n - r, k = std::min(b, m);
     T* panel = a + idx(r, j, lda);
     T* v = v1 + idx(r, j, n);
     launch_panel_qr<T>(s, m, b, panel, lda, v, n, buf, tau);
-    // T factor
-    gemm<T>(s, 'T', 'N', k, k, m, T(1), buf, m, buf, m, T(0), g, k);
-    hipLaunchKernelGGL(larft_kernel<T>, dim3(1), dim3(64), 0, s, k, g, tau, t1 + idx(0, j, b), b);
-    XTB_LAUNCH_CHECK();
     T* tt = t1 + idx(0, j, b);
     T* a2 = a + idx(r, r, lda);
     T* x = buf + (size_t)m * k;  // slot 1 of [V | W | V]
-    gemm<T>(s, 'N', 'N', m, k, k, T(1), buf, m, tt, b, T(0), vt, m);            // VT = V T
-    gemm<T>(s, 'N', 'N', m, k, m, T(1), a2, lda, vt, m, T(0), x, m);            // X = A2 VT
-    gemm<T>(s, 'T', 'N', k, k, m, T(1), vt, m, x, m, T(0), y, k);               // Y = VT^T X
-    gemm<T>(s, 'N', 'N', m, k, k, T(-0.5), buf, m, y, k, T(1), x, m);           // W = X - V Y / 2
+    hipLaunchKernelGGL(panel_t_kernel<T>, dim3(1), dim3(256), 0, s, m, k, buf, tau, tt, b, vt);  // T, VT = V T
+    XTB_LAUNCH_CHECK();
+    gemm<T>(s, 'N', 'N', m, k, m, T(1), a2, lda, vt, m, T(0), x, m);                           // X = A2 VT
+    hipLaunchKernelGGL(panel_finish_kernel<T>, dim3(1), dim3(256), 0, s, m, k, buf, vt, x);     // W = X - V Y / 2
+    XTB_LAUNCH_CHECK();
     gemm<T>(s, 'N', 'T', m, m, 2 * k, T(-1), buf, m, buf + (size_t)m * k, m, T(1), a2, lda);  // A2 -= [V W][W V]^T
   }
   hipLaunchKernelGGL(extract_band_kernel<T>, dim3(n), dim3(128), 0, s, n, b, a, lda, ab);
   XTB_LAUNCH_CHECK();
 }
 
-template <typename T>
-void back_transform_band(hipStream_t s, int n, int b, const T* v1, const T* t1, T* z, int ldz, int m, T* work) {
-  // z(r:n, :) = (I - V T V^T) z(r:n, :) per panel, last panel first. work: 2 b m.
-  T* w1 = work;
-  T* w2 = work + (size_t)b * m;
-  int last = -1;
-  for (int j = 0; j < n - b - 1; j += b) last = j;
-  for (int j = last; j >= 0; j -= b) {
-    const int r = j + b, rows = n - r, k = std::min(b, rows);
-    const T* v = v1 + idx(r, j, n);
-    gemm<T>(s, 'T', 'N', k, m, rows, T(1), v, n, z + r, ldz, T(0), w1, k);       // W1 = V^T Z
-    gemm<T>(s, 'N', 'N', k, m, k, T(1), t1 + idx(0, j, b), b, w1, k, T(0), w2, k);  // W2 = T W1
-    gemm<T>(s, 'N', 'N', rows, m, k, T(-1), v, n, w2, k, T(1), z + r, ldz);       // Z -= V W2
-  }
-}
-
 template size_t band_workspace<double>(int, int);
 template size_t band_workspace<float>(int, int);
 template void reduce_to_band<double>(hipStream_t, int, int, double*, int, double*, double*, double*, double*);
 template void reduce_to_band<float>(hipStream_t, int, int, float*, int, float*, float*, float*, float*);
-template void back_transform_band<double>(hipStream_t, int, int, const double*, const double*, double*, int, int,
-                                          double*);
-template void back_transform_band<float>(hipStream_t, int, int, const float*, const float*, float*, int, int,
-                                         float*);
-
 }  // namespace xtbeig

@@ -95,9 +95,18 @@ int reduce_to_band_impl(xtbeig_handle_t handle, int n, int b, T* a, int lda, T* 
 template <typename T>
 int back_band_impl(xtbeig_handle_t handle, int n, int b, const T* v1, const T* t1, T* z, int ldz, int m) {
   return guarded([&] {
+    xtbeig::back_transform_band_wy<T>(handle->h.stream, handle->h.wyplan(n, b), v1, t1, z, ldz, m);
+  });
+}
+
+template <typename T>
+int back_bulges_impl(xtbeig_handle_t handle, int n, int b, const T* v2, T* z, int ldz, int m) {
+  return guarded([&] {
     Handle& h = handle->h;
-    h.reserve(2 * (size_t)b * m * sizeof(T) + 4096);
-    xtbeig::back_transform_band<T>(h.stream, n, b, v1, t1, z, ldz, m, static_cast<T*>(h.arena));
+    const xtbeig::WyPlan& p = h.wyplan(n, b);
+    h.reserve((p.v_elems() + p.t_elems()) * sizeof(T) + 8192);
+    T* vw = static_cast<T*>(h.arena);
+    xtbeig::back_transform_bulges_wy<T>(h.stream, p, v2, z, ldz, m, vw, vw + p.v_elems());
   });
 }
 
@@ -246,10 +255,10 @@ int xtbeig_dev_schase(xtbeig_handle_t handle, int n, int b, float* ab, float* v2
   return guarded([&] { xtbeig::chase_bulges<float>(handle->h.stream, n, b, ab, v2, d, e, handle->h.dinfo + 1); });
 }
 int xtbeig_dev_dback_bulges(xtbeig_handle_t handle, int n, int b, const double* v2, double* z, int ldz, int m) {
-  return guarded([&] { xtbeig::back_transform_bulges<double>(handle->h.stream, n, b, v2, z, ldz, m); });
+  return back_bulges_impl<double>(handle, n, b, v2, z, ldz, m);
 }
 int xtbeig_dev_sback_bulges(xtbeig_handle_t handle, int n, int b, const float* v2, float* z, int ldz, int m) {
-  return guarded([&] { xtbeig::back_transform_bulges<float>(handle->h.stream, n, b, v2, z, ldz, m); });
+  return back_bulges_impl<float>(handle, n, b, v2, z, ldz, m);
 }
 int xtbeig_dev_dback_band(xtbeig_handle_t handle, int n, int b, const double* v1, const double* t1, double* z,
                           int ldz, int m) {

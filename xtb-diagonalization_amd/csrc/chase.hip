@@ -1,11 +1,15 @@
 // Stage 2 of the two-stage reduction: band (width b) -> tridiagonal by Schwarz bulge chasing, one Householder
 // reflector per bulge (the reference's chaseBulgesLower / chaseBulgeLower, Stages/BulgeChasing.swift:123-321), and
-// the back-transformation through its reflectors (Stages/BulgeBackTransformation.swift:11-30).
+// its reflectors for the back-transformation (wy.hip).
 //
 // Chase: one workgroup; each wavefront owns one sweep at a time (sweep s on wave s mod W) and chases its bulges down
-// the band. Bulge k of sweep s waits until sweep s - 1 has finished bulges 0 .. k + 3 (the reference's pipelining
-// rule, BulgeChasing.swift:143-153), published through LDS progress counters. Every pair of bulges whose windows
-// overlap therefore runs in sequential order, and the result equals the one-sweep-at-a-time chase. Per bulge, with
+// the band. Bulge k of sweep s waits until sweep s - 1 has finished bulges 0 .. k + 2, published through LDS
+// progress counters. (The reference waits for 0 .. k + 3, BulgeChasing.swift:143-153, from the windows rows
+// rs - b ..< rs + 2b; counting only the lower-triangle elements a bulge touches -- left block rows R x columns
+// rs - b ..< rs, block R x R, block below rows re ..< re + b x columns R -- bulge (s - 1, k + 3) starts at row
+// rs + 3b - 1 and shares none of them, while (s - 1, k + 2) shares element (rs + 2b - 1, rs + b - 1).) Every pair of
+// bulges that touch a common element therefore runs in sequential order, and the result equals the
+// one-sweep-at-a-time chase. Per bulge, with
 // H = I - w w^T on rows R = rs ..< re, the lower triangle of the window splits into three blocks:
 //   columns lo ..< rs, rows R:   A[R, j] -= w (w^T A[R, j])        lane = column
 //   rows re ..< hi, columns R:   A[i, R] -= (A[i, R] w) w^T        lane = row
@@ -15,18 +19,13 @@
 // 2b - 1 below the diagonal (left block: rs + b - 1 - (rs - b); block below: re + b - 1 - rs), so one diagonal less
 // is stored and no two stored elements share an address. The band lives in LDS when its 2b n elements fit (FP32 up
 // to b = 16, FP64 up to b = 8 at n = 1204), else in global memory (L2-resident at these sizes).
-//
-// Back-transformation: Z = H_1 H_2 ... H_N Z. Bulges of one level p = 4 s + k act on disjoint rows (the reference's
-// ordering argument, BulgeChasing.swift:133-140), so levels run from last to first and the reflectors of a level in
-// parallel, one wavefront each, on a block of Z's columns kept in LDS.
+
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace xtbeig {
 
 namespace {
-
-constexpr int CHASE_WAVES = 16;
 
 __device__ inline int ld_acquire(const int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
 __device__ inline void st_release(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
@@ -105,7 +104,123 @@ __device__ inline bool chase_bulge(int n, int b, const Band<T>& A, T* __restrict
   return true;
 }
 
-template <typename T, bool LDS>
+// One bulge (s, k) by one wavefront with every load of the bulge issued up front. Band width b <= B (compile time);
+// the 64 lanes split as B columns (or rows) x LPC = 64 / B lanes, each lane holding RPL = B / LPC elements of each of
+// the three blocks in registers: lane = j * LPC + q handles column (row) j and the elements jj = q + LPC u, u < RPL.
+// Sums over a column (row) run across its LPC lanes (lane-xor offsets below LPC).
+template <typename T, int B>
+__device__ inline T group_sum(T v) {
+  constexpr int LPC = 64 / B;
+#pragma unroll
+  for (int off = 1; off < LPC; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T, int B>
+__device__ inline bool chase_bulge_reg(int n, int b, const Band<T>& A, T* __restrict__ v2, int ldv2, int s, int k,
+                                       T* wsh, T* qsh) {
+  constexpr int LPC = 64 / B, RPL = B / LPC;
+  const int lane = threadIdx.x & 63;
+  const int j = lane / LPC, q = lane % LPC;
+  const int rs = s + 1 + k * b;
+  if (rs >= n) return false;
+  const int re = min(rs + b, n), len = re - rs;
+  const int c = (k == 0) ? s : rs - b;
+  const int lo = max(rs - b, 0), hi = min(re + b, n);
+  const int nl = rs - lo, nb = hi - re;
+  // ---- loads: left block (column lo + j, rows rs + jj), block below (row re + j, columns rs + jj), diagonal block
+  // (row rs + j, columns rs + jj, read from the lower triangle)
+  T L[RPL], Bl[RPL], D[RPL];
+#pragma unroll
+  for (int u = 0; u < RPL; ++u) {
+    const int jj = q + LPC * u;
+    L[u] = (j < nl && jj < len) ? A(rs + jj, lo + j) : T(0);
+    Bl[u] = (j < nb && jj < len) ? A(re + j, rs + jj) : T(0);
+    D[u] = (j < len && jj < len) ? ((jj <= j) ? A(rs + j, rs + jj) : A(rs + jj, rs + j)) : T(0);
+  }
+  // ---- larfg on x = A[R, c]: column c is left-block column c - lo
+  const int jc = c - lo;
+  T part = 0, x0 = 0;
+#pragma unroll
+  for (int u = 0; u < RPL; ++u) {
+    const int jj = q + LPC * u;
+    if (j == jc && jj >= 1 && jj < len) part += L[u] * L[u];
+    if (j == jc && jj == 0) x0 = L[u];
+  }
+  const T xn2 = wave_sum(part);
+  const T alpha = wave_sum(x0);
+  T tau = 0, beta = alpha, scal = 0;
+  if (xn2 != T(0)) {
+    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+    tau = (beta - alpha) / beta;
+    scal = T(1) / (alpha - beta);
+  }
+  const T sq = sqrt(tau);
+  // w_i for i < len: lane (jc, q) owns x_i for i = q + LPC u; publish w through LDS
+  if (j == jc) {
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const int jj = q + LPC * u;
+      if (jj < len) {
+        const T wv = (jj == 0) ? sq : sq * (L[u] * scal);
+        wsh[jj] = wv;
+        v2[(int64_t)(rs + jj) + (int64_t)ldv2 * s] = wv;
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  T w[RPL];
+#pragma unroll
+  for (int u = 0; u < RPL; ++u) {
+    const int jj = q + LPC * u;
+    w[u] = (jj < len) ? wsh[jj] : T(0);
+  }
+  if (tau != T(0)) {
+    // left block: A[R, lo + j] -= w (w^T A[R, lo + j])
+    T dl = 0;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) dl += w[u] * L[u];
+    dl = group_sum<T, B>(dl);
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) L[u] -= w[u] * dl;
+    // block below: A[re + j, R] -= (A[re + j, R] w) w^T
+    T y = 0;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) y += Bl[u] * w[u];
+    y = group_sum<T, B>(y);
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) Bl[u] -= y * w[u];
+    // diagonal block: p = A_RR w, q = p - (w^T p) w / 2, A_RR -= w q^T + q w^T (lower triangle)
+    T pj = 0;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) pj += D[u] * w[u];
+    pj = group_sum<T, B>(pj);
+    const T wj = (j < len) ? wsh[j] : T(0);
+    const T half = wave_sum((q == 0 && j < len) ? wj * pj : T(0)) * T(0.5);
+    const T qj = pj - half * wj;
+    if (q == 0 && j < len) qsh[j] = qj;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const int jj = q + LPC * u;
+      if (jj < len) D[u] -= wj * qsh[jj] + qj * w[u];
+    }
+  }
+  // ---- stores (the column annihilated gets exact zeros)
+#pragma unroll
+  for (int u = 0; u < RPL; ++u) {
+    const int jj = q + LPC * u;
+    if (j < nl && jj < len) A(rs + jj, lo + j) = (j == jc) ? ((jj == 0) ? beta : T(0)) : L[u];
+    if (tau != T(0)) {
+      if (j < nb && jj < len) A(re + j, rs + jj) = Bl[u];
+      if (j < len && jj <= j) A(rs + j, rs + jj) = D[u];
+    }
+  }
+  __threadfence_block();
+  return true;
+}
+
+template <typename T, int B, bool LDS, int CHASE_WAVES>
 __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T* __restrict__ ab_global,
                                                                  T* __restrict__ v2, int ldv2, T* __restrict__ d,
                                                                  T* __restrict__ e, int* __restrict__ status) {
@@ -134,7 +249,7 @@ __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T
       const int pw = (s + CHASE_WAVES - 1) % CHASE_WAVES;
       for (int k = 0;; ++k) {
         if (s > 0) {
-          const int need = (s - 1) * KB + (k + 4);
+          const int need = (s - 1) * KB + (k + 3);
           long spins = 0;
           while (ld_acquire(&prog[pw]) < need) {
             __builtin_amdgcn_s_sleep(1);
@@ -142,7 +257,9 @@ __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T
           }
           if (failed) break;
         }
-        const bool more = chase_bulge<T>(n, b, A, v2, ldv2, s, k, wbuf + wid * b, qbuf + wid * b);
+        const bool more = (B > 0) ? chase_bulge_reg<T, (B > 0 ? B : 8)>(n, b, A, v2, ldv2, s, k, wbuf + wid * b,
+                                                                          qbuf + wid * b)
+                                  : chase_bulge<T>(n, b, A, v2, ldv2, s, k, wbuf + wid * b, qbuf + wid * b);
         if (!more) break;
         if (lane == 0) st_release(&prog[wid], s * KB + (k + 1));
       }
@@ -161,82 +278,36 @@ __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T
   }
 }
 
-// Z = H_1 ... H_N Z on a block of CB columns (LDS, n x CB), one level of disjoint reflectors at a time.
-template <typename T, int CB>
-__global__ __launch_bounds__(1024) void bulge_back_kernel(int n, int b, const T* __restrict__ v2, int ldv2,
-                                                          T* __restrict__ z, int ldz, int m) {
-  extern __shared__ char smem_raw[];
-  T* zs = reinterpret_cast<T*>(smem_raw);  // n x CB, ld n
-  const int t = threadIdx.x, lane = t & 63, wid = t >> 6, nw = blockDim.x >> 6;
-  const int c0 = blockIdx.x * CB;
-  const int cols = min(CB, m - c0);
-  for (int q = t; q < n * CB; q += blockDim.x) {
-    const int i = q % n, c = q / n;
-    zs[q] = (c < cols) ? z[idx(i, c0 + c, ldz)] : T(0);
-  }
-  __syncthreads();
-  const int sweeps = n - 2;
-  const int kmax0 = (n - 2) / b + 1;
-  const int pmax = 4 * (sweeps - 1) + kmax0;
-  // lanes: row r = lane % 32 (two passes for b > 32), column c = lane / 32 (+2 per pass)
-  for (int p = pmax; p >= 0; --p) {
-    // reflectors (s, k) with 4 s + k = p: s from max(0, ceil((p - kmax0) / 4)) to min(sweeps - 1, p / 4)
-    const int smin = max(0, (p - kmax0 + 3) / 4), smax = min(sweeps - 1, p / 4);
-    for (int s = smin + wid; s <= smax; s += nw) {
-      const int k = p - 4 * s;
-      const int rs = s + 1 + k * b;
-      if (rs >= n) continue;
-      const int len = min(rs + b, n) - rs;
-      const T* w = v2 + (int64_t)rs + (int64_t)ldv2 * s;
-      // w values per lane (rows lane and lane + 64 unused: b <= 64)
-      const T wl = (lane < len) ? w[lane] : T(0);
-      for (int c = 0; c < CB; ++c) {
-        T* zc = zs + (int64_t)c * n + rs;
-        const T zv = (lane < len) ? zc[lane] : T(0);
-        const T dot = wave_sum(wl * zv);
-        if (lane < len) zc[lane] = zv - wl * dot;
-      }
-    }
-    __syncthreads();
-  }
-  for (int q = t; q < n * cols; q += blockDim.x) {
-    const int i = q % n, c = q / n;
-    z[idx(i, c0 + c, ldz)] = zs[q];
-  }
-}
-
 }  // namespace
+
+template <typename T, int B>
+void launch_chase(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* status) {
+  // 16 wavefronts (sweeps in flight); 8 for b > 16, whose per-lane blocks need more than 128 registers
+  constexpr int CHASE_WAVES = (B == 32) ? 8 : 16;
+  const size_t vec_bytes = 2 * sizeof(T) * CHASE_WAVES * (size_t)b;
+  const size_t band_bytes = sizeof(T) * (size_t)band_elems(n, b);
+  if (vec_bytes + band_bytes + sizeof(int) * CHASE_WAVES <= 160 * 1024) {
+    allow_lds(chase_kernel<T, B, true, CHASE_WAVES>, vec_bytes + band_bytes);
+    hipLaunchKernelGGL((chase_kernel<T, B, true, CHASE_WAVES>), dim3(1), dim3(64 * CHASE_WAVES), vec_bytes + band_bytes, s, n, b,
+                       ab, v2, n, d, e, status);
+  } else {
+    hipLaunchKernelGGL((chase_kernel<T, B, false, CHASE_WAVES>), dim3(1), dim3(64 * CHASE_WAVES), vec_bytes, s, n, b, ab, v2, n,
+                       d, e, status);
+  }
+  XTB_LAUNCH_CHECK();
+}
 
 template <typename T>
 void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* status) {
   if (b > 64) throw std::runtime_error("chase_bulges: band width must be <= 64");
-  const size_t vec_bytes = 2 * sizeof(T) * CHASE_WAVES * (size_t)b;
-  const size_t band_bytes = sizeof(T) * (size_t)band_elems(n, b);
-  if (vec_bytes + band_bytes + sizeof(int) * CHASE_WAVES <= 160 * 1024) {
-    allow_lds(chase_kernel<T, true>, vec_bytes + band_bytes);
-    hipLaunchKernelGGL((chase_kernel<T, true>), dim3(1), dim3(64 * CHASE_WAVES), vec_bytes + band_bytes, s, n, b, ab,
-                       v2, n, d, e, status);
-  } else {
-    hipLaunchKernelGGL((chase_kernel<T, false>), dim3(1), dim3(64 * CHASE_WAVES), vec_bytes, s, n, b, ab, v2, n, d,
-                       e, status);
-  }
-  XTB_LAUNCH_CHECK();
-}
-
-template <typename T>
-void back_transform_bulges(hipStream_t s, int n, int b, const T* v2, T* z, int ldz, int m) {
-  if (n <= 2 || b <= 1 || m <= 0) return;
-  constexpr int CB = sizeof(T) == 8 ? 8 : 16;
-  const size_t bytes = sizeof(T) * (size_t)n * CB;
-  if (bytes > 160 * 1024) throw std::runtime_error("back_transform_bulges: n too large for the LDS column block");
-  allow_lds(bulge_back_kernel<T, CB>, bytes);
-  hipLaunchKernelGGL((bulge_back_kernel<T, CB>), dim3(cdiv(m, CB)), dim3(1024), bytes, s, n, b, v2, n, z, ldz, m);
-  XTB_LAUNCH_CHECK();
+  // register-resident bulges for b <= 32 (one template per power-of-two bound), the scalar form beyond
+  if (b <= 8) launch_chase<T, 8>(s, n, b, ab, v2, d, e, status);
+  else if (b <= 16) launch_chase<T, 16>(s, n, b, ab, v2, d, e, status);
+  else if (b <= 32) launch_chase<T, 32>(s, n, b, ab, v2, d, e, status);
+  else launch_chase<T, 0>(s, n, b, ab, v2, d, e, status);
 }
 
 template void chase_bulges<double>(hipStream_t, int, int, double*, double*, double*, double*, int*);
 template void chase_bulges<float>(hipStream_t, int, int, float*, float*, float*, float*, int*);
-template void back_transform_bulges<double>(hipStream_t, int, int, const double*, double*, int, int);
-template void back_transform_bulges<float>(hipStream_t, int, int, const float*, float*, int, int);
 
 }  // namespace xtbeig

@@ -14,38 +14,34 @@ namespace {
 constexpr int NB = 64;
 
 // One workgroup of 256 threads: factor A(k0:k0+kb, k0:k0+kb) = U^T U in LDS, write U to u and U^{-1} to x (both at
-// (k0, k0); the lower parts of those blocks zero).
+// (k0, k0); the lower parts of those blocks zero). Right-looking, two barriers per column; the trailing update maps
+// column q to threads q, q + 64, q + 128, q + 192 (rows interleaved by four). The inverse solves U Y = I with four
+// lanes of one wavefront per column of Y, partial sums combined by shuffles (no barrier between rows).
 __global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* __restrict__ a, int lda,
                                                          double* __restrict__ u, int ldu, double* __restrict__ x,
                                                          int ldx, int* info) {
   __shared__ double L[NB][NB + 1];  // L[i][j] = A(k0 + i, k0 + j)
   __shared__ double Y[NB][NB + 1];
-  __shared__ int bad;
   const int kb = min(NB, n - k0), t = threadIdx.x;
   if (*info != 0) return;
-  for (int e = t; e < kb * kb; e += 256) {
-    const int i = e % kb, j = e / kb;
-    L[i][j] = a[idx(k0 + i, k0 + j, lda)];
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e & (NB - 1), j = e / NB;
+    L[i][j] = (i < kb && j < kb) ? a[idx(k0 + i, k0 + j, lda)] : 0.0;
   }
-  if (t == 0) bad = 0;
   __syncthreads();
+  int bad = 0;
   for (int j = 0; j < kb; ++j) {
-    // pivot
-    if (t == 0) {
-      const double p = L[j][j];
-      if (!(p > 0.0)) bad = j + 1;
-      else L[j][j] = sqrt(p);
-    }
+    const double p = L[j][j];
+    if (!(p > 0.0)) { bad = j + 1; break; }
+    const double r = sqrt(p);
+    __syncthreads();  // every thread has read the pivot
+    if (t == 0) L[j][j] = r;
+    for (int c = j + 1 + t; c < kb; c += 256) L[j][c] /= r;
     __syncthreads();
-    if (bad) break;
-    const double r = L[j][j];
-    for (int c = j + 1 + t; c < kb; c += 256) L[j][c] /= r;  // row j of U
-    __syncthreads();
-    // trailing upper triangle: L[p][q] -= U[j][p] U[j][q], j < p <= q
-    const int m = kb - j - 1;
-    for (int e = t; e < m * m; e += 256) {
-      const int p = j + 1 + e % m, q = j + 1 + e / m;
-      if (p <= q) L[p][q] -= L[j][p] * L[j][q];
+    const int q = t & (NB - 1);
+    if (q > j && q < kb) {
+      const double ljq = L[j][q];
+      for (int pp = j + 1 + (t >> 6); pp <= q; pp += 4) L[pp][q] -= L[j][pp] * ljq;
     }
     __syncthreads();
   }
@@ -53,18 +49,15 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* 
     if (t == 0) atomicCAS(info, 0, k0 + bad);
     return;
   }
-  // Y = U^{-1} by columns: thread c solves U y = e_c (back substitution).
-  if (t < kb) {
-    const int c = t;
+  {
+    const int c = t >> 2, part = t & 3;
     for (int i = kb - 1; i >= 0; --i) {
-      double v = (i == c) ? 1.0 : 0.0;
-      if (i <= c) {
-        for (int l = i + 1; l <= c; ++l) v -= L[i][l] * Y[l][c];
-        v /= L[i][i];
-      } else {
-        v = 0.0;
-      }
-      Y[i][c] = v;
+      double sum = 0.0;
+      for (int l = i + 1 + part; l <= c && l < kb; l += 4) sum += L[i][l] * Y[l][c];
+      sum += __shfl_xor(sum, 1, 64);
+      sum += __shfl_xor(sum, 2, 64);
+      if (part == 0 && c < kb) Y[i][c] = (i <= c) ? ((i == c ? 1.0 : 0.0) - sum) / L[i][i] : 0.0;
+      __builtin_amdgcn_wave_barrier();
     }
   }
   __syncthreads();
@@ -74,8 +67,6 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* 
     x[idx(k0 + i, k0 + j, ldx)] = i <= j ? Y[i][j] : 0.0;
   }
 }
-
-__global__ void check_info_zero(const int*) {}
 
 }  // namespace
 

@@ -28,9 +28,9 @@ namespace {
 constexpr double EPS = 1.1102230246251565e-16;
 
 using Node = DcNode;
-// per merge results: K, ctot[4]
+// per merge results: K, ctot[4], Givens rotations recorded by the deflation
 struct NodeOut {
-  int k, ctot[4], pad[3];
+  int k, ctot[4], nrot, pad[2];
 };
 
 __device__ inline double lapy2(double x, double y) {
@@ -91,12 +91,13 @@ __global__ void unscale_kernel(int n, double* d, const double* scale) {
 }
 
 // dlaed2 for one merge (one workgroup). LDS: d, z, dlamda, w (doubles) and indxq, indx, indxc, indxp, coltyp (ints).
-__global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const int* node_pre_index, int n_all,
-                                                      double* __restrict__ d_all, double* __restrict__ q_all, int ldq,
-                                                      const double* __restrict__ rho_pre, double* __restrict__ rho_lvl,
-                                                      double* __restrict__ dlamda_all, double* __restrict__ w_all,
-                                                      double* __restrict__ q2_all, int* __restrict__ indxc_all,
-                                                      NodeOut* __restrict__ out) {
+__global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const int* node_pre_index,
+                                                      double* __restrict__ d_all, const double* __restrict__ q_all,
+                                                      int ldq, const double* __restrict__ rho_pre,
+                                                      double* __restrict__ rho_lvl, double* __restrict__ dlamda_all,
+                                                      double* __restrict__ w_all, int* __restrict__ indx_all,
+                                                      int* __restrict__ indxc_all, int* __restrict__ rotp_all,
+                                                      double* __restrict__ rotcs_all, NodeOut* __restrict__ out) {
 #pragma clang fp contract(off)  // dlaed2's scalar recurrences rounded as written (no fused multiply-add)
   extern __shared__ char smem[];
   const Node nd = nodes[blockIdx.x];
@@ -117,8 +118,7 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
   __shared__ int sh_k, sh_nrot, sh_ctot[4], sh_early;
   __shared__ double sh_rho;
   const int t = threadIdx.x;
-  double* q = q_all + idx(off, off, ldq);
-  double* q2 = q2_all + nd.q2off;
+  const double* q = q_all + idx(off, off, ldq);
   for (int i = t; i < n; i += blockDim.x) {
     D[i] = d_all[off + i];
     Z[i] = (i < n1) ? q[idx(n1 - 1, i, ldq)] : q[idx(n1, i, ldq)];
@@ -228,66 +228,83 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
   }
   __syncthreads();
   if (sh_early) {
-    // K = 0: Q columns in sorted order, D sorted
-    for (int j = 0; j < n; ++j) {
-      const int src = INDX[j] - 1;
-      for (int i = t; i < n; i += blockDim.x) q2[idx(i, j, n)] = q[idx(i, src, ldq)];
+    // K = 0: every column deflates, in sorted order (Q2 = Q(:, INDX), d = D(INDX))
+    if (t == 0) {
+      sh_k = 0;
+      sh_nrot = 0;
+      sh_ctot[0] = sh_ctot[1] = sh_ctot[2] = 0;
+      sh_ctot[3] = n;
     }
     __syncthreads();
-    for (int j = 0; j < n; ++j)
-      for (int i = t; i < n; i += blockDim.x) q[idx(i, j, ldq)] = q2[idx(i, j, n)];
-    for (int j = t; j < n; j += blockDim.x) d_all[off + j] = D[INDX[j] - 1];
-    if (t == 0) {
-      out[blockIdx.x].k = 0;
-      for (int c = 0; c < 4; ++c) out[blockIdx.x].ctot[c] = 0;
-      rho_lvl[blockIdx.x] = sh_rho;
-    }
-    return;
   }
-  // Givens rotations on Q's columns, in order (each thread owns its rows)
-  for (int r = 0; r < sh_nrot; ++r) {
-    const int cp = ROT_P[r], cn = ROT_N[r];
-    const double c = ROT_C[r], s = ROT_S[r];
-    for (int i = t; i < n; i += blockDim.x) {
-      const double x = q[idx(i, cp, ldq)], y = q[idx(i, cn, ldq)];
-      q[idx(i, cp, ldq)] = c * x + s * y;
-      q[idx(i, cn, ldq)] = c * y - s * x;
-    }
+  const int k = sh_k;
+  for (int j = t; j < n; j += blockDim.x) {
+    indx_all[off + j] = INDX[j];
+    indxc_all[off + j] = INDXC[j];
+    if (j >= k) d_all[off + j] = D[INDX[j] - 1];
   }
-  __syncthreads();
-  const int k = sh_k, c0 = sh_ctot[0], c1 = sh_ctot[1], c2 = sh_ctot[2], c3 = sh_ctot[3];
-  const int n12 = c0 + c1, n23 = c1 + c2;
-  // Q2 = [ n1 x n12 | n2 x n23 | n x c3 ]
-  double* q2a = q2;
-  double* q2b = q2 + (size_t)n1 * n12;
-  double* q2c = q2b + (size_t)n2 * n23;
-  for (int pos = 0; pos < n; ++pos) {
-    const int js = INDX[pos] - 1;
-    if (pos < c0) {
-      for (int i = t; i < n1; i += blockDim.x) q2a[idx(i, pos, n1)] = q[idx(i, js, ldq)];
-    } else if (pos < c0 + c1) {
-      for (int i = t; i < n1; i += blockDim.x) q2a[idx(i, pos, n1)] = q[idx(i, js, ldq)];
-      for (int i = t; i < n2; i += blockDim.x) q2b[idx(i, pos - c0, n2)] = q[idx(n1 + i, js, ldq)];
-    } else if (pos < c0 + c1 + c2) {
-      for (int i = t; i < n2; i += blockDim.x) q2b[idx(i, pos - c0, n2)] = q[idx(n1 + i, js, ldq)];
-    } else {
-      for (int i = t; i < n; i += blockDim.x) q2c[idx(i, pos - k, n)] = q[idx(i, js, ldq)];
-    }
-  }
-  __syncthreads();
-  // deflated eigenpairs into the last n - K slots
-  for (int j = 0; j < c3; ++j)
-    for (int i = t; i < n; i += blockDim.x) q[idx(i, k + j, ldq)] = q2c[idx(i, j, n)];
-  for (int j = k + t; j < n; j += blockDim.x) d_all[off + j] = D[INDX[j] - 1];
   for (int j = t; j < k; j += blockDim.x) {
     dlamda_all[off + j] = DL[j];
     w_all[off + j] = W[j];
   }
-  for (int j = t; j < n; j += blockDim.x) indxc_all[off + j] = INDXC[j];
+  for (int r = t; r < sh_nrot; r += blockDim.x) {
+    rotp_all[2 * (off + r)] = ROT_P[r];
+    rotp_all[2 * (off + r) + 1] = ROT_N[r];
+    rotcs_all[2 * (off + r)] = ROT_C[r];
+    rotcs_all[2 * (off + r) + 1] = ROT_S[r];
+  }
   if (t == 0) {
     out[blockIdx.x].k = k;
     for (int c = 0; c < 4; ++c) out[blockIdx.x].ctot[c] = sh_ctot[c];
+    out[blockIdx.x].nrot = sh_nrot;
     rho_lvl[blockIdx.x] = sh_rho;
+  }
+}
+
+// The deflation's Givens rotations on Q's columns, in order; each thread owns one row of the merge's block.
+__global__ void rotate_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ rotp_all,
+                              const double* __restrict__ rotcs_all, double* __restrict__ q_all, int ldq) {
+  const Node nd = nodes[blockIdx.y];
+  const int nrot = out[blockIdx.y].nrot;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nrot == 0 || i >= nd.sz) return;
+  double* row = q_all + nd.off + i + (int64_t)nd.off * ldq;
+  for (int r = 0; r < nrot; ++r) {
+    const int cp = rotp_all[2 * (nd.off + r)], cn = rotp_all[2 * (nd.off + r) + 1];
+    const double c = rotcs_all[2 * (nd.off + r)], sn = rotcs_all[2 * (nd.off + r) + 1];
+    const double x = row[(int64_t)cp * ldq], y = row[(int64_t)cn * ldq];
+    row[(int64_t)cp * ldq] = c * x + sn * y;
+    row[(int64_t)cn * ldq] = c * y - sn * x;
+  }
+}
+
+// Q2 = [ n1 x n12 | n2 x n23 | n x ctot3 ]: column pos of the grouped order from Q's column INDX(pos) (dlaed2's
+// copies). One workgroup per column.
+__global__ void q2_gather_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ indx_all,
+                                 const double* __restrict__ q_all, int ldq, double* __restrict__ q2_all) {
+  const Node nd = nodes[blockIdx.y];
+  const int pos = blockIdx.x;
+  if (pos >= nd.sz) return;
+  const NodeOut o = out[blockIdx.y];
+  const int n = nd.sz, n1 = nd.n1, n2 = n - n1, k = o.k;
+  const int c0 = o.ctot[0], c1 = o.ctot[1], c2 = o.ctot[2];
+  const int n12 = c0 + c1, n23 = c1 + c2;
+  const double* q = q_all + idx(nd.off, nd.off, ldq);
+  double* q2a = q2_all + nd.q2off;
+  double* q2b = q2a + (size_t)n1 * n12;
+  double* q2c = q2b + (size_t)n2 * n23;
+  const int js = indx_all[nd.off + pos] - 1;
+  const double* src = q + (int64_t)js * ldq;
+  const int t = threadIdx.x;
+  if (pos < c0) {
+    for (int i = t; i < n1; i += blockDim.x) q2a[idx(i, pos, n1)] = src[i];
+  } else if (pos < c0 + c1) {
+    for (int i = t; i < n1; i += blockDim.x) q2a[idx(i, pos, n1)] = src[i];
+    for (int i = t; i < n2; i += blockDim.x) q2b[idx(i, pos - c0, n2)] = src[n1 + i];
+  } else if (pos < c0 + c1 + c2) {
+    for (int i = t; i < n2; i += blockDim.x) q2b[idx(i, pos - c0, n2)] = src[n1 + i];
+  } else {
+    for (int i = t; i < n; i += blockDim.x) q2c[idx(i, pos - k, n)] = src[i];
   }
 }
 
@@ -298,12 +315,19 @@ __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const No
                                                      double* __restrict__ d_all, int* __restrict__ orig_all,
                                                      double* __restrict__ tau_all, double* __restrict__ s_all,
                                                      int* info) {
+  extern __shared__ double poles[];  // dlamda (k), then z (k): every root reads all of them each iteration
   const Node nd = nodes[blockIdx.y];
   const int k = out[blockIdx.y].k;
+  if (blockIdx.x * 64 >= k) return;
+  for (int i = threadIdx.x; i < k; i += 64) {
+    poles[i] = dlamda_all[nd.off + i];
+    poles[k + i] = w_all[nd.off + i];
+  }
+  __syncthreads();
   const int j = blockIdx.x * 64 + threadIdx.x;
   if (j >= k) return;
-  const double* dl = dlamda_all + nd.off;
-  const double* w = w_all + nd.off;
+  const double* dl = poles;
+  const double* w = poles + k;
   const double rho = rho_lvl[blockIdx.y];
   double* S = s_all + nd.q2off;
   if (k == 1) {
@@ -423,17 +447,28 @@ __global__ void lamrg_kernel(int count, const Node* nodes, const NodeOut* out, c
   }
 }
 
-// temp(:, i) = Q(:, perm[i]) within each merge's block; dtemp(i) = d(perm[i]).
-__global__ void gather_kernel(const Node* nodes, const int* __restrict__ perm_all, const double* __restrict__ q_all,
-                              int ldq, const double* __restrict__ d_all, double* __restrict__ tmp_all,
+// temp(:, i) = column perm[i] of the merged system: the merge products (Q(:, 0 ..< K)) or a deflated eigenvector
+// (Q2's last group); dtemp(i) = d(perm[i]).
+__global__ void gather_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ perm_all,
+                              const double* __restrict__ q_all, int ldq, const double* __restrict__ q2_all,
+                              const double* __restrict__ d_all, double* __restrict__ tmp_all,
                               double* __restrict__ dtmp_all) {
   const Node nd = nodes[blockIdx.z];
   const int col = blockIdx.y;
   if (col >= nd.sz) return;
+  const NodeOut o = out[blockIdx.z];
+  const int n = nd.sz, n1 = nd.n1, n2 = n - n1, k = o.k;
   const int src = perm_all[nd.off + col] - 1;
+  const double* from;
+  if (src < k) {
+    from = q_all + idx(nd.off, nd.off + src, ldq);
+  } else {
+    const int n12 = o.ctot[0] + o.ctot[1], n23 = o.ctot[1] + o.ctot[2];
+    from = q2_all + nd.q2off + (size_t)n1 * n12 + (size_t)n2 * n23 + (size_t)(src - k) * n;
+  }
   double* tmp = tmp_all + nd.q2off;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd.sz; i += gridDim.x * blockDim.x)
-    tmp[idx(i, col, nd.sz)] = q_all[idx(nd.off + i, nd.off + src, ldq)];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    tmp[idx(i, col, n)] = from[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) dtmp_all[nd.off + col] = d_all[nd.off + src];
 }
 
@@ -498,10 +533,97 @@ DcPlan::~DcPlan() {
   if (pre_index_dev) (void)hipFree(pre_index_dev);
 }
 
-// Workspace layout (doubles): dlamda, w, tau, wnew, dtmp (n each, dtmp 2n), rho_pre (n), rho_lvl (n), scale (16),
-// Q2 (n^2), S (n^2). Ints: orig, indxc, perm (n each), out (8 n).
-size_t dc_workspace_doubles(int n) { return 8 * (size_t)n + 16 + 2 * (size_t)n * n; }
-size_t dc_workspace_ints(int n) { return 3 * (size_t)n + 8 * (size_t)n + 16; }
+// Workspace (doubles): dlamda, w, tau, wnew (n each), dtmp (2n), rho_pre, rho_lvl (n each), scale (16), rotation
+// cosines/sines (2n), Q2 (n^2), S (n^2). Ints: orig, indxc, indx, perm (n each), rotation columns (2n), out (8n),
+// probe scratch (16).
+size_t dc_workspace_doubles(int n) { return 10 * (size_t)n + 16 + 2 * (size_t)n * n; }
+size_t dc_workspace_ints(int n) { return 14 * (size_t)n + 32; }
+
+namespace {
+
+struct DcWork {
+  double *dlamda, *w, *tau, *wnew, *dtmp, *rho_pre, *rho_lvl, *scale, *rotcs, *q2, *sbuf;
+  int *orig, *indxc, *indx, *perm, *rotp;
+  NodeOut* outs;
+  int* scratch;
+};
+
+DcWork carve(int n, double* work, int* iwork) {
+  DcWork w;
+  w.dlamda = work;
+  w.w = w.dlamda + n;
+  w.tau = w.w + n;
+  w.wnew = w.tau + n;
+  w.dtmp = w.wnew + n;
+  w.rho_pre = w.dtmp + 2 * (size_t)n;
+  w.rho_lvl = w.rho_pre + n;
+  w.scale = w.rho_lvl + n;
+  w.rotcs = w.scale + 16;
+  w.q2 = w.rotcs + 2 * (size_t)n;
+  w.sbuf = w.q2 + (size_t)n * n;
+  w.orig = iwork;
+  w.indxc = w.orig + n;
+  w.indx = w.indxc + n;
+  w.perm = w.indx + n;
+  w.rotp = w.perm + n;
+  w.outs = reinterpret_cast<NodeOut*>(w.rotp + 2 * (size_t)n);
+  w.scratch = reinterpret_cast<int*>(w.outs + n);
+  return w;
+}
+
+// Every kernel of one tree level (cnt merges, the largest of size maxsz). probe: optional snapshot of dlaed2's
+// outputs right after the deflation.
+struct Probe {
+  double *d_defl, *q2, *dlamda, *w, *rho;
+  int *kctot, *indxc;
+};
+
+void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cnt, int maxsz, double* d, double* z,
+               int ldz, const DcWork& W, int* info, const Probe* probe) {
+  const size_t lds = (size_t)maxsz * (6 * sizeof(double) + 7 * sizeof(int)) + 64;
+  if (lds > 160 * 1024) throw std::runtime_error("stedc: n too large for the LDS-staged deflation");
+  allow_lds(deflate_kernel, lds);
+  hipLaunchKernelGGL(deflate_kernel, dim3(cnt), dim3(256), lds, s, lvl, lvl_pre, d, z, ldz, W.rho_pre, W.rho_lvl,
+                     W.dlamda, W.w, W.indx, W.indxc, W.rotp, W.rotcs, W.outs);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(rotate_kernel, dim3(cdiv(maxsz, 256), cnt), dim3(256), 0, s, lvl, W.outs, W.rotp, W.rotcs, z,
+                     ldz);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(q2_gather_kernel, dim3(maxsz, cnt), dim3(256), 0, s, lvl, W.outs, W.indx, z, ldz, W.q2);
+  XTB_LAUNCH_CHECK();
+  if (probe) {
+    XTB_HIP(hipMemcpyAsync(probe->d_defl, d, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    XTB_HIP(hipMemcpyAsync(probe->q2, W.q2, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+    XTB_HIP(hipMemcpyAsync(probe->kctot, W.outs, sizeof(int) * 5, hipMemcpyDeviceToDevice, s));
+    XTB_HIP(hipMemcpyAsync(probe->dlamda, W.dlamda, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    XTB_HIP(hipMemcpyAsync(probe->w, W.w, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    XTB_HIP(hipMemcpyAsync(probe->indxc, W.indxc, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    XTB_HIP(hipMemcpyAsync(probe->rho, W.rho_lvl, sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+  allow_lds(secular_kernel, 2 * sizeof(double) * maxsz);
+  hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 2 * sizeof(double) * maxsz, s, lvl, W.outs,
+                     W.rho_lvl, W.dlamda, W.w, W.indxc, d, W.orig, W.tau, W.sbuf, info);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 0, s, lvl, W.outs, W.dlamda, W.w, W.orig,
+                     W.tau, W.wnew);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(maxsz, 4), cnt), dim3(256), 0, s, lvl, W.outs, W.dlamda, W.orig,
+                     W.tau, W.wnew, W.indxc, W.sbuf);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), 2 * cnt), dim3(256), 0, s, lvl,
+                     W.outs, W.q2, W.sbuf, z, ldz);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(lamrg_kernel, dim3(cdiv(cnt, 256)), dim3(256), 0, s, cnt, lvl, W.outs, d, W.perm);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(gather_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, W.outs, W.perm, z, ldz,
+                     W.q2, d, W.sbuf, W.dtmp);
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, W.sbuf, W.dtmp, z,
+                     ldz, d);
+  XTB_LAUNCH_CHECK();
+}
+
+}  // namespace
 
 void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, int ldz, double* work, int* iwork,
            int* info) {
@@ -511,62 +633,19 @@ void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, i
     set_identity(s, 1, z, ldz);
     return;
   }
-  double* dlamda = work;
-  double* w = dlamda + n;
-  double* tau = w + n;
-  double* wnew = tau + n;
-  double* dtmp = wnew + n;
-  double* rho_pre = dtmp + 2 * (size_t)n;
-  double* rho_lvl = rho_pre + n;
-  double* scale = rho_lvl + n;
-  double* q2 = scale + 16;
-  double* sbuf = q2 + (size_t)n * n;
-  int* orig = iwork;
-  int* indxc = orig + n;
-  int* perm = indxc + n;
-  NodeOut* outs = reinterpret_cast<NodeOut*>(perm + n);
+  const DcWork W = carve(n, work, iwork);
   const Node* nodes = plan.nodes_dev;
   set_identity(s, n, z, ldz);
-  hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(256), 0, s, n, d, e, nodes, plan.nmerge, rho_pre, scale);
+  hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(256), 0, s, n, d, e, nodes, plan.nmerge, W.rho_pre, W.scale);
   XTB_LAUNCH_CHECK();
   for (int L = (int)plan.lvls.size() - 1; L >= 0; --L) {
     const DcPlan::Level& lv = plan.lvls[L];
-    const int cnt = lv.count, maxsz = lv.maxsz;
-    const Node* lvl = nodes + lv.start;
-    const int* lvl_pre = plan.pre_index_dev + lv.pre_start;
-    const size_t lds = (size_t)maxsz * (6 * sizeof(double) + 7 * sizeof(int)) + 64;
-    if (lds > 160 * 1024) throw std::runtime_error("stedc: n too large for the LDS-staged deflation");
-    allow_lds(deflate_kernel, lds);
-    hipLaunchKernelGGL(deflate_kernel, dim3(cnt), dim3(256), lds, s, lvl, lvl_pre, n, d, z, ldz, rho_pre, rho_lvl,
-                       dlamda, w, q2, indxc, outs);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 0, s, lvl, outs, rho_lvl, dlamda, w,
-                       indxc, d, orig, tau, sbuf, info);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 0, s, lvl, outs, dlamda, w, orig, tau, wnew);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(maxsz, 4), cnt), dim3(256), 0, s, lvl, outs, dlamda, orig, tau,
-                       wnew, indxc, sbuf);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), 2 * cnt), dim3(256), 0, s, lvl,
-                       outs, q2, sbuf, z, ldz);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(lamrg_kernel, dim3(cdiv(cnt, 256)), dim3(256), 0, s, cnt, lvl, outs, d, perm);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gather_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, perm, z, ldz, d,
-                       sbuf, dtmp);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, sbuf, dtmp, z, ldz,
-                       d);
-    XTB_LAUNCH_CHECK();
+    run_level(s, n, nodes + lv.start, plan.pre_index_dev + lv.pre_start, lv.count, lv.maxsz, d, z, ldz, W, info,
+              nullptr);
   }
-  hipLaunchKernelGGL(unscale_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, scale);
+  hipLaunchKernelGGL(unscale_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, W.scale);
   XTB_LAUNCH_CHECK();
 }
-
-}  // namespace xtbeig
-
-namespace xtbeig {
 
 // One merge (dlaed1 semantics) of two solved halves on the device, exposing dlaed2's outputs for comparison with the
 // host LAPACK: d (n) ascending within each half, q (n x n, ldq) block diagonal with the halves' eigenvectors.
@@ -575,49 +654,16 @@ void dc_merge_probe(hipStream_t s, int n, int n1, double rho, double* d, double*
                     int* iwork, int* info, double* d_defl_out, double* q2_out, int* kctot_out, double* dlamda_out,
                     double* w_out, int* indxc_out, double* rho_out) {
   if (n < 2 || n1 < 1 || n1 >= n) throw std::runtime_error("dc_merge_probe: need 1 <= n1 < n");
-  double* dlamda = work;
-  double* w = dlamda + n;
-  double* tau = w + n;
-  double* wnew = tau + n;
-  double* dtmp = wnew + n;
-  double* rho_pre = dtmp + 2 * (size_t)n;
-  double* rho_lvl = rho_pre + n;
-  double* q2 = rho_lvl + n + 16;
-  double* sbuf = q2 + (size_t)n * n;
-  int* orig = iwork;
-  int* indxc = orig + n;
-  int* perm = indxc + n;
-  NodeOut* outs = reinterpret_cast<NodeOut*>(perm + n);
-  int* pre_index = reinterpret_cast<int*>(outs + 1);
-  Node* node = reinterpret_cast<Node*>(pre_index + 4);
+  const DcWork W = carve(n, work, iwork);
+  int* pre_index = W.scratch;
+  Node* node = reinterpret_cast<Node*>(W.scratch + 4);
   const Node host_node{0, n, n1, 0};
   const int zero = 0;
   XTB_HIP(hipMemcpy(node, &host_node, sizeof(Node), hipMemcpyHostToDevice));
   XTB_HIP(hipMemcpy(pre_index, &zero, sizeof(int), hipMemcpyHostToDevice));
-  XTB_HIP(hipMemcpy(rho_pre, &rho, sizeof(double), hipMemcpyHostToDevice));
-  const size_t lds = (size_t)n * (6 * sizeof(double) + 7 * sizeof(int)) + 64;
-  allow_lds(deflate_kernel, lds);
-  hipLaunchKernelGGL(deflate_kernel, dim3(1), dim3(256), lds, s, node, pre_index, n, d, q, ldq, rho_pre, rho_lvl,
-                     dlamda, w, q2, indxc, outs);
-  XTB_LAUNCH_CHECK();
-  XTB_HIP(hipMemcpyAsync(d_defl_out, d, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  XTB_HIP(hipMemcpyAsync(q2_out, q2, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
-  XTB_HIP(hipMemcpyAsync(kctot_out, outs, sizeof(int) * 5, hipMemcpyDeviceToDevice, s));
-  XTB_HIP(hipMemcpyAsync(dlamda_out, dlamda, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  XTB_HIP(hipMemcpyAsync(w_out, w, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-  XTB_HIP(hipMemcpyAsync(indxc_out, indxc, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
-  XTB_HIP(hipMemcpyAsync(rho_out, rho_lvl, sizeof(double), hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(secular_kernel, dim3(cdiv(n, 64), 1), dim3(64), 0, s, node, outs, rho_lvl, dlamda, w, indxc, d,
-                     orig, tau, sbuf, info);
-  hipLaunchKernelGGL(gu_kernel, dim3(cdiv(n, 64), 1), dim3(64), 0, s, node, outs, dlamda, w, orig, tau, wnew);
-  hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(n, 4), 1), dim3(256), 0, s, node, outs, dlamda, orig, tau, wnew,
-                     indxc, sbuf);
-  hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(n, 64), cdiv(n, 64), 2), dim3(256), 0, s, node, outs, q2, sbuf, q,
-                     ldq);
-  hipLaunchKernelGGL(lamrg_kernel, dim3(1), dim3(64), 0, s, 1, node, outs, d, perm);
-  hipLaunchKernelGGL(gather_kernel, dim3(cdiv(n, 256), n, 1), dim3(256), 0, s, node, perm, q, ldq, d, sbuf, dtmp);
-  hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(n, 256), n, 1), dim3(256), 0, s, node, sbuf, dtmp, q, ldq, d);
-  XTB_LAUNCH_CHECK();
+  XTB_HIP(hipMemcpy(W.rho_pre, &rho, sizeof(double), hipMemcpyHostToDevice));
+  const Probe pr{d_defl_out, q2_out, dlamda_out, w_out, rho_out, kctot_out, indxc_out};
+  run_level(s, n, node, pre_index, 1, n, d, q, ldq, W, info, &pr);
 }
 
 }  // namespace xtbeig

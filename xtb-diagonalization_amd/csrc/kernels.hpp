@@ -41,8 +41,6 @@ size_t band_workspace(int n, int b);  // elements of T
 // T factors t1 (b x n).
 template <typename T>
 void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T* ab, T* work);
-template <typename T>
-void back_transform_band(hipStream_t s, int n, int b, const T* v1, const T* t1, T* z, int ldz, int m, T* work);
 
 // chase.hip --------------------------------------------------------------------------------------------------------
 // Band storage of the chase: element (i, j), 0 <= i - j <= 2b - 1, at i + band_ld(b) j.
@@ -52,8 +50,6 @@ __host__ __device__ inline int64_t band_elems(int n, int b) { return (int64_t)ba
 // holds sweep s, rows rs ..< re).
 template <typename T>
 void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* flags);
-template <typename T>
-void back_transform_bulges(hipStream_t s, int n, int b, const T* v2, T* z, int ldz, int m);
 
 // dc.hip -----------------------------------------------------------------------------------------------------------
 // One merge of the divide-and-conquer tree: rows/columns off ..< off + sz, torn after n1; q2off: its Q2 / S offset.
@@ -85,6 +81,32 @@ void dc_merge_probe(hipStream_t s, int n, int n1, double rho, double* d, double*
 }  // namespace xtbeig
 
 namespace xtbeig {
+// wy.hip -----------------------------------------------------------------------------------------------------------
+// One compact-WY block Z(row0 .. row0 + height, :) <- (I - V T V^T) Z(...): V height x count (ld ldv, at v_off),
+// T count x count upper (ld ldt, at t_off). s0, k: the bulge block's first sweep and bulge index.
+struct WyBlock {
+  int row0, height, count, s0, k, ldv, ldt, pad;
+  int64_t v_off, t_off;
+};
+// The block lists of the two back-transformations for one (n, b), uploaded once.
+struct WyPlan {
+  int n = 0, b = 0, g = 0, nbulge = 0, npanel = 0;
+  WyBlock* bulge_dev = nullptr;
+  WyBlock* panel_dev = nullptr;
+  WyPlan(int n, int b);
+  ~WyPlan();
+  WyPlan(const WyPlan&) = delete;
+  WyPlan& operator=(const WyPlan&) = delete;
+  size_t v_elems() const;  // workspace for the bulge blocks' V and T
+  size_t t_elems() const;
+};
+int bulge_group(int b);
+template <typename T>
+void back_transform_bulges_wy(hipStream_t s, const WyPlan& p, const T* v2, T* z, int ldz, int m, T* vwork,
+                              T* twork);
+template <typename T>
+void back_transform_band_wy(hipStream_t s, const WyPlan& p, const T* v1, const T* t1, T* z, int ldz, int m);
+
 // refine.hip -------------------------------------------------------------------------------------------------------
 size_t refine_workspace(int n);  // doubles; iwork n ints
 void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm, int lds, double* c, int ldc,

@@ -16,11 +16,14 @@
 
 namespace xtbeig {
 
+// Band width per n: the widest band whose chase storage ((2b - 1) n elements plus the per-wave vectors) fits the
+// 160 KiB LDS of one CU, so the bulge chase runs LDS-resident: FP32 b = 16 up to n = 1280 and b = 8 up to 2600, FP64
+// b = 8 up to n = 1290; beyond that b = 16 with the band in global memory.
 int default_band(int n, bool fp32) {
-  (void)fp32;
   if (n <= 3) return 1;
-  int b = 32;
-  if (n < 100) b = 16;
+  int b;
+  if (fp32) b = n <= 1280 ? 16 : (n <= 2600 ? 8 : 16);
+  else b = n <= 1290 ? 8 : 16;
   return std::min(b, std::max(2, n - 1));
 }
 
@@ -72,6 +75,16 @@ const DcPlan& Handle::plan(int n) {
   return ref;
 }
 
+const WyPlan& Handle::wyplan(int n, int b) {
+  const auto key = std::make_pair(n, b);
+  auto it = wyplans.find(key);
+  if (it != wyplans.end()) return *it->second;
+  auto p = std::make_unique<WyPlan>(n, b);
+  const WyPlan& ref = *p;
+  wyplans.emplace(key, std::move(p));
+  return ref;
+}
+
 // ---- workspace carving --------------------------------------------------------------------------------------------
 
 struct Carver {
@@ -89,7 +102,7 @@ struct Carver {
 
 template <typename T>
 struct StdBuffers {
-  T *v1, *t1, *ab, *bwork, *v2, *dT, *eT, *btwork;
+  T *v1, *t1, *ab, *bwork, *v2, *dT, *eT, *wyv, *wyt;
   double *d64, *e64, *z64, *dcwork;
   int* dciwork;
 };
@@ -105,7 +118,11 @@ StdBuffers<T> carve_std(Carver& c, int n, int b) {
   s.v2 = c.take<T>(nn);
   s.dT = c.take<T>(n);
   s.eT = c.take<T>(n);
-  s.btwork = c.take<T>(2 * (size_t)b * n);
+  // bulge WY blocks: sized for the plan of (n, b) without building it here
+  const int g = bulge_group(b);
+  const size_t nblk = (size_t)((n + b - 1) / b + 1) * (size_t)((n + g - 1) / g + 1);
+  s.wyv = c.take<T>(nblk * (size_t)(b + g) * g + 1);
+  s.wyt = c.take<T>(nblk * (size_t)g * g + 1);
   s.d64 = c.take<double>(n);
   s.e64 = c.take<double>(n);
   s.z64 = std::is_same<T, double>::value ? nullptr : c.take<double>(nn);
@@ -132,8 +149,11 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   stedc(st, h.plan(n), s.d64, s.e64, z64, ldz64, s.dcwork, s.dciwork, h.dinfo + 2);
   if (!std::is_same<T, double>::value) copy_matrix<double, T>(st, n, n, z64, n, z, ldz);
   copy_matrix<double, double>(st, n, 1, s.d64, n, w, n);
-  back_transform_bulges<T>(st, n, b, s.v2, z, ldz, n);
-  back_transform_band<T>(st, n, b, s.v1, s.t1, z, ldz, n, s.btwork);
+  const WyPlan& wp = h.wyplan(n, b);
+  if (wp.v_elems() > ((size_t)((n + b - 1) / b + 1) * (size_t)((n + wp.g - 1) / wp.g + 1)) * (b + wp.g) * wp.g + 1)
+    throw std::runtime_error("syevd: bulge block workspace too small");
+  back_transform_bulges_wy<T>(st, wp, s.v2, z, ldz, n, s.wyv, s.wyt);
+  back_transform_band_wy<T>(st, wp, s.v1, s.t1, z, ldz, n);
 }
 
 template <typename T>
