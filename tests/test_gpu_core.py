@@ -88,7 +88,8 @@ def test_reduce_to_band(handle, torch_cuda, prec, b, n):
     assert np.max(np.abs(q1.T @ a @ q1 - band)) < (1e-13 if prec == "d" else 3e-6) * n
 
 
-@pytest.mark.parametrize("prec,b,n", [("d", 32, 260), ("d", 8, 131), ("s", 16, 200), ("d", 3, 40), ("d", 64, 300)])
+@pytest.mark.parametrize("prec,b,n", [("d", 32, 260), ("d", 8, 131), ("s", 16, 200), ("d", 3, 40), ("d", 16, 300),
+                                      ("s", 8, 517), ("s", 32, 100)])
 def test_chase(handle, torch_cuda, prec, b, n):
     torch = torch_cuda
     rng = np.random.default_rng(5)
@@ -109,9 +110,12 @@ def test_chase(handle, torch_cuda, prec, b, n):
     handle.chase(n, b, AB, V2, D, E)
     assert handle.synchronize() == [0, 0, 0]
     d, e = D.double().cpu().numpy(), E.double().cpu().numpy()
-    od, oe, _ = sbr.chase_bulges(full, b)
-    assert np.max(np.abs(d - od)) < (1e-13 if prec == "d" else 1e-4) * n
-    assert np.max(np.abs(np.abs(e[: n - 1]) - np.abs(oe))) < (1e-13 if prec == "d" else 1e-4) * n
+    if prec == "d":
+        # same reflectors as the oracle's sequential chase: same tridiagonal to rounding (in FP32 the tridiagonal
+        # form itself is ill-conditioned, only its spectrum is compared)
+        od, oe, _ = sbr.chase_bulges(full, b)
+        assert np.max(np.abs(d - od)) < 1e-13 * n
+        assert np.max(np.abs(np.abs(e[: n - 1]) - np.abs(oe))) < 1e-13 * n
     wt = sl.eigvalsh_tridiagonal(d, e[: n - 1])
     wref = np.linalg.eigvalsh(full)
     tol = 1e-12 if prec == "d" else 2e-4

@@ -1,0 +1,338 @@
+// Back-transformations of the two-stage reduction, on zt = Z^T: row i of the eigenvector matrix Z is the contiguous
+// column i of zt, so the 64 lanes of a wavefront hold 64 eigenvectors.
+//
+// Bulge reflectors (the reference's backTransformBulges / backTransformBulgesBlocked,
+// Stages/BulgeBackTransformation.swift:11-85): Z <- Q2 Z, Q2 the chase-order product of H(s, k) = I - w w^T, the
+// reflector of sweep s at bulge k acting on rows rs = s + 1 + k b ..< rs + b (w = sqrt(tau) v as the chase stores
+// it). The reflectors of g consecutive sweeps s0 ..< s0 + g at one k form a block on rows row0 ..< row0 + g + b - 1,
+// applied last sweep first. Blocks in the order k ascending, s0 descending are a valid application order: relative
+// to the reverse chase order only pairs on disjoint rows change places ((s, k) and (s', k') with s < s', k < k' start
+// at least b + 1 rows apart). The host levels that sequence: a block's stage is one more than the latest stage of an
+// earlier block sharing a row with it, so the blocks of a stage act on disjoint rows and need only earlier stages.
+// One workgroup of BULGE_WAVES wavefronts owns 64 columns of Z (lane = column) and walks the stages with a barrier
+// between them; its wavefronts take a stage's blocks round robin. A block's g + b - 1 rows sit in registers and its
+// reflector entries in the wavefront's LDS slot (read as broadcasts), so a reflector costs 2b FMAs per lane and no
+// shuffles.
+//
+// Band panels (the reference's backTransformBand, Stages/BandFormBackTransformation.swift:10-21): Z <- P_0 ... P_p Z
+// with P_j = I - V_j T_j V_j^T. ppg consecutive panels merge into one block I - V T V^T, V = [V_a V_b ...] and T
+// upper block triangular (T(0:c, a) = -T(0:c, 0:c) (V^T V)(0:c, a) T_aa, the merge of two compact-WY factors),
+// applied as three products on the matrix cores: W = zt(:, r0:) V (split over rows), W2 = W T^T,
+// zt(:, r0:) -= W2 V^T.
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace xtbeig {
+
+namespace {
+
+constexpr int BULGE_G = 32;     // sweeps per bulge block
+constexpr int BULGE_WAVES = 8;  // wavefronts per workgroup
+constexpr int GROUP_K = 64;     // reflectors per merged panel block
+constexpr int SPLIT_KC = 128;   // rows per chunk of the split product W = zt V
+
+template <typename T, int B>
+__global__ __launch_bounds__(64 * BULGE_WAVES) void bulge_apply_kernel(int n, int b, int m,
+                                                                       const BulgeBlock* __restrict__ blocks,
+                                                                       const int* __restrict__ stage_off,
+                                                                       const int* __restrict__ order, int nstages,
+                                                                       const T* __restrict__ v2, int ldv2,
+                                                                       T* __restrict__ zt, int ldzt) {
+  constexpr int H = BULGE_G + B - 1;
+  constexpr int WE = BULGE_G * B;  // reflector entries of one block
+  // reflector j at ws[j WS ..< j WS + B]; the gap keeps the compiler from merging reads across reflectors
+  constexpr int WS = B + 16 / (int)sizeof(T);
+  __shared__ T wsh[BULGE_WAVES][BULGE_G * WS];
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col = blockIdx.x * 64 + lane;  // < ldzt; columns past m are padding
+  T* ws = wsh[wid];
+  for (int st = 0; st < nstages; ++st) {
+    const int beg = stage_off[st], end = stage_off[st + 1];
+    for (int q = beg + wid; q < end; q += BULGE_WAVES) {
+      const BulgeBlock bk = blocks[order[q]];
+      const int r0 = bk.row0, cnt = bk.count, s0 = bk.s0;
+      // the block's reflectors into this wavefront's LDS slot, entry r of sweep s0 + j at ws[j WS + r], zero past
+      // the reflector's length and for absent sweeps (j >= cnt)
+#pragma unroll
+      for (int u = 0; u < WE / 64; ++u) {
+        const int e = lane + 64 * u, j = e / B, r = e % B;
+        const int rs = r0 + j;
+        T x = 0;
+        if (j < cnt && r < min(b, n - rs)) x = v2[(int64_t)(rs + r) + (int64_t)ldv2 * (s0 + j)];
+        ws[j * WS + r] = x;
+      }
+      __builtin_amdgcn_wave_barrier();
+      // the window: rows r0 ..< r0 + H (the stage's footprint of this block; rows past n are zero padding)
+      T z[H];
+#pragma unroll
+      for (int i = 0; i < H; ++i) z[i] = zt[(int64_t)(r0 + i) * ldzt + col];
+#pragma unroll
+      for (int j = BULGE_G - 1; j >= 0; --j) {
+        T wr[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) wr[r] = ws[j * WS + r];
+        T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+#pragma unroll
+        for (int r = 0; r < B; r += 4) {
+          d0 += wr[r] * z[j + r];
+          d1 += wr[r + 1] * z[j + r + 1];
+          d2 += wr[r + 2] * z[j + r + 2];
+          d3 += wr[r + 3] * z[j + r + 3];
+        }
+        const T dot = (d0 + d1) + (d2 + d3);
+#pragma unroll
+        for (int r = 0; r < B; ++r) z[j + r] -= wr[r] * dot;
+        // compiler-only fence: the next reflector's LDS reads are not hoisted above this one (register pressure)
+        __asm__ volatile("" ::: "memory");
+      }
+#pragma unroll
+      for (int i = 0; i < H; ++i) zt[(int64_t)(r0 + i) * ldzt + col] = z[i];
+      __builtin_amdgcn_wave_barrier();  // ws is rewritten by the next block
+    }
+    __syncthreads();
+  }
+}
+
+// Merged T of panel group blockIdx.x (panels j0 + p b, p < ppg, j <= last): the Gram V^T V over rows r0 ..< n,
+// streamed in 32-row chunks (each thread a 4 x 4 tile), then the block recurrence. Output tg (64 x 64, zero outside
+// the group's K columns).
+template <typename T>
+__global__ __launch_bounds__(256) void group_t_kernel(int n, int b, int ppg, int last, const T* __restrict__ v1,
+                                                      const T* __restrict__ t1, T* __restrict__ tg) {
+  extern __shared__ char smem_raw[];  // three GROUP_K x (GROUP_K + 1) arrays
+  using Row = T[GROUP_K + 1];
+  Row* G = reinterpret_cast<Row*>(smem_raw);
+  Row* Tm = G + GROUP_K;
+  Row* X = Tm + GROUP_K;
+  const int g = blockIdx.x, t = threadIdx.x;
+  const int j0 = g * ppg * b, r0 = j0 + b;
+  int q = 0;
+  while (q < ppg && j0 + q * b <= last) ++q;
+  const int K = (q - 1) * b + min(b, n - (j0 + (q - 1) * b + b));
+  {
+    T* Vs = &X[0][0];  // 32 x 64 chunk, ld GROUP_K + 1
+    const int ta = (t & 15) * 4, tc = (t >> 4) * 4;
+    T acc[4][4] = {};
+    for (int i0 = r0; i0 < n; i0 += 32) {
+      for (int e = t; e < 32 * GROUP_K; e += 256) {
+        const int i = e & 31, c = e >> 5;
+        Vs[i * (GROUP_K + 1) + c] = (i0 + i < n && c < K) ? v1[(int64_t)(i0 + i) + (int64_t)(j0 + c) * n] : T(0);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int i = 0; i < 32; ++i) {
+        T va[4], vc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          va[u] = Vs[i * (GROUP_K + 1) + ta + u];
+          vc[u] = Vs[i * (GROUP_K + 1) + tc + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] += va[u] * vc[v];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) G[ta + u][tc + v] = acc[u][v];
+  }
+  for (int e = t; e < GROUP_K * GROUP_K; e += 256) {
+    const int r = e & (GROUP_K - 1), c = e / GROUP_K;
+    const int pr = r / b, pc = c / b;
+    T val = 0;
+    if (pr == pc && r < K && c < K && r <= c)
+      val = t1[(int64_t)(j0 + pr * b) * b + (r - pr * b) + (int64_t)(c - pc * b) * b];
+    Tm[r][c] = val;
+  }
+  __syncthreads();
+  for (int p = 1; p < q; ++p) {
+    const int c0 = p * b, kp = min(b, K - c0);
+    // X(0:c0, 0:kp) = G(0:c0, c0:c0+kp) T_pp
+    for (int e = t; e < c0 * kp; e += 256) {
+      const int r = e % c0, c = e / c0;
+      T sum = 0;
+      for (int l = 0; l <= c; ++l) sum += G[r][c0 + l] * Tm[c0 + l][c0 + c];
+      X[r][c] = sum;
+    }
+    __syncthreads();
+    // T(0:c0, c0:c0+kp) = -T(0:c0, 0:c0) X
+    for (int e = t; e < c0 * kp; e += 256) {
+      const int r = e % c0, c = e / c0;
+      T sum = 0;
+      for (int l = r; l < c0; ++l) sum += Tm[r][l] * X[l][c];
+      Tm[r][c0 + c] = -sum;
+    }
+    __syncthreads();
+  }
+  T* out = tg + (int64_t)g * GROUP_K * GROUP_K;
+  for (int e = t; e < GROUP_K * GROUP_K; e += 256) {
+    const int r = e & (GROUP_K - 1), c = e / GROUP_K;
+    out[e] = Tm[r][c];
+  }
+}
+
+// W2 (m x K, ld m) = (sum_z P_z) T^T for COMBINE_ROWS rows per workgroup; P_z (m x K, ld m) at P + z m K.
+constexpr int COMBINE_ROWS = 32;
+template <typename T>
+__global__ __launch_bounds__(256) void group_combine_kernel(int m, int K, int splits, const T* __restrict__ P,
+                                                            const T* __restrict__ tg, T* __restrict__ w2) {
+  __shared__ T Ts[GROUP_K][GROUP_K + 1];
+  __shared__ T Ws[COMBINE_ROWS][GROUP_K + 1];
+  const int i0 = blockIdx.x * COMBINE_ROWS, t = threadIdx.x;
+  for (int e = t; e < GROUP_K * GROUP_K; e += 256) Ts[e & (GROUP_K - 1)][e / GROUP_K] = tg[e];
+  const int64_t mk = (int64_t)m * K;
+  for (int e = t; e < COMBINE_ROWS * K; e += 256) {
+    const int i = e % COMBINE_ROWS, l = e / COMBINE_ROWS;
+    T sum = 0;
+    if (i0 + i < m)
+      for (int z = 0; z < splits; ++z) sum += P[z * mk + (i0 + i) + (int64_t)l * m];
+    Ws[i][l] = sum;
+  }
+  __syncthreads();
+  for (int e = t; e < COMBINE_ROWS * K; e += 256) {
+    const int i = e % COMBINE_ROWS, c = e / COMBINE_ROWS;
+    if (i0 + i >= m) continue;
+    T s0 = 0, s1 = 0;
+    int l = c;
+    for (; l + 2 <= K; l += 2) {
+      s0 += Ws[i][l] * Ts[c][l];
+      s1 += Ws[i][l + 1] * Ts[c][l + 1];
+    }
+    for (; l < K; ++l) s0 += Ws[i][l] * Ts[c][l];
+    w2[(i0 + i) + (int64_t)c * m] = s0 + s1;
+  }
+}
+
+template <typename T, int B>
+void launch_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m) {
+  hipLaunchKernelGGL((bulge_apply_kernel<T, B>), dim3(cdiv(m, 64)), dim3(64 * BULGE_WAVES), 0, s, p.n, p.b, m,
+                     p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt);
+  XTB_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+int bulge_width_class(int b) { return b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : 32; }
+
+BackPlan::BackPlan(int n_, int b_) : n(n_), b(b_), g(BULGE_G) {
+  if (b > 32) throw std::runtime_error("back-transform: band width above 32");
+  // bulge blocks in a valid application order: k ascending, s0 descending. A block's footprint is the kernel's
+  // window, g + B - 1 rows (B = b rounded up to 4, 8, 16 or 32), running into the zero padding below row n.
+  std::vector<BulgeBlock> bl;
+  const int sweeps = n - 2;
+  const int foot = g + bulge_width_class(b) - 1;
+  if (b > 1 && sweeps > 0) {
+    for (int k = 0; 1 + k * b < n; ++k) {
+      for (int s0 = ((sweeps - 1) / g) * g; s0 >= 0; s0 -= g) {
+        const int r0 = s0 + 1 + k * b;
+        if (r0 >= n) continue;
+        BulgeBlock w{};
+        w.row0 = r0;
+        w.height = foot;
+        w.count = std::min(std::min(g, sweeps - s0), n - r0);
+        w.s0 = s0;
+        bl.push_back(w);
+      }
+    }
+  }
+  nbulge = (int)bl.size();
+  if (nbulge) {
+    std::vector<int> rowstage(n + foot, 0), stage(nbulge);
+    for (int q = 0; q < nbulge; ++q) {
+      int st = 0;
+      for (int r = bl[q].row0; r < bl[q].row0 + bl[q].height; ++r) st = std::max(st, rowstage[r]);
+      stage[q] = st + 1;
+      for (int r = bl[q].row0; r < bl[q].row0 + bl[q].height; ++r) rowstage[r] = st + 1;
+      nstages = std::max(nstages, st + 1);
+    }
+    std::vector<int> off(nstages + 1, 0), order(nbulge);
+    for (int q = 0; q < nbulge; ++q) off[stage[q]]++;  // stage is 1-based: counts land at off[1..]
+    for (int st = 0; st < nstages; ++st) {
+      max_stage = std::max(max_stage, off[st + 1]);
+      off[st + 1] += off[st];
+    }
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int q = 0; q < nbulge; ++q) order[fill[stage[q] - 1]++] = q;
+    XTB_HIP(hipMalloc(&blocks_dev, sizeof(BulgeBlock) * nbulge));
+    XTB_HIP(hipMalloc(&stage_off_dev, sizeof(int) * (nstages + 1)));
+    XTB_HIP(hipMalloc(&order_dev, sizeof(int) * nbulge));
+    XTB_HIP(hipMemcpy(blocks_dev, bl.data(), sizeof(BulgeBlock) * nbulge, hipMemcpyHostToDevice));
+    XTB_HIP(hipMemcpy(stage_off_dev, off.data(), sizeof(int) * (nstages + 1), hipMemcpyHostToDevice));
+    XTB_HIP(hipMemcpy(order_dev, order.data(), sizeof(int) * nbulge, hipMemcpyHostToDevice));
+  }
+  for (int j = 0; j < n - b - 1; j += b) last = j;
+  ppg = std::max(1, GROUP_K / std::max(b, 1));
+  ngroups = last < 0 ? 0 : last / (ppg * b) + 1;
+}
+
+BackPlan::~BackPlan() {
+  if (blocks_dev) (void)hipFree(blocks_dev);
+  if (stage_off_dev) (void)hipFree(stage_off_dev);
+  if (order_dev) (void)hipFree(order_dev);
+}
+
+int zt_ld(int m) { return cdiv(std::max(m, 1), 64) * 64; }
+int zt_rows(int n) { return n + 64; }
+
+template <typename T>
+size_t back_band_workspace(int n, int b, int m) {
+  const int ppg = std::max(1, GROUP_K / std::max(b, 1));
+  const size_t groups = (size_t)cdiv(std::max(n, 1), ppg * std::max(b, 1)) + 1;
+  return groups * GROUP_K * GROUP_K + (size_t)m * GROUP_K * (cdiv(std::max(n, 1), SPLIT_KC) + 1) + 256;
+}
+
+template <typename T>
+void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m) {
+  if (p.nbulge == 0 || m <= 0) return;
+  if (ldzt < zt_ld(m)) throw std::runtime_error("bulge back-transform: zt leading dimension below zt_ld(m)");
+  switch (bulge_width_class(p.b)) {
+    case 4: launch_bulges<T, 4>(s, p, v2, zt, ldzt, m); break;
+    case 8: launch_bulges<T, 8>(s, p, v2, zt, ldzt, m); break;
+    case 16: launch_bulges<T, 16>(s, p, v2, zt, ldzt, m); break;
+    default: launch_bulges<T, 32>(s, p, v2, zt, ldzt, m); break;
+  }
+}
+
+template <typename T>
+void back_transform_band(hipStream_t s, const BackPlan& p, const T* v1, const T* t1, T* zt, int ldzt, int m,
+                         T* work) {
+  if (p.ngroups == 0 || m <= 0) return;
+  const int n = p.n, b = p.b;
+  T* tg = work;
+  T* w2 = tg + (size_t)(p.ngroups + 1) * GROUP_K * GROUP_K;
+  T* parts = w2 + (size_t)m * GROUP_K;
+  const size_t lds = 3 * sizeof(T) * GROUP_K * (GROUP_K + 1);
+  allow_lds(group_t_kernel<T>, lds);
+  hipLaunchKernelGGL(group_t_kernel<T>, dim3(p.ngroups), dim3(256), lds, s, n, b, p.ppg, p.last, v1, t1, tg);
+  XTB_LAUNCH_CHECK();
+  for (int g = p.ngroups - 1; g >= 0; --g) {
+    const int j0 = g * p.ppg * b, r0 = j0 + b, h = n - r0;
+    int q = 0;
+    while (q < p.ppg && j0 + q * b <= p.last) ++q;
+    const int K = (q - 1) * b + std::min(b, n - (j0 + (q - 1) * b + b));
+    const T* V = v1 + (int64_t)r0 + (int64_t)j0 * n;
+    const int splits = gemm_splitk<T>(s, m, K, h, zt + (int64_t)r0 * ldzt, ldzt, V, n, parts, SPLIT_KC);
+    hipLaunchKernelGGL(group_combine_kernel<T>, dim3(cdiv(m, COMBINE_ROWS)), dim3(256), 0, s, m, K, splits, parts,
+                       tg + (size_t)g * GROUP_K * GROUP_K, w2);
+    XTB_LAUNCH_CHECK();
+    gemm<T>(s, 'N', 'T', m, h, K, T(-1), w2, m, V, n, T(1), zt + (int64_t)r0 * ldzt, ldzt);
+  }
+}
+
+template size_t back_band_workspace<double>(int, int, int);
+template size_t back_band_workspace<float>(int, int, int);
+template void back_transform_bulges<double>(hipStream_t, const BackPlan&, const double*, double*, int, int);
+template void back_transform_bulges<float>(hipStream_t, const BackPlan&, const float*, float*, int, int);
+template void back_transform_band<double>(hipStream_t, const BackPlan&, const double*, const double*, double*, int,
+                                          int, double*);
+template void back_transform_band<float>(hipStream_t, const BackPlan&, const float*, const float*, float*, int, int,
+                                         float*);
+
+}  // namespace xtbeig

@@ -7,8 +7,9 @@
 //      Writes R into the band (zeros below), V explicitly (unit diagonal, zeros above) into the back-transform store
 //      v1 and into the update buffer [V | . | V], and tau.
 //   2. panel_t_kernel: G = V^T V, the forward columnwise T (I - V T V^T = H_1 ... H_k) and VT = V T.
-//   3. X = A2 VT (GEMM); panel_finish_kernel: Y = VT^T X, W = X - 1/2 V Y; A2 -= [V W] [W V]^T, one rank-2k GEMM
-//      on the matrix cores (full storage: both triangles of the trailing block stay current).
+//   3. X = A2 VT (split-K GEMM, partials summed in fixed order); panel_finish_kernel: Y = VT^T X, W = X - 1/2 V Y;
+//      A2 -= [V W] [W V]^T, one rank-2k GEMM on the matrix cores (full storage: both triangles of the trailing block
+//      stay current).
 // After the last panel the lower band is copied into the chase's band storage (element (i, j), 0 <= i - j <= 2b - 1,
 // at i + (2b - 1) j; see chase.hip).
 #include "common.hpp"
@@ -157,25 +158,56 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
   }
 }
 
+constexpr int CH = 128;  // rows staged in LDS per step of the panel reductions
+
+// out (k x k, LDS, ld 65) = A^T B for A, B m x k (ld m), k <= 32: rows staged through LDS CH at a time, thread o owns
+// outputs o, o + 256, ... (sym: only r <= c).
+template <typename T>
+__device__ inline void block_atb(int m, int k, const T* __restrict__ A, const T* __restrict__ B, T (*out)[65],
+                                 T (*sa)[CH + 1], T (*sb)[CH + 1], bool sym) {
+  const int t = threadIdx.x;
+  T acc[4] = {0, 0, 0, 0};
+  for (int i0 = 0; i0 < m; i0 += CH) {
+    const int hr = min(CH, m - i0);
+    __syncthreads();
+    for (int e = t; e < hr * k; e += 256) {
+      const int i = e % hr, c = e / hr;
+      sa[c][i] = A[(int64_t)(i0 + i) + (int64_t)c * m];
+      if (!sym) sb[c][i] = B[(int64_t)(i0 + i) + (int64_t)c * m];
+    }
+    __syncthreads();
+    const T(*bb)[CH + 1] = sym ? sa : sb;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int o = t + 256 * u;
+      if (o < k * k) {
+        const int r = o % k, c = o / k;
+        if (!sym || r <= c) {
+          T sum = 0;
+          for (int i = 0; i < hr; ++i) sum += sa[r][i] * bb[c][i];
+          acc[u] += sum;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int o = t + 256 * u;
+    if (o < k * k) out[o % k][o / k] = acc[u];
+  }
+  __syncthreads();
+}
+
 // After the panel QR, one workgroup: G = V^T V, the forward columnwise T (I - V T V^T = H_1 ... H_k, stored in tt, ld
-// ldt), and VT = V T (m x k, ld m). v: m x k explicit (ld m), tau (k).
+// ldt), and VT = V T (m x k, ld m). v: m x k explicit (ld m), tau (k), k <= 32.
 template <typename T>
 __global__ __launch_bounds__(256) void panel_t_kernel(int m, int k, const T* __restrict__ v, const T* __restrict__ tau,
                                                       T* __restrict__ tt, int ldt, T* __restrict__ vt) {
-  __shared__ T G[64][65];
-  __shared__ T Ts[64][65];
+  __shared__ T G[32][65];
+  __shared__ T Ts[32][65];
+  __shared__ T sa[32][CH + 1];
   const int t = threadIdx.x;
-  for (int e = t; e < k * k; e += blockDim.x) {
-    const int r = e % k, c = e / k;
-    if (r < c) {
-      const T* vr = v + (int64_t)r * m;
-      const T* vc = v + (int64_t)c * m;
-      T sum = 0;
-      for (int i = 0; i < m; ++i) sum += vr[i] * vc[i];
-      G[r][c] = sum;
-    }
-  }
-  __syncthreads();
+  block_atb<T>(m, k, v, v, G, sa, sa, true);
   for (int c = 0; c < k; ++c) {
     if (t < c) {
       T sum = 0;
@@ -189,35 +221,49 @@ __global__ __launch_bounds__(256) void panel_t_kernel(int m, int k, const T* __r
     const int r = e % k, c = e / k;
     tt[r + (int64_t)c * ldt] = (r <= c) ? Ts[r][c] : T(0);
   }
-  // VT = V T
-  for (int e = t; e < m * k; e += blockDim.x) {
-    const int i = e % m, c = e / m;
-    T sum = 0;
-    for (int l = 0; l <= c; ++l) sum += v[i + (int64_t)l * m] * Ts[l][c];
-    vt[e] = sum;
+  // VT = V T, one row per thread (coalesced column loads)
+  for (int i = t; i < m; i += blockDim.x) {
+    T row[32];
+#pragma unroll
+    for (int l = 0; l < 32; ++l) row[l] = (l < k) ? v[i + (int64_t)l * m] : T(0);
+    for (int c = 0; c < k; ++c) {
+      T sum = 0;
+#pragma unroll
+      for (int l = 0; l < 32; ++l)
+        if (l <= c) sum += row[l] * Ts[l][c];
+      vt[i + (int64_t)c * m] = sum;
+    }
   }
 }
 
-// Y = VT^T X (k x k), then W = X - V Y / 2 in place of X (m x k). One workgroup.
+// X = sum of the split-K partials (nsplit, each m x k); Y = VT^T X (k x k); W = X - V Y / 2 into x. One workgroup.
 template <typename T>
 __global__ __launch_bounds__(256) void panel_finish_kernel(int m, int k, const T* __restrict__ v,
-                                                           const T* __restrict__ vt, T* __restrict__ x) {
-  __shared__ T Y[64][65];
+                                                           const T* __restrict__ vt, const T* __restrict__ xpart,
+                                                           int nsplit, T* __restrict__ x) {
+  __shared__ T Y[32][65];
+  __shared__ T sa[32][CH + 1];
+  __shared__ T sb[32][CH + 1];
   const int t = threadIdx.x;
-  for (int e = t; e < k * k; e += blockDim.x) {
-    const int r = e % k, c = e / k;
-    const T* a = vt + (int64_t)r * m;
-    const T* bb = x + (int64_t)c * m;
+  const int64_t mk = (int64_t)m * k;
+  for (int64_t e = t; e < mk; e += blockDim.x) {
     T sum = 0;
-    for (int i = 0; i < m; ++i) sum += a[i] * bb[i];
-    Y[r][c] = sum;
+    for (int p = 0; p < nsplit; ++p) sum += xpart[e + p * mk];
+    x[e] = sum;
   }
   __syncthreads();
-  for (int e = t; e < m * k; e += blockDim.x) {
-    const int i = e % m, c = e / m;
-    T sum = 0;
-    for (int l = 0; l < k; ++l) sum += v[i + (int64_t)l * m] * Y[l][c];
-    x[e] -= T(0.5) * sum;
+  block_atb<T>(m, k, vt, x, Y, sa, sb, false);
+  for (int i = t; i < m; i += blockDim.x) {
+    T row[32];
+#pragma unroll
+    for (int l = 0; l < 32; ++l) row[l] = (l < k) ? v[i + (int64_t)l * m] : T(0);
+    for (int c = 0; c < k; ++c) {
+      T sum = 0;
+#pragma unroll
+      for (int l = 0; l < 32; ++l)
+        if (l < k) sum += row[l] * Y[l][c];
+      x[i + (int64_t)c * m] -= T(0.5) * sum;
+    }
   }
 }
 
@@ -256,18 +302,23 @@ void launch_panel_qr(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv
 
 }  // namespace
 
+constexpr int SPLIT_ROWS = 128;  // K per split of the X = A2 VT product
+
 template <typename T>
 size_t band_workspace(int n, int b) {
-  // tau (b) | buf [V W V] (3 n b) | G (b b) | Y (b b) | VT (n b)
-  return (size_t)b + 3 * (size_t)n * b + 2 * (size_t)b * b + (size_t)n * b + 64;
+  // tau (b) | buf [V W V] (3 n b) | VT (n b) | split-K partials of X (ceil(n / SPLIT_ROWS) n b)
+  return (size_t)b + 4 * (size_t)n * b + (size_t)((n + SPLIT_ROWS - 1) / SPLIT_ROWS) * n * b + 64;
 }
 
 template <typename T>
 void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T* ab, T* work) {
-  if (b > 64 || b < 1) throw std::runtime_error("reduce_to_band: band width must be in [1, 64]");
+  if (b > 32 || b < 1) throw std::runtime_error("reduce_to_band: band width must be in [1, 32]");
   T* tau = work;
   T* buf = tau + b;
   T* vt = buf + 3 * (size_t)n * b;
+  T* xpart = vt + (size_t)n * b;
+  // v1 is zero outside the panels' V, so that consecutive panels read as one block [V_a V_b ...] from row r_a
+  set_zero<T>(s, n, n, v1, n);
   for (int j = 0; j < n - b - 1; j += b) {
     const int r = j + b, m = n - r, k = std::min(b, m);
     T* panel = a + idx(r, j, lda);
@@ -278,8 +329,8 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
     T* x = buf + (size_t)m * k;  // slot 1 of [V | W | V]
     hipLaunchKernelGGL(panel_t_kernel<T>, dim3(1), dim3(256), 0, s, m, k, buf, tau, tt, b, vt);  // T, VT = V T
     XTB_LAUNCH_CHECK();
-    gemm<T>(s, 'N', 'N', m, k, m, T(1), a2, lda, vt, m, T(0), x, m);                           // X = A2 VT
-    hipLaunchKernelGGL(panel_finish_kernel<T>, dim3(1), dim3(256), 0, s, m, k, buf, vt, x);     // W = X - V Y / 2
+    const int nsplit = gemm_splitk<T>(s, m, k, m, a2, lda, vt, m, xpart, SPLIT_ROWS);            // A2 VT in parts
+    hipLaunchKernelGGL(panel_finish_kernel<T>, dim3(1), dim3(256), 0, s, m, k, buf, vt, xpart, nsplit, x);
     XTB_LAUNCH_CHECK();
     gemm<T>(s, 'N', 'T', m, m, 2 * k, T(-1), buf, m, buf + (size_t)m * k, m, T(1), a2, lda);  // A2 -= [V W][W V]^T
   }

@@ -92,10 +92,20 @@ int reduce_to_band_impl(xtbeig_handle_t handle, int n, int b, T* a, int lda, T* 
   });
 }
 
+// The back-transformations work on Z^T: the column-major z is transposed into the arena and back.
 template <typename T>
 int back_band_impl(xtbeig_handle_t handle, int n, int b, const T* v1, const T* t1, T* z, int ldz, int m) {
   return guarded([&] {
-    xtbeig::back_transform_band_wy<T>(handle->h.stream, handle->h.wyplan(n, b), v1, t1, z, ldz, m);
+    Handle& h = handle->h;
+    if (n <= 0 || m <= 0) return;
+    const xtbeig::BackPlan& p = h.backplan(n, b);
+    const int ldzt = xtbeig::zt_ld(m);
+    const size_t zt_elems = (size_t)ldzt * xtbeig::zt_rows(n);
+    h.reserve((zt_elems + xtbeig::back_band_workspace<T>(n, b, m)) * sizeof(T) + 8192);
+    T* zt = static_cast<T*>(h.arena);
+    xtbeig::transpose_matrix<T, T>(h.stream, n, m, z, ldz, zt, ldzt);
+    xtbeig::back_transform_band<T>(h.stream, p, v1, t1, zt, ldzt, m, zt + zt_elems);
+    xtbeig::transpose_matrix<T, T>(h.stream, m, n, zt, ldzt, z, ldz);
   });
 }
 
@@ -103,10 +113,15 @@ template <typename T>
 int back_bulges_impl(xtbeig_handle_t handle, int n, int b, const T* v2, T* z, int ldz, int m) {
   return guarded([&] {
     Handle& h = handle->h;
-    const xtbeig::WyPlan& p = h.wyplan(n, b);
-    h.reserve((p.v_elems() + p.t_elems()) * sizeof(T) + 8192);
-    T* vw = static_cast<T*>(h.arena);
-    xtbeig::back_transform_bulges_wy<T>(h.stream, p, v2, z, ldz, m, vw, vw + p.v_elems());
+    if (n <= 0 || m <= 0) return;
+    const xtbeig::BackPlan& p = h.backplan(n, b);
+    const int ldzt = xtbeig::zt_ld(m);
+    h.reserve((size_t)ldzt * xtbeig::zt_rows(n) * sizeof(T) + 8192);
+    T* zt = static_cast<T*>(h.arena);
+    xtbeig::set_zero<T>(h.stream, ldzt, xtbeig::zt_rows(n) - n, zt + (size_t)n * ldzt, ldzt);
+    xtbeig::transpose_matrix<T, T>(h.stream, n, m, z, ldz, zt, ldzt);
+    xtbeig::back_transform_bulges<T>(h.stream, p, v2, zt, ldzt, m);
+    xtbeig::transpose_matrix<T, T>(h.stream, m, n, zt, ldzt, z, ldz);
   });
 }
 

@@ -34,7 +34,7 @@ template <typename T>
 struct Band {
   T* ab;
   int ld;  // 2b
-  __device__ inline T& operator()(int i, int j) const { return ab[(int64_t)i + (int64_t)ld * j]; }
+  __device__ inline T& operator()(int i, int j) const { return ab[i + ld * j]; }  // band_elems < 2^31
 };
 
 // One bulge (s, k) by one wavefront. Returns false when the sweep has no such bulge.
@@ -110,10 +110,7 @@ __device__ inline bool chase_bulge(int n, int b, const Band<T>& A, T* __restrict
 // Sums over a column (row) run across its LPC lanes (lane-xor offsets below LPC).
 template <typename T, int B>
 __device__ inline T group_sum(T v) {
-  constexpr int LPC = 64 / B;
-#pragma unroll
-  for (int off = 1; off < LPC; off <<= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return group_sum_lanes<64 / B>(v);
 }
 
 template <typename T, int B>

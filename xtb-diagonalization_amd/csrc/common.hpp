@@ -40,11 +40,39 @@ __host__ __device__ inline int64_t idx(int64_t i, int64_t j, int64_t ld) { retur
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ---- wavefront (64-lane) reductions -------------------------------------------------------------
+// DPP moves within rows of 16 lanes (quad permutes, half-row and row mirrors), for 32-bit lanes; 64-bit values move
+// as two halves.
+template <int CTRL>
+__device__ inline float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1, 0, 3, 2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2, 3, 0, 1]
+constexpr int DPP_HALF_MIRROR = 0x141;  // lane i <-> 7 - i within 8
+constexpr int DPP_MIRROR = 0x140;       // lane i <-> 15 - i within 16
+
+// Sum over aligned groups of G lanes (G a power of two <= 64); every lane of a group receives its group's total.
+// The mirror steps pair whole sub-groups once those are uniform.
+template <int G, typename T>
+__device__ inline T group_sum_lanes(T v) {
+  if (G >= 2) v += dpp_mov<DPP_XOR1>(v);
+  if (G >= 4) v += dpp_mov<DPP_XOR2>(v);
+  if (G >= 8) v += dpp_mov<DPP_HALF_MIRROR>(v);
+  if (G >= 16) v += dpp_mov<DPP_MIRROR>(v);
+  if (G >= 32) v += __shfl_xor(v, 16, 64);
+  if (G >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
 template <typename T>
 __device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return group_sum_lanes<64>(v);
 }
 template <typename T>
 __device__ inline T wave_max(T v) {

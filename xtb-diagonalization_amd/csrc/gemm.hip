@@ -63,6 +63,34 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
   else launch<T, true, true>(s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
 }
 
+// C_z = op(A)(:, K_z) op(B)(K_z, :) for K split into chunks of kc: partial products written to c + z M N (ld M), one
+// grid layer per chunk. Returns the number of chunks. A is M x K (ld lda), B K x N (ld ldb), no transposes.
+namespace {
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_splitk_kernel(int M, int N, int K, int kc, const T* __restrict__ A,
+                                                          int lda, const T* __restrict__ B, int ldb,
+                                                          T* __restrict__ C) {
+  __shared__ T As[BK][BM + PAD];
+  __shared__ T Bs[BK][BN + PAD];
+  const int z = blockIdx.z;
+  const int k0 = z * kc, kz = min(kc, K - k0);
+  gemm_tile<T, false, false>(M, N, kz, T(1), A + (int64_t)k0 * lda, lda, B + k0, ldb, T(0),
+                             C + (int64_t)z * M * N, M, blockIdx.x * BM, blockIdx.y * BN, As, Bs);
+}
+}  // namespace
+
+template <typename T>
+int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int kc) {
+  if (m <= 0 || n <= 0 || k <= 0) return 0;
+  const int splits = cdiv(k, kc);
+  hipLaunchKernelGGL(gemm_splitk_kernel<T>, dim3(cdiv(m, BM), cdiv(n, BN), splits), dim3(256), 0, s, m, n, k, kc, A,
+                     lda, B, ldb, C);
+  XTB_LAUNCH_CHECK();
+  return splits;
+}
+template int gemm_splitk<double>(hipStream_t, int, int, int, const double*, int, const double*, int, double*, int);
+template int gemm_splitk<float>(hipStream_t, int, int, int, const float*, int, const float*, int, float*, int);
+
 template void gemm<double>(hipStream_t, char, char, int, int, int, double, const double*, int, const double*, int,
                            double, double*, int, int, int64_t, int64_t, int64_t);
 template void gemm<float>(hipStream_t, char, char, int, int, int, float, const float*, int, const float*, int, float,

@@ -11,10 +11,16 @@ template <typename T>
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb,
           T beta, T* C, int ldc, int batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0);
 
+template <typename T>
+int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int kc);
+
 // util.hip ---------------------------------------------------------------------------------------------------------
 // dst(m x n) = src (with conversion); both column-major.
 template <typename S, typename D>
 void copy_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd);
+// dst (n x m) = src (m x n)^T (with conversion).
+template <typename S, typename D>
+void transpose_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd);
 // Mirror the referenced triangle of a (uplo 'U': upper is read) into the other one, in place.
 template <typename T>
 void symmetrize(hipStream_t s, char uplo, int n, T* a, int lda);
@@ -81,31 +87,40 @@ void dc_merge_probe(hipStream_t s, int n, int n1, double rho, double* d, double*
 }  // namespace xtbeig
 
 namespace xtbeig {
-// wy.hip -----------------------------------------------------------------------------------------------------------
-// One compact-WY block Z(row0 .. row0 + height, :) <- (I - V T V^T) Z(...): V height x count (ld ldv, at v_off),
-// T count x count upper (ld ldt, at t_off). s0, k: the bulge block's first sweep and bulge index.
-struct WyBlock {
-  int row0, height, count, s0, k, ldv, ldt, pad;
-  int64_t v_off, t_off;
+// backtransform.hip ------------------------------------------------------------------------------------------------
+// Both back-transformations of the two-stage reduction act on zt = Z^T (m x n, column-major, ld ldzt: row i of Z is
+// the contiguous column i of zt), so that a wavefront's lanes hold 64 consecutive eigenvectors.
+// Bulge reflectors: the chase's reflectors of g consecutive sweeps s0 ..< s0 + g at one bulge index k (rows
+// row0 ..< row0 + height) form one block; blocks are grouped into stages of row-disjoint blocks, each stage after
+// every block it overlaps.
+struct BulgeBlock {
+  int row0, height, count, s0;
 };
-// The block lists of the two back-transformations for one (n, b), uploaded once.
-struct WyPlan {
-  int n = 0, b = 0, g = 0, nbulge = 0, npanel = 0;
-  WyBlock* bulge_dev = nullptr;
-  WyBlock* panel_dev = nullptr;
-  WyPlan(int n, int b);
-  ~WyPlan();
-  WyPlan(const WyPlan&) = delete;
-  WyPlan& operator=(const WyPlan&) = delete;
-  size_t v_elems() const;  // workspace for the bulge blocks' V and T
-  size_t t_elems() const;
+struct BackPlan {
+  int n = 0, b = 0, g = 0;
+  int nbulge = 0, nstages = 0, max_stage = 0;
+  BulgeBlock* blocks_dev = nullptr;
+  int* stage_off_dev = nullptr;  // nstages + 1
+  int* order_dev = nullptr;      // block indices by stage
+  // band panels j = 0, b, ..., last, merged ppg at a time into groups of up to ppg * b reflectors
+  int last = -1, ppg = 1, ngroups = 0;
+  BackPlan(int n, int b);
+  ~BackPlan();
+  BackPlan(const BackPlan&) = delete;
+  BackPlan& operator=(const BackPlan&) = delete;
 };
-int bulge_group(int b);
 template <typename T>
-void back_transform_bulges_wy(hipStream_t s, const WyPlan& p, const T* v2, T* z, int ldz, int m, T* vwork,
-                              T* twork);
+size_t back_band_workspace(int n, int b, int m);  // elements of T
+// zt's shape for m eigenvectors of size n: leading dimension zt_ld(m) (m rounded up to 64) and zt_rows(n) columns,
+// the last zt_rows(n) - n of them zero (the bulge kernel's windows run into them).
+int zt_ld(int m);
+int zt_rows(int n);
+// zt <- zt Q2^T, i.e. Z <- Q2 Z (v2: the chase's reflectors w, H = I - w w^T, column s = sweep s).
 template <typename T>
-void back_transform_band_wy(hipStream_t s, const WyPlan& p, const T* v1, const T* t1, T* z, int ldz, int m);
+void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m);
+// zt <- zt Q1^T, i.e. Z <- Q1 Z (v1, t1: the panels of reduce_to_band; v1 zero outside the panels' V).
+template <typename T>
+void back_transform_band(hipStream_t s, const BackPlan& p, const T* v1, const T* t1, T* zt, int ldzt, int m, T* work);
 
 // refine.hip -------------------------------------------------------------------------------------------------------
 size_t refine_workspace(int n);  // doubles; iwork n ints

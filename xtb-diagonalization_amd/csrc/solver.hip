@@ -21,9 +21,10 @@ namespace xtbeig {
 // b = 8 up to n = 1290; beyond that b = 16 with the band in global memory.
 int default_band(int n, bool fp32) {
   if (n <= 3) return 1;
-  int b;
-  if (fp32) b = n <= 1280 ? 16 : (n <= 2600 ? 8 : 16);
-  else b = n <= 1290 ? 8 : 16;
+  // b = 16 in both precisions: the FP32 band of 8 leaves the refined C^T S C - I above 1e-12 n at n = 1204, and the
+  // FP64 band of 16 is the faster one there.
+  (void)fp32;
+  const int b = 16;
   return std::min(b, std::max(2, n - 1));
 }
 
@@ -75,13 +76,13 @@ const DcPlan& Handle::plan(int n) {
   return ref;
 }
 
-const WyPlan& Handle::wyplan(int n, int b) {
+const BackPlan& Handle::backplan(int n, int b) {
   const auto key = std::make_pair(n, b);
-  auto it = wyplans.find(key);
-  if (it != wyplans.end()) return *it->second;
-  auto p = std::make_unique<WyPlan>(n, b);
-  const WyPlan& ref = *p;
-  wyplans.emplace(key, std::move(p));
+  auto it = backplans.find(key);
+  if (it != backplans.end()) return *it->second;
+  auto p = std::make_unique<BackPlan>(n, b);
+  const BackPlan& ref = *p;
+  backplans.emplace(key, std::move(p));
   return ref;
 }
 
@@ -102,7 +103,7 @@ struct Carver {
 
 template <typename T>
 struct StdBuffers {
-  T *v1, *t1, *ab, *bwork, *v2, *dT, *eT, *wyv, *wyt;
+  T *v1, *t1, *ab, *bwork, *v2, *dT, *eT, *btwork, *zt;
   double *d64, *e64, *z64, *dcwork;
   int* dciwork;
 };
@@ -118,11 +119,8 @@ StdBuffers<T> carve_std(Carver& c, int n, int b) {
   s.v2 = c.take<T>(nn);
   s.dT = c.take<T>(n);
   s.eT = c.take<T>(n);
-  // bulge WY blocks: sized for the plan of (n, b) without building it here
-  const int g = bulge_group(b);
-  const size_t nblk = (size_t)((n + b - 1) / b + 1) * (size_t)((n + g - 1) / g + 1);
-  s.wyv = c.take<T>(nblk * (size_t)(b + g) * g + 1);
-  s.wyt = c.take<T>(nblk * (size_t)g * g + 1);
+  s.btwork = c.take<T>(back_band_workspace<T>(n, b, n));
+  s.zt = c.take<T>((size_t)zt_ld(n) * zt_rows(n));
   s.d64 = c.take<double>(n);
   s.e64 = c.take<double>(n);
   s.z64 = std::is_same<T, double>::value ? nullptr : c.take<double>(nn);
@@ -147,13 +145,15 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   double* z64 = std::is_same<T, double>::value ? reinterpret_cast<double*>(z) : s.z64;
   const int ldz64 = std::is_same<T, double>::value ? ldz : n;
   stedc(st, h.plan(n), s.d64, s.e64, z64, ldz64, s.dcwork, s.dciwork, h.dinfo + 2);
-  if (!std::is_same<T, double>::value) copy_matrix<double, T>(st, n, n, z64, n, z, ldz);
   copy_matrix<double, double>(st, n, 1, s.d64, n, w, n);
-  const WyPlan& wp = h.wyplan(n, b);
-  if (wp.v_elems() > ((size_t)((n + b - 1) / b + 1) * (size_t)((n + wp.g - 1) / wp.g + 1)) * (b + wp.g) * wp.g + 1)
-    throw std::runtime_error("syevd: bulge block workspace too small");
-  back_transform_bulges_wy<T>(st, wp, s.v2, z, ldz, n, s.wyv, s.wyt);
-  back_transform_band_wy<T>(st, wp, s.v1, s.t1, z, ldz, n);
+  // The back-transformations run on zt = Z^T (padded: zt_ld(n) x zt_rows(n), rows past n zero).
+  const int ldzt = zt_ld(n);
+  set_zero<T>(st, ldzt, zt_rows(n) - n, s.zt + (size_t)n * ldzt, ldzt);
+  transpose_matrix<double, T>(st, n, n, z64, ldz64, s.zt, ldzt);
+  const BackPlan& bp = h.backplan(n, b);
+  back_transform_bulges<T>(st, bp, s.v2, s.zt, ldzt, n);
+  back_transform_band<T>(st, bp, s.v1, s.t1, s.zt, ldzt, n, s.btwork);
+  transpose_matrix<T, T>(st, n, n, s.zt, ldzt, z, ldz);
 }
 
 template <typename T>

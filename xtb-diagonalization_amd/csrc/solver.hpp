@@ -22,7 +22,7 @@ struct Handle {
   int refine_iters = 2;
   int band_override = 0;
   std::map<int, std::unique_ptr<DcPlan>> plans;
-  std::map<std::pair<int, int>, std::unique_ptr<WyPlan>> wyplans;
+  std::map<std::pair<int, int>, std::unique_ptr<BackPlan>> backplans;
 
   explicit Handle(int dev);
   ~Handle();
@@ -31,7 +31,7 @@ struct Handle {
   void reserve(size_t bytes);
   void* host_staging(size_t bytes);
   const DcPlan& plan(int n);
-  const WyPlan& wyplan(int n, int b);
+  const BackPlan& backplan(int n, int b);
 };
 
 int default_band(int n, bool fp32);

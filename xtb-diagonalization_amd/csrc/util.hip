@@ -66,7 +66,33 @@ __global__ void symmetrize_pad_kernel(int n, const double* __restrict__ src, int
   }
 }
 
+// dst(j, i) = src(i, j) through 32 x 32 LDS tiles; src m x n, dst n x m.
+template <typename S, typename D>
+__global__ void transpose_kernel(int m, int n, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd) {
+  __shared__ D tile[32][33];
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    if (i < m && j < n) tile[r][tx] = D(src[idx(i, j, lds)]);
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + tx, i = i0 + r;  // dst row j, column i
+    if (i < m && j < n) dst[idx(j, i, ldd)] = tile[tx][r];
+  }
+}
+
 }  // namespace
+
+template <typename S, typename D>
+void transpose_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd) {
+  if (m <= 0 || n <= 0) return;
+  if (cdiv(n, 32) > 65535) throw std::runtime_error("transpose: matrix too wide");
+  hipLaunchKernelGGL((transpose_kernel<S, D>), dim3(cdiv(m, 32), cdiv(n, 32)), dim3(32, 8), 0, s, m, n, src, lds, dst,
+                     ldd);
+  XTB_LAUNCH_CHECK();
+}
 
 template <typename S, typename D>
 void copy_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd) {
@@ -105,6 +131,9 @@ void symmetrize_pad(hipStream_t s, int n, const double* src, int lds, double* ds
   XTB_LAUNCH_CHECK();
 }
 
+template void transpose_matrix<double, double>(hipStream_t, int, int, const double*, int, double*, int);
+template void transpose_matrix<double, float>(hipStream_t, int, int, const double*, int, float*, int);
+template void transpose_matrix<float, float>(hipStream_t, int, int, const float*, int, float*, int);
 template void copy_matrix<double, double>(hipStream_t, int, int, const double*, int, double*, int);
 template void copy_matrix<double, float>(hipStream_t, int, int, const double*, int, float*, int);
 template void copy_matrix<float, double>(hipStream_t, int, int, const float*, int, double*, int);
