@@ -6,8 +6,10 @@
 //      in sub-panels of SB columns held in registers (R rows per thread), earlier reflectors applied left-looking.
 //      Writes R into the band (zeros below), V explicitly (unit diagonal, zeros above) into the back-transform store
 //      v1 and into the update buffer [V | . | V], and tau.
-//   2. panel_t_kernel: G = V^T V, the forward columnwise T (I - V T V^T = H_1 ... H_k) and VT = V T.
-//   3. X = A2 VT (split-K GEMM, partials summed in fixed order); panel_finish_kernel: Y = VT^T X, W = X - 1/2 V Y;
+//   2. panel_gram_kernel + panel_tvt_kernel: G = V^T V, the forward columnwise T (I - V T V^T = H_1 ... H_k) and
+//      VT = V T, over row blocks of 64 on many workgroups.
+//   3. X = A2 VT (split-K GEMM, partials summed in fixed order); panel_xy_kernel + panel_w_kernel: Y = VT^T X,
+//      W = X - 1/2 V Y, again over row blocks;
 //      A2 -= [V W] [W V]^T, one rank-2k GEMM on the matrix cores (full storage: both triangles of the trailing block
 //      stay current).
 // After the last panel the lower band is copied into the chase's band storage (element (i, j), 0 <= i - j <= 2b - 1,
@@ -158,56 +160,75 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
   }
 }
 
-constexpr int CH = 128;  // rows staged in LDS per step of the panel reductions
+// The per-panel reductions over the m rows run on cdiv(m, PR) workgroups, PR rows each; k x k partial products per
+// row block go to a partial buffer (ld 32, 1024 elements per block) and are summed by the consumer in block order.
+constexpr int PR = 64;
 
-// out (k x k, LDS, ld 65) = A^T B for A, B m x k (ld m), k <= 32: rows staged through LDS CH at a time, thread o owns
-// outputs o, o + 256, ... (sym: only r <= c).
+// Rows i0 ..< i0 + PR of the m x k (ld m) matrix a into sh (PR x 33), zeros past m and k.
 template <typename T>
-__device__ inline void block_atb(int m, int k, const T* __restrict__ A, const T* __restrict__ B, T (*out)[65],
-                                 T (*sa)[CH + 1], T (*sb)[CH + 1], bool sym) {
-  const int t = threadIdx.x;
-  T acc[4] = {0, 0, 0, 0};
-  for (int i0 = 0; i0 < m; i0 += CH) {
-    const int hr = min(CH, m - i0);
-    __syncthreads();
-    for (int e = t; e < hr * k; e += 256) {
-      const int i = e % hr, c = e / hr;
-      sa[c][i] = A[(int64_t)(i0 + i) + (int64_t)c * m];
-      if (!sym) sb[c][i] = B[(int64_t)(i0 + i) + (int64_t)c * m];
-    }
-    __syncthreads();
-    const T(*bb)[CH + 1] = sym ? sa : sb;
+__device__ inline void stage_rows(int m, int k, int i0, const T* __restrict__ a, T (*sh)[33]) {
+  for (int e = threadIdx.x; e < PR * 32; e += blockDim.x) {
+    const int i = e % PR, c = e / PR;
+    sh[i][c] = (i0 + i < m && c < k) ? a[(int64_t)(i0 + i) + (int64_t)c * m] : T(0);
+  }
+}
+
+// out[r + 32 c] = sum_i A(i, r) B(i, c) over the staged rows (k x k; sym: r <= c only).
+template <typename T>
+__device__ inline void rows_atb(int k, const T (*A)[33], const T (*B)[33], T* __restrict__ out, bool sym) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int o = t + 256 * u;
-      if (o < k * k) {
-        const int r = o % k, c = o / k;
-        if (!sym || r <= c) {
-          T sum = 0;
-          for (int i = 0; i < hr; ++i) sum += sa[r][i] * bb[c][i];
-          acc[u] += sum;
+  for (int u = 0; u < 4; ++u) {
+    const int o = threadIdx.x + 256 * u;
+    if (o < k * k) {
+      const int r = o % k, c = o / k;
+      if (!sym || r <= c) {
+        T s0 = 0, s1 = 0;
+#pragma unroll 8
+        for (int i = 0; i < PR; i += 2) {
+          s0 += A[i][r] * B[i][c];
+          s1 += A[i + 1][r] * B[i + 1][c];
         }
+        out[r + 32 * c] = s0 + s1;
       }
     }
   }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int o = t + 256 * u;
-    if (o < k * k) out[o % k][o / k] = acc[u];
-  }
-  __syncthreads();
 }
 
-// After the panel QR, one workgroup: G = V^T V, the forward columnwise T (I - V T V^T = H_1 ... H_k, stored in tt, ld
-// ldt), and VT = V T (m x k, ld m). v: m x k explicit (ld m), tau (k), k <= 32.
+// k x k (ld 32) sum of the nblk partials into sh (32 x 33).
 template <typename T>
-__global__ __launch_bounds__(256) void panel_t_kernel(int m, int k, const T* __restrict__ v, const T* __restrict__ tau,
-                                                      T* __restrict__ tt, int ldt, T* __restrict__ vt) {
-  __shared__ T G[32][65];
-  __shared__ T Ts[32][65];
-  __shared__ T sa[32][CH + 1];
-  const int t = threadIdx.x;
-  block_atb<T>(m, k, v, v, G, sa, sa, true);
+__device__ inline void sum_partials(int k, int nblk, const T* __restrict__ part, T (*sh)[33], bool sym) {
+  for (int o = threadIdx.x; o < k * k; o += blockDim.x) {
+    const int r = o % k, c = o / k;
+    if (sym && r > c) continue;
+    T sum = 0;
+    for (int q = 0; q < nblk; ++q) sum += part[(int64_t)q * 1024 + r + 32 * c];
+    sh[r][c] = sum;
+  }
+}
+
+// Gp[blk] = V(rows)^T V(rows), upper triangle.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_gram_kernel(int m, int k, const T* __restrict__ v, T* __restrict__ gp) {
+  __shared__ T Vs[PR][33];
+  stage_rows<T>(m, k, blockIdx.x * PR, v, Vs);
+  __syncthreads();
+  rows_atb<T>(k, Vs, Vs, gp + (int64_t)blockIdx.x * 1024, true);
+}
+
+// G = sum of the Gram partials; the forward columnwise T (I - V T V^T = H_1 ... H_k; T(0:c, c) = -tau_c T(0:c, 0:c)
+// G(0:c, c), T(c, c) = tau_c), written by workgroup 0 to tt (ld ldt); VT(rows) = V(rows) T.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_tvt_kernel(int m, int k, const T* __restrict__ v,
+                                                        const T* __restrict__ tau, const T* __restrict__ gp, int nblk,
+                                                        T* __restrict__ tt, int ldt, T* __restrict__ vt) {
+  __shared__ T G[32][33];
+  __shared__ T Ts[32][33];
+  __shared__ T Vs[PR][33];
+  const int t = threadIdx.x, i0 = blockIdx.x * PR;
+  sum_partials<T>(k, nblk, gp, G, true);
+  stage_rows<T>(m, k, i0, v, Vs);
+  for (int e = t; e < 32 * 33; e += blockDim.x) (&Ts[0][0])[e] = T(0);
+  __syncthreads();
   for (int c = 0; c < k; ++c) {
     if (t < c) {
       T sum = 0;
@@ -217,53 +238,60 @@ __global__ __launch_bounds__(256) void panel_t_kernel(int m, int k, const T* __r
     if (t == 0) Ts[c][c] = tau[c];
     __syncthreads();
   }
-  for (int e = t; e < k * k; e += blockDim.x) {
-    const int r = e % k, c = e / k;
-    tt[r + (int64_t)c * ldt] = (r <= c) ? Ts[r][c] : T(0);
-  }
-  // VT = V T, one row per thread (coalesced column loads)
-  for (int i = t; i < m; i += blockDim.x) {
-    T row[32];
-#pragma unroll
-    for (int l = 0; l < 32; ++l) row[l] = (l < k) ? v[i + (int64_t)l * m] : T(0);
-    for (int c = 0; c < k; ++c) {
-      T sum = 0;
-#pragma unroll
-      for (int l = 0; l < 32; ++l)
-        if (l <= c) sum += row[l] * Ts[l][c];
-      vt[i + (int64_t)c * m] = sum;
+  if (blockIdx.x == 0)
+    for (int e = t; e < k * k; e += blockDim.x) {
+      const int r = e % k, c = e / k;
+      tt[r + (int64_t)c * ldt] = (r <= c) ? Ts[r][c] : T(0);
     }
+  for (int e = t; e < PR * k; e += blockDim.x) {
+    const int i = e % PR, c = e / PR;
+    if (i0 + i >= m) continue;
+    T sum = 0;
+    for (int l = 0; l <= c; ++l) sum += Vs[i][l] * Ts[l][c];
+    vt[(int64_t)(i0 + i) + (int64_t)c * m] = sum;
   }
 }
 
-// X = sum of the split-K partials (nsplit, each m x k); Y = VT^T X (k x k); W = X - V Y / 2 into x. One workgroup.
+// X(rows) = sum of the nsplit split-K partials (each m x k) into x; Yp[blk] = VT(rows)^T X(rows).
 template <typename T>
-__global__ __launch_bounds__(256) void panel_finish_kernel(int m, int k, const T* __restrict__ v,
-                                                           const T* __restrict__ vt, const T* __restrict__ xpart,
-                                                           int nsplit, T* __restrict__ x) {
-  __shared__ T Y[32][65];
-  __shared__ T sa[32][CH + 1];
-  __shared__ T sb[32][CH + 1];
-  const int t = threadIdx.x;
+__global__ __launch_bounds__(256) void panel_xy_kernel(int m, int k, const T* __restrict__ vt,
+                                                       const T* __restrict__ xpart, int nsplit, T* __restrict__ x,
+                                                       T* __restrict__ yp) {
+  __shared__ T Xs[PR][33];
+  __shared__ T Ws[PR][33];
+  const int t = threadIdx.x, i0 = blockIdx.x * PR;
   const int64_t mk = (int64_t)m * k;
-  for (int64_t e = t; e < mk; e += blockDim.x) {
+  for (int e = t; e < PR * 32; e += blockDim.x) {
+    const int i = e % PR, c = e / PR;
     T sum = 0;
-    for (int p = 0; p < nsplit; ++p) sum += xpart[e + p * mk];
-    x[e] = sum;
-  }
-  __syncthreads();
-  block_atb<T>(m, k, vt, x, Y, sa, sb, false);
-  for (int i = t; i < m; i += blockDim.x) {
-    T row[32];
-#pragma unroll
-    for (int l = 0; l < 32; ++l) row[l] = (l < k) ? v[i + (int64_t)l * m] : T(0);
-    for (int c = 0; c < k; ++c) {
-      T sum = 0;
-#pragma unroll
-      for (int l = 0; l < 32; ++l)
-        if (l < k) sum += row[l] * Y[l][c];
-      x[i + (int64_t)c * m] -= T(0.5) * sum;
+    if (i0 + i < m && c < k) {
+      const int64_t o = (int64_t)(i0 + i) + (int64_t)c * m;
+      for (int q = 0; q < nsplit; ++q) sum += xpart[o + q * mk];
+      x[o] = sum;
     }
+    Xs[i][c] = sum;
+  }
+  stage_rows<T>(m, k, i0, vt, Ws);
+  __syncthreads();
+  rows_atb<T>(k, Ws, Xs, yp + (int64_t)blockIdx.x * 1024, false);
+}
+
+// Y = sum of the Yp partials; W(rows) = X(rows) - V(rows) Y / 2, in place of x.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_w_kernel(int m, int k, const T* __restrict__ v, const T* __restrict__ yp,
+                                                      int nblk, T* __restrict__ x) {
+  __shared__ T Y[32][33];
+  __shared__ T Vs[PR][33];
+  const int t = threadIdx.x, i0 = blockIdx.x * PR;
+  sum_partials<T>(k, nblk, yp, Y, false);
+  stage_rows<T>(m, k, i0, v, Vs);
+  __syncthreads();
+  for (int e = t; e < PR * k; e += blockDim.x) {
+    const int i = e % PR, c = e / PR;
+    if (i0 + i >= m) continue;
+    T sum = 0;
+    for (int l = 0; l < k; ++l) sum += Vs[i][l] * Y[l][c];
+    x[(int64_t)(i0 + i) + (int64_t)c * m] -= T(0.5) * sum;
   }
 }
 
@@ -290,9 +318,9 @@ void launch_panel_qr(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv
     case 2: launch_qr<T, SB, 2>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
     case 3: launch_qr<T, SB, 3>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
     case 4: launch_qr<T, SB, 4>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
-    case 5: launch_qr<T, SB, 5>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
-    case 6: launch_qr<T, SB, 6>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
-    // FP64 with seven or eight rows per thread: 8-column sub-panels keep the register array out of scratch
+    // FP64 with five or more rows per thread: 8-column sub-panels keep the register array out of scratch
+    case 5: launch_qr<T, sizeof(T) == 8 ? 8 : SB, 5>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
+    case 6: launch_qr<T, sizeof(T) == 8 ? 8 : SB, 6>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
     case 7: launch_qr<T, sizeof(T) == 8 ? 8 : SB, 7>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
     case 8: launch_qr<T, sizeof(T) == 8 ? 8 : SB, 8>(s, m, nb, p, lda, v1, ldv1, buf, tau); break;
     default: throw std::runtime_error("reduce_to_band: n > 2048 + b is not supported");
@@ -306,8 +334,10 @@ constexpr int SPLIT_ROWS = 128;  // K per split of the X = A2 VT product
 
 template <typename T>
 size_t band_workspace(int n, int b) {
-  // tau (b) | buf [V W V] (3 n b) | VT (n b) | split-K partials of X (ceil(n / SPLIT_ROWS) n b)
-  return (size_t)b + 4 * (size_t)n * b + (size_t)((n + SPLIT_ROWS - 1) / SPLIT_ROWS) * n * b + 64;
+  // tau (b) | buf [V W V] (3 n b) | VT (n b) | split-K partials of X (ceil(n / SPLIT_ROWS) n b) | k x k partials
+  // of the row blocks (ceil(n / PR) 1024)
+  return (size_t)b + 4 * (size_t)n * b + (size_t)((n + SPLIT_ROWS - 1) / SPLIT_ROWS) * n * b +
+         (size_t)((n + PR - 1) / PR) * 1024 + 64;
 }
 
 template <typename T>
@@ -317,6 +347,7 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
   T* buf = tau + b;
   T* vt = buf + 3 * (size_t)n * b;
   T* xpart = vt + (size_t)n * b;
+  T* kpart = xpart + (size_t)((n + SPLIT_ROWS - 1) / SPLIT_ROWS) * n * b;
   // v1 is zero outside the panels' V, so that consecutive panels read as one block [V_a V_b ...] from row r_a
   set_zero<T>(s, n, n, v1, n);
   for (int j = 0; j < n - b - 1; j += b) {
@@ -327,10 +358,16 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
     T* tt = t1 + idx(0, j, b);
     T* a2 = a + idx(r, r, lda);
     T* x = buf + (size_t)m * k;  // slot 1 of [V | W | V]
-    hipLaunchKernelGGL(panel_t_kernel<T>, dim3(1), dim3(256), 0, s, m, k, buf, tau, tt, b, vt);  // T, VT = V T
+    const int nblk = cdiv(m, PR);
+    hipLaunchKernelGGL(panel_gram_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, kpart);      // V^T V parts
     XTB_LAUNCH_CHECK();
-    const int nsplit = gemm_splitk<T>(s, m, k, m, a2, lda, vt, m, xpart, SPLIT_ROWS);            // A2 VT in parts
-    hipLaunchKernelGGL(panel_finish_kernel<T>, dim3(1), dim3(256), 0, s, m, k, buf, vt, xpart, nsplit, x);
+    hipLaunchKernelGGL(panel_tvt_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, tau, kpart, nblk, tt, b,
+                       vt);  // T, VT = V T
+    XTB_LAUNCH_CHECK();
+    const int nsplit = gemm_splitk<T>(s, m, k, m, a2, lda, vt, m, xpart, SPLIT_ROWS);  // A2 VT in parts
+    hipLaunchKernelGGL(panel_xy_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, vt, xpart, nsplit, x, kpart);  // X, Y
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(panel_w_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, kpart, nblk, x);  // W
     XTB_LAUNCH_CHECK();
     gemm<T>(s, 'N', 'T', m, m, 2 * k, T(-1), buf, m, buf + (size_t)m * k, m, T(1), a2, lda);  // A2 -= [V W][W V]^T
   }
