@@ -26,6 +26,10 @@ SEED = 1001  # configuration 1
 MI355X_FP64_TFLOPS = 78.6
 MI355X_FP32_TFLOPS = 157.3
 MI355X_HBM_GBS = 8000.0
+# HBM bytes per chase launch (FP32, n = 1204, b = 16) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this command,
+# each scaled by the same counter's reading for a 256 MiB copy (--pmc-calibration): fetch 81.19 KiB x 2.00, write
+# 3014.69 KiB x 1.00.
+TRAFFIC_BYTES = 3253312
 
 
 def log(*a):
@@ -55,12 +59,13 @@ def timed_solves(torch, hnd, hd, sd, w, c, prec, steps, warmup):
     return t0.elapsed_time(t1) / steps, wall / steps * 1e3
 
 
-def kernel_roofline(torch, xtbeig, hnd, h, s):
-    """The bulge chase (one workgroup, latency/LDS-bound) is the dominant single kernel; time it alone on the FP32
-    band of the workload with HIP events on the handle's stream. Bytes: the window each bulge reads and writes."""
+def kernel_roofline(torch, xtbeig, hnd, h, s, b):
+    """The bulge chase of the FP32+refine solve (one workgroup, band resident in LDS) is its largest kernel. It is
+    timed alone, with HIP events on the handle's stream, on the FP32 band that the library's own stage 1 makes from
+    H' = X^T H X. Algorithmic bytes per launch: the chase streams 2 (2b + 1) n^2 elements (read + write of the band
+    window of every bulge), the count charged to it when nothing stays in LDS or cache."""
     import numpy as np
-    n, b = N, 32
-    # the FP32 band of H' = X^T H X, produced by the library's own stage 1
+    n = N
     hd = torch.from_numpy(np.ascontiguousarray(h.T)).cuda()
     sd = torch.from_numpy(np.ascontiguousarray(s.T)).cuda()
     u = torch.zeros_like(sd)
@@ -82,29 +87,36 @@ def kernel_roofline(torch, xtbeig, hnd, h, s):
     e = torch.zeros(n, dtype=torch.float32, device="cuda")
     stream = torch.cuda.ExternalStream(hnd.stream_ptr())
     reps = 5
-    times = []
+    abw = [ab.clone() for _ in range(reps + 1)]
+    hnd.synchronize()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * (reps + 1))]
     for r in range(reps + 1):
-        abw = ab.clone()
-        hnd.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        hnd.chase(n, b, abw, v2, d, e)
-        e1.record(stream)
-        hnd.synchronize()
-        if r:
-            times.append(e0.elapsed_time(e1))
+        ev[2 * r].record(stream)
+        hnd.chase(n, b, abw[r], v2, d, e)
+        ev[2 * r + 1].record(stream)
+    st = hnd.synchronize()
+    torch.cuda.synchronize()
+    if st != [0, 0, 0]:
+        raise RuntimeError(f"chase status {st}")
+    times = [ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(1, reps + 1)]
     ms = sum(times) / len(times)
-    # algorithmic bytes: each bulge of width b reads and writes its three b x b blocks (lower triangle), FP32
-    bulges = sum(((n - s - 2) // b) + 1 for s in range(n - 2))
-    bytes_per_launch = bulges * (2 * b * b + b * (b + 1) // 2) * 4 * 2
+    bytes_per_launch = 2 * (2 * b + 1) * n * n * 4
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9
-    return {"kernel": "chase_kernel<float> (bulge chase, n=1204, b=32)", "bound": "hbm", "achieved": achieved,
-            "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS, "traffic": None,
-            "ms_per_launch": ms, "bytes_per_launch": bytes_per_launch}
+    return {"kernel": f"chase_kernel<float, 16, LDS> (bulge chase, n={n}, b={b})", "bound": "hbm",
+            "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
+            "traffic": TRAFFIC_BYTES, "ms_per_launch": ms, "bytes_per_launch": bytes_per_launch}
 
 
-def cpu_baseline(h, s, sample=1):
+def pmc_calibration(torch):
+    """A plain 256 MiB device copy (one read, one write of 256 MiB), for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def cpu_baseline(h, s, sample=5):
     """The oracle's FP64 generalized solve on one host core: host dpotrf + dsygst, the C restatement of sytrd.c's
     reduction (LAPACK dsytrd 'U', nb = 32), host dstedc, the restated gemm back-transformation, host dtrsm."""
     import numpy as np
@@ -123,7 +135,7 @@ def cpu_baseline(h, s, sample=1):
         z = sbr.ormtr_upper(a, tau, zt)
         c = hl.dtrsm_left_upper(u, z)
         times.append(time.perf_counter() - t0)
-    return min(times), w
+    return sum(times) / len(times), w
 
 
 def main():
@@ -131,7 +143,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline (for profiler runs)")
+    ap.add_argument("--cpu-sample", type=int, default=5, help="solves in the CPU baseline's sample")
+    ap.add_argument("--pmc-calibration", action="store_true", help="run a 256 MiB copy first (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the eigensolves are "
@@ -153,6 +167,9 @@ def main():
     w = torch.zeros(N, dtype=torch.float64, device="cuda")
     c = torch.zeros((N, N), dtype=torch.float64, device="cuda")
     hnd = xtbeig.Handle(0)
+    if args.pmc_calibration:
+        pmc_calibration(torch)
+    band = xtbeig.default_band(N, fp32=True)
 
     res = {}
     for name, prec in (("fp64", xtbeig.FP64), ("fp32_refine", xtbeig.FP32_REFINE)):
@@ -164,16 +181,17 @@ def main():
         log(name, res[name])
 
     log("kernel roofline ...")
-    roof = kernel_roofline(torch, xtbeig, hnd, h, s)
+    roof = kernel_roofline(torch, xtbeig, hnd, h, s, band)
     log(roof)
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 core) ...")
-        t, wc = cpu_baseline(h, s, sample=1)
+        t, wc = cpu_baseline(h, s, sample=args.cpu_sample)
         cpu = {"value": 1.0 / t, "unit": "gen. eigensolves/s", "cores": 1, "kind": "port",
-               "sample": "1 FP64 generalized solve at n=1204 of the same workload (oracle: host dpotrf/dsygst/"
-                         "dstedc/dtrsm + C restatement of the reduction and back-transform), best of 1",
+               "sample": f"{args.cpu_sample} FP64 generalized solves at n=1204 of the same workload (oracle: host "
+                         "dpotrf/dsygst/dstedc/dtrsm, one thread, + C restatement of the 1-stage reduction and "
+                         "gemm back-transform), rate of the mean",
                "seconds_per_solve": t,
                "eig_err_vs_lapack": float(np.max(np.abs(wc - wref)) / nh)}
         log(cpu)

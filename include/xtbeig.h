@@ -74,6 +74,9 @@ int xtbeig_set_stream(xtbeig_handle_t handle, void *stream);
 void *xtbeig_get_stream(xtbeig_handle_t handle);
 /* Band width of the two-stage reduction (0: the built-in table per n) and the refinement iterations. */
 int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations);
+/* The built-in band width for an n x n solve (fp32 nonzero: the FP32 reduction of the mixed path). No reference
+ * counterpart beyond SBROptions.recommended (SBROptions.swift:73-82), which picks b per n the same way. */
+int xtbeig_default_band(int n, int fp32);
 /* Wait for the handle's stream; status[0..2] = Cholesky info, bulge-chase stall flag, divide-and-conquer info
  * (status may be NULL). Returns 0 when all three are zero. Clears them. */
 int xtbeig_synchronize(xtbeig_handle_t handle, int *status);

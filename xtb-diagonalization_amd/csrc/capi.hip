@@ -195,6 +195,8 @@ int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations) {
   return 0;
 }
 
+int xtbeig_default_band(int n, int fp32) { return xtbeig::default_band(n, fp32 != 0); }
+
 int xtbeig_synchronize(xtbeig_handle_t handle, int* status) {
   int st[3] = {0, 0, 0};
   int rc = guarded([&] { read_status(handle->h, st); });

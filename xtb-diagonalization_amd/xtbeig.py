@@ -44,6 +44,7 @@ SIGNATURES = {
     "xtbeig_set_stream": (ci, [vp, vp]),
     "xtbeig_get_stream": (vp, [vp]),
     "xtbeig_set_tuning": (ci, [vp, ci, ci]),
+    "xtbeig_default_band": (ci, [ci, ci]),
     "xtbeig_synchronize": (ci, [vp, c_int_p]),
     "xtbeig_last_error": (ctypes.c_char_p, []),
     "xtbeig_dev_sygvd": (ci, [vp, ci, ci, vp, ci, vp, ci, vp, vp, ci]),
@@ -275,6 +276,11 @@ def dc_merge(handle, d, q, n1, rho):
     return dict(k=k, ctot=kc[1:].cpu().numpy(), d_deflated=dd.cpu().numpy(), q2=q2.cpu().numpy(),
                 dlamda=dl.cpu().numpy()[:k], w=w.cpu().numpy()[:k], indxc=ix.cpu().numpy(), rho=float(ro[0]),
                 d=d.cpu().numpy(), q=q.cpu().numpy().T, status=st)
+
+
+def default_band(n, fp32=False):
+    """The library's built-in band width for an n x n solve."""
+    return int(lib().xtbeig_default_band(int(n), 1 if fp32 else 0))
 
 
 def band_elems(n, b):
