@@ -37,7 +37,33 @@ void xtb_sstedc(const int *n, float *d, float *e, float *z, const int *ldz, floa
  * (Exports.swift:117-138, xtb_dsymmetrize_pad). */
 void xtb_dsymmetrize_pad(const int *n, const double *src, const int *lds, double *dst, const int *ldd);
 
+/* One-stage reduction to tridiagonal form, LAPACK dsytrd('U') convention (nb = 32, nx = 32): on exit d (n), e (n-1),
+ * tau (n-1), and the Householder vectors above the superdiagonal of a; the strictly lower triangle of a is not
+ * touched. work (n * 32 in the reference) is not used. Replaces Exports.swift:6-16 (xtb_dsytrd_upper, the
+ * reference's tridiagonalize, Tridiagonalization.swift:595-618). */
+void xtb_dsytrd_upper(const int *n, double *a, const int *lda, double *d, double *e, double *tau, double *work);
+/* FP32 twin (Exports.swift:18-27, xtb_ssytrd_upper). */
+void xtb_ssytrd_upper(const int *n, float *a, const int *lda, float *d, float *e, float *tau, float *work);
+/* The same reduction on the full symmetric matrix: both triangles of a are read and both are updated and returned
+ * (Exports.swift:29-38, xtb_dsytrd_full; tridiagonalize with symmetric = false). */
+void xtb_dsytrd_full(const int *n, double *a, const int *lda, double *d, double *e, double *tau, double *work);
+/* z = Q z for the first m columns (dormtr 'L', 'U', 'N'), Q from xtb_dsytrd_upper's a and tau; work (32 (2n + 2m +
+ * 32) in the reference) is not used. Replaces Exports.swift:56-65 (xtb_dormtr_upper, backTransformTridiagonal,
+ * TridiagonalBackTransformation.swift:16-45). */
+void xtb_dormtr_upper(const int *n, const int *m, const double *a, const int *lda, const double *tau, double *z,
+                      const int *ldz, double *work);
+/* FP32 twin (Exports.swift:67-75, xtb_sormtr_upper). */
+void xtb_sormtr_upper(const int *n, const int *m, const float *a, const int *lda, const float *tau, float *z,
+                      const int *ldz, float *work);
+
 /* ---- 1b. LAPACK-convention entry points ------------------------------------------------------------------------ */
+
+/* dsytrd(uplo, ...): uplo = 'U' only (info = -1 otherwise), as xtb_dsytrd_upper; lwork >= 1, lwork = -1 queries
+ * the optimal n * 32 into work[0]. The north_star's d/s sytrd (LAPACK dsytrd's interface). */
+void xtbeig_dsytrd(const char *uplo, const int *n, double *a, const int *lda, double *d, double *e, double *tau,
+                   double *work, const int *lwork, int *info);
+void xtbeig_ssytrd(const char *uplo, const int *n, float *a, const int *lda, float *d, float *e, float *tau,
+                   float *work, const int *lwork, int *info);
 
 /* dsyevd(jobz, uplo, ...): all eigenvalues (w, ascending) and, for jobz = 'V', eigenvectors in a. lwork >=
  * 1 + 6n + 2n^2, liwork >= 3 + 5n ('V'); lwork = -1 or liwork = -1 is a workspace query. The reference's
@@ -58,6 +84,11 @@ void xtbeig_dsygvd(const int *itype, const char *jobz, const char *uplo, const i
 void xtbeig_dsygvd_mixed(const int *itype, const char *jobz, const char *uplo, const int *n, double *a,
                          const int *lda, double *b, const int *ldb, double *w, double *work, const int *lwork,
                          int *iwork, const int *liwork, int *info);
+/* ssygvd: FP32 interface of the generalized solve (LAPACK ssygvd's arguments); computed by the FP32 + FP64-refinement
+ * path and rounded to FP32. */
+void xtbeig_ssygvd(const int *itype, const char *jobz, const char *uplo, const int *n, float *a, const int *lda,
+                   float *b, const int *ldb, float *w, float *work, const int *lwork, int *iwork, const int *liwork,
+                   int *info);
 
 /* ---- 2. Device-memory API ------------------------------------------------------------------------------------- */
 
@@ -123,6 +154,18 @@ int xtbeig_dev_dback_band(xtbeig_handle_t handle, int n, int b, const double *v1
                           int ldz, int m);
 int xtbeig_dev_sback_band(xtbeig_handle_t handle, int n, int b, const float *v1, const float *t1, float *z, int ldz,
                           int m);
+
+/* Device twins of xtb_{d,s}sytrd_upper (full = 0) and xtb_dsytrd_full (full = 1): a (n x n, lda) is reduced in
+ * place and left in full storage (both triangles updated); d, e, tau device arrays. */
+int xtbeig_dev_dsytrd_upper(xtbeig_handle_t handle, int n, double *a, int lda, double *d, double *e, double *tau,
+                            int full);
+int xtbeig_dev_ssytrd_upper(xtbeig_handle_t handle, int n, float *a, int lda, float *d, float *e, float *tau,
+                            int full);
+/* Device twins of xtb_{d,s}ormtr_upper. */
+int xtbeig_dev_dormtr_upper(xtbeig_handle_t handle, int n, int m, const double *a, int lda, const double *tau,
+                            double *z, int ldz);
+int xtbeig_dev_sormtr_upper(xtbeig_handle_t handle, int n, int m, const float *a, int lda, const float *tau,
+                            float *z, int ldz);
 
 /* One divide-and-conquer merge (dlaed1 semantics; reference: Stages/ParallelDivideAndConquer.swift:77-124) of two
  * solved halves: d (n) ascending within rows 0 ..< n1 and n1 ..< n, q (n x n, ldq) block diagonal with their

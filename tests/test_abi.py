@@ -16,7 +16,9 @@ def declared():
 
 def test_header_declares_the_reference_symbol_set():
     names = declared()
-    for s in ["xtb_dstedc", "xtb_sstedc", "xtb_dsymmetrize_pad", "xtbeig_dsyevd", "xtbeig_dsygvd"]:
+    for s in ["xtb_dsytrd_upper", "xtb_ssytrd_upper", "xtb_dsytrd_full", "xtb_dormtr_upper", "xtb_sormtr_upper",
+              "xtb_dstedc", "xtb_sstedc", "xtb_dsymmetrize_pad", "xtbeig_dsytrd", "xtbeig_ssytrd", "xtbeig_dsyevd",
+              "xtbeig_ssyevd", "xtbeig_dsygvd", "xtbeig_ssygvd"]:
         assert s in names
 
 
@@ -48,3 +50,20 @@ def test_workspace_queries_without_gpu():
     xtbeig.lib().xtbeig_dsygvd(xtbeig._i(2), b"V", b"U", xtbeig._i(n), None, xtbeig._i(n), None, xtbeig._i(n), None,
                                xtbeig._ptr(work), xtbeig._i(-1), xtbeig._ptr(iwork), xtbeig._i(-1), ctypes.byref(info))
     assert info.value == -1
+
+
+def test_sytrd_query_and_argument_checks_without_gpu():
+    """xtbeig_dsytrd: lwork = -1 answers n * 32; uplo 'L' and a short lda are rejected (no device touched)."""
+    import numpy as np
+    n = 50
+    q, info = xtbeig.sytrd(np.zeros((n, n)), lwork=-1)
+    assert info == 0 and q == 32 * n
+    import ctypes
+    work = np.zeros(1)
+    info = ctypes.c_int(0)
+    xtbeig.lib().xtbeig_dsytrd(b"L", xtbeig._i(n), None, xtbeig._i(n), None, None, None, xtbeig._ptr(work),
+                               xtbeig._i(-1), ctypes.byref(info))
+    assert info.value == -1
+    xtbeig.lib().xtbeig_dsytrd(b"U", xtbeig._i(n), None, xtbeig._i(n - 1), None, None, None, xtbeig._ptr(work),
+                               xtbeig._i(-1), ctypes.byref(info))
+    assert info.value == -4

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 using xtbeig::Handle;
 
@@ -80,6 +81,19 @@ void d2h_mat(Handle& h, T* dst, int ldd, const T* src, int lds, int m, int n) {
 }
 
 bool is_char(const char* c, char u) { return c && (*c == u || *c == (char)(u + 32)); }
+
+// The referenced triangle only (upper: i <= j; lower: i >= j) of the device n x n matrix (ld n) into host memory,
+// the other triangle of the host matrix left as it is. Synchronizes the handle's stream.
+template <typename T>
+void d2h_triangle(Handle& h, bool upper, T* dst, int ldd, const T* src, int n) {
+  T* stage = static_cast<T*>(h.host_staging(sizeof(T) * (size_t)n * n));
+  XTB_HIP(hipMemcpyAsync(stage, src, sizeof(T) * (size_t)n * n, hipMemcpyDeviceToHost, h.stream));
+  XTB_HIP(hipStreamSynchronize(h.stream));
+  for (int j = 0; j < n; ++j) {
+    const int i0 = upper ? 0 : j, i1 = upper ? j + 1 : n;
+    std::memcpy(dst + (size_t)j * ldd + i0, stage + (size_t)j * n + i0, sizeof(T) * (size_t)(i1 - i0));
+  }
+}
 
 }  // namespace
 
@@ -161,6 +175,143 @@ void syevd_host(const char* jobz, const char* uplo, const int* n_, T* a, const i
   });
   if (rc) *info = n + 1;
   else if (st[1] || st[2]) *info = st[2] ? st[2] : n + 1;
+}
+
+// Generalized solve from host memory. T is the caller's type; the GPU works on FP64 copies (the FP32 interface runs
+// the FP32 + FP64-refinement path and rounds the results).
+template <typename T>
+static void sygvd_host(bool mixed, const int* itype, const char* jobz, const char* uplo, const int* n_, T* a,
+                       const int* lda_, T* b, const int* ldb_, T* w, T* work, const int* lwork, int* iwork,
+                       const int* liwork, int* info) {
+  const int n = *n_, lda = *lda_, ldb = *ldb_;
+  const bool wantz = is_char(jobz, 'V');
+  const long long lmin = n <= 1 ? 1 : (wantz ? 1 + 6LL * n + 2LL * n * n : 2LL * n + 1);
+  const long long limin = (n <= 1 || !wantz) ? 1 : 3 + 5LL * n;
+  *info = 0;
+  if (*itype != 1) *info = -1;  // itype 2 / 3 (A B x, B A x) are not part of xTB's path
+  else if (!wantz && !is_char(jobz, 'N')) *info = -2;
+  else if (!is_char(uplo, 'U') && !is_char(uplo, 'L')) *info = -3;
+  else if (n < 0) *info = -4;
+  else if (lda < (n > 1 ? n : 1)) *info = -6;
+  else if (ldb < (n > 1 ? n : 1)) *info = -8;
+  if (*info != 0) return;
+  if (*lwork == -1 || *liwork == -1) {
+    work[0] = (T)lmin;
+    iwork[0] = (int)limin;
+    return;
+  }
+  if (*lwork < lmin) { *info = -11; return; }
+  if (*liwork < limin) { *info = -13; return; }
+  if (n == 0) return;
+  int st[3] = {0, 0, 0};
+  const int rc = guarded([&] {
+    Handle& h = thread_handle();
+    const size_t nn = (size_t)n * n;
+    DevBuf dh(sizeof(double) * nn), ds(sizeof(double) * nn), dc(sizeof(double) * nn), dw(sizeof(double) * n),
+        du(sizeof(double) * nn), dx(sizeof(double) * nn), ds2(sizeof(double) * nn), dt(sizeof(T) * nn);
+    const char ul = is_char(uplo, 'U') ? 'U' : 'L';
+    auto upload = [&](double* dst, const T* src, int ld) {
+      if (std::is_same<T, double>::value) {
+        h2d_mat<double>(h, dst, n, reinterpret_cast<const double*>(src), ld, n, n);
+      } else {
+        h2d_mat<T>(h, dt.as<T>(), n, src, ld, n, n);
+        xtbeig::copy_matrix<T, double>(h.stream, n, n, dt.as<T>(), n, dst, n);
+      }
+    };
+    auto download = [&](T* dst, int ld, const double* src, int rows, int cols, bool tri, bool upper) {
+      const T* from;
+      if (std::is_same<T, double>::value) {
+        from = reinterpret_cast<const T*>(src);
+      } else {
+        xtbeig::copy_matrix<double, T>(h.stream, rows, cols, src, n, dt.as<T>(), n);
+        from = dt.as<T>();
+      }
+      if (tri) d2h_triangle<T>(h, upper, dst, ld, from, n);
+      else d2h_mat<T>(h, dst, ld, from, n, rows, cols);
+    };
+    upload(dh.as<double>(), a, lda);
+    upload(ds.as<double>(), b, ldb);
+    xtbeig::symmetrize<double>(h.stream, ul, n, dh.as<double>(), n);
+    xtbeig::symmetrize<double>(h.stream, ul, n, ds.as<double>(), n);
+    xtbeig::sygvd_device(h, n, dh.as<double>(), n, ds.as<double>(), n, dw.as<double>(), dc.as<double>(), n, mixed,
+                         h.band_override);
+    download(w, n, dw.as<double>(), n, 1, false, false);
+    if (wantz) download(a, lda, dc.as<double>(), n, n, false, false);
+    read_status(h, st);
+    if (st[0] == 0) {
+      // b on exit: the Cholesky factor in the referenced triangle (U, or L = U^T), the other triangle untouched,
+      // as LAPACK's dsygvd leaves it
+      xtbeig::copy_matrix<double, double>(h.stream, n, n, ds.as<double>(), n, ds2.as<double>(), n);
+      h.reserve((size_t)n * 64 * sizeof(double) + 4096);
+      xtbeig::potrf_inverse(h.stream, n, ds2.as<double>(), n, du.as<double>(), n, dx.as<double>(), n,
+                            static_cast<double*>(h.arena), h.dinfo);
+      if (ul == 'L') xtbeig::transpose_matrix<double, double>(h.stream, n, n, du.as<double>(), n, dx.as<double>(), n);
+      download(b, ldb, ul == 'U' ? du.as<double>() : dx.as<double>(), n, n, true, ul == 'U');
+      int st2[3];
+      read_status(h, st2);
+    }
+  });
+  if (rc) *info = 2 * n + 1;
+  else if (st[0]) *info = n + st[0];
+  else if (st[1] || st[2]) *info = st[2] ? st[2] : n;
+}
+
+// One-stage reduction from host memory (upper triangle referenced; full: both triangles stored and returned).
+template <typename T>
+static void sytrd_host(int n, T* a, int lda, T* d, T* e, T* tau, bool full) {
+  if (n <= 0) return;
+  guarded([&] {
+    Handle& h = thread_handle();
+    const size_t nn = (size_t)n * n;
+    h.reserve(xtbeig::sytrd_workspace<T>(n) * sizeof(T) + 4096);
+    DevBuf da(sizeof(T) * nn), dd(sizeof(T) * n), de(sizeof(T) * n), dt(sizeof(T) * n);
+    h2d_mat<T>(h, da.as<T>(), n, a, lda, n, n);
+    xtbeig::sytrd_upper<T>(h.stream, n, da.as<T>(), n, dd.as<T>(), de.as<T>(), dt.as<T>(),
+                           static_cast<T*>(h.arena), full);
+    if (full) d2h_mat<T>(h, a, lda, da.as<T>(), n, n, n);
+    else d2h_triangle<T>(h, true, a, lda, da.as<T>(), n);
+    d2h(h, d, dd.as<T>(), sizeof(T) * n);
+    if (n > 1) {
+      d2h(h, e, de.as<T>(), sizeof(T) * (n - 1));
+      d2h(h, tau, dt.as<T>(), sizeof(T) * (n - 1));
+    }
+    XTB_HIP(hipStreamSynchronize(h.stream));
+  });
+}
+
+// z = Q z for m columns from host memory, Q from sytrd_host (a, tau).
+template <typename T>
+static void ormtr_host(int n, int m, const T* a, int lda, const T* tau, T* z, int ldz) {
+  if (n <= 1 || m <= 0) return;
+  guarded([&] {
+    Handle& h = thread_handle();
+    h.reserve(xtbeig::ormtr_workspace<T>(n, m) * sizeof(T) + 4096);
+    DevBuf da(sizeof(T) * (size_t)n * n), dt(sizeof(T) * n), dz(sizeof(T) * (size_t)n * m);
+    h2d_mat<T>(h, da.as<T>(), n, a, lda, n, n);
+    h2d(h, dt.as<T>(), tau, sizeof(T) * (n - 1));
+    h2d_mat<T>(h, dz.as<T>(), n, z, ldz, n, m);
+    xtbeig::ormtr_upper<T>(h.stream, n, m, da.as<T>(), n, dt.as<T>(), dz.as<T>(), n, static_cast<T*>(h.arena));
+    d2h_mat<T>(h, z, ldz, dz.as<T>(), n, n, m);
+    XTB_HIP(hipStreamSynchronize(h.stream));
+  });
+}
+
+// LAPACK xsytrd: uplo 'U' only (the convention dormtr 'U' and the reference use); lwork >= 1, optimal n * 32.
+template <typename T>
+static void sytrd_lapack(const char* uplo, const int* n_, T* a, const int* lda_, T* d, T* e, T* tau, T* work,
+                         const int* lwork, int* info) {
+  const int n = *n_, lda = *lda_;
+  *info = 0;
+  if (!is_char(uplo, 'U')) *info = -1;
+  else if (n < 0) *info = -2;
+  else if (lda < (n > 1 ? n : 1)) *info = -4;
+  else if (*lwork < 1 && *lwork != -1) *info = -9;
+  if (*info != 0) return;
+  if (*lwork == -1) {
+    work[0] = (T)((n > 1 ? n : 1) * 32);
+    return;
+  }
+  sytrd_host<T>(n, a, lda, d, e, tau, false);
 }
 
 extern "C" {
@@ -394,80 +545,86 @@ void xtbeig_ssyevd(const char* jobz, const char* uplo, const int* n, float* a, c
   syevd_host<float>(jobz, uplo, n, a, lda, w, work, lwork, iwork, liwork, info);
 }
 
-static void sygvd_host(bool mixed, const int* itype, const char* jobz, const char* uplo, const int* n_, double* a,
-                       const int* lda_, double* b, const int* ldb_, double* w, double* work, const int* lwork,
-                       int* iwork, const int* liwork, int* info) {
-  const int n = *n_, lda = *lda_, ldb = *ldb_;
-  const bool wantz = is_char(jobz, 'V');
-  const long long lmin = n <= 1 ? 1 : (wantz ? 1 + 6LL * n + 2LL * n * n : 2LL * n + 1);
-  const long long limin = (n <= 1 || !wantz) ? 1 : 3 + 5LL * n;
-  *info = 0;
-  if (*itype != 1) *info = -1;  // itype 2 / 3 (A B x, B A x) are not part of xTB's path
-  else if (!wantz && !is_char(jobz, 'N')) *info = -2;
-  else if (!is_char(uplo, 'U') && !is_char(uplo, 'L')) *info = -3;
-  else if (n < 0) *info = -4;
-  else if (lda < (n > 1 ? n : 1)) *info = -6;
-  else if (ldb < (n > 1 ? n : 1)) *info = -8;
-  if (*info != 0) return;
-  if (*lwork == -1 || *liwork == -1) {
-    work[0] = (double)lmin;
-    iwork[0] = (int)limin;
-    return;
-  }
-  if (*lwork < lmin) { *info = -11; return; }
-  if (*liwork < limin) { *info = -13; return; }
-  if (n == 0) return;
-  int st[3] = {0, 0, 0};
-  const int rc = guarded([&] {
-    Handle& h = thread_handle();
-    const size_t nn = (size_t)n * n;
-    DevBuf dh(sizeof(double) * nn), ds(sizeof(double) * nn), dc(sizeof(double) * nn), dw(sizeof(double) * n),
-        du(sizeof(double) * nn), dx(sizeof(double) * nn), ds2(sizeof(double) * nn);
-    const char ul = is_char(uplo, 'U') ? 'U' : 'L';
-    h2d_mat<double>(h, dh.as<double>(), n, a, lda, n, n);
-    h2d_mat<double>(h, ds.as<double>(), n, b, ldb, n, n);
-    xtbeig::symmetrize<double>(h.stream, ul, n, dh.as<double>(), n);
-    xtbeig::symmetrize<double>(h.stream, ul, n, ds.as<double>(), n);
-    xtbeig::sygvd_device(h, n, dh.as<double>(), n, ds.as<double>(), n, dw.as<double>(), dc.as<double>(), n, mixed,
-                         h.band_override);
-    d2h(h, w, dw.as<double>(), sizeof(double) * n);
-    if (wantz) d2h_mat<double>(h, a, lda, dc.as<double>(), n, n, n);
-    read_status(h, st);
-    if (st[0] == 0) {
-      // b on exit: the Cholesky factor in the referenced triangle, as LAPACK's dsygvd leaves it
-      xtbeig::copy_matrix<double, double>(h.stream, n, n, ds.as<double>(), n, ds2.as<double>(), n);
-      XTB_HIP(hipStreamSynchronize(h.stream));
-      h.reserve((size_t)n * 64 * sizeof(double) + 4096);
-      xtbeig::potrf_inverse(h.stream, n, ds2.as<double>(), n, du.as<double>(), n, dx.as<double>(), n,
-                            static_cast<double*>(h.arena), h.dinfo);
-      if (ul == 'U') {
-        d2h_mat<double>(h, b, ldb, du.as<double>(), n, n, n);
-      } else {
-        // L = U^T into the lower triangle
-        xtbeig::copy_matrix<double, double>(h.stream, n, n, du.as<double>(), n, dx.as<double>(), n);
-        XTB_HIP(hipStreamSynchronize(h.stream));
-        xtbeig::symmetrize<double>(h.stream, 'U', n, dx.as<double>(), n);
-        d2h_mat<double>(h, b, ldb, dx.as<double>(), n, n, n);
-      }
-      int st2[3];
-      read_status(h, st2);
-    }
-  });
-  if (rc) *info = 2 * n + 1;
-  else if (st[0]) *info = n + st[0];
-  else if (st[1] || st[2]) *info = st[2] ? st[2] : n;
-}
-
 void xtbeig_dsygvd(const int* itype, const char* jobz, const char* uplo, const int* n, double* a, const int* lda,
                    double* b, const int* ldb, double* w, double* work, const int* lwork, int* iwork,
                    const int* liwork, int* info) {
-  sygvd_host(false, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
+  sygvd_host<double>(false, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
 }
 
 void xtbeig_dsygvd_mixed(const int* itype, const char* jobz, const char* uplo, const int* n, double* a,
                          const int* lda, double* b, const int* ldb, double* w, double* work, const int* lwork,
                          int* iwork, const int* liwork, int* info) {
-  sygvd_host(true, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
+  sygvd_host<double>(true, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
+}
+
+void xtbeig_ssygvd(const int* itype, const char* jobz, const char* uplo, const int* n, float* a, const int* lda,
+                   float* b, const int* ldb, float* w, float* work, const int* lwork, int* iwork, const int* liwork,
+                   int* info) {
+  sygvd_host<float>(true, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
+}
+
+void xtb_dsytrd_upper(const int* n, double* a, const int* lda, double* d, double* e, double* tau, double* work) {
+  (void)work;
+  sytrd_host<double>(*n, a, *lda, d, e, tau, false);
+}
+void xtb_ssytrd_upper(const int* n, float* a, const int* lda, float* d, float* e, float* tau, float* work) {
+  (void)work;
+  sytrd_host<float>(*n, a, *lda, d, e, tau, false);
+}
+void xtb_dsytrd_full(const int* n, double* a, const int* lda, double* d, double* e, double* tau, double* work) {
+  (void)work;
+  sytrd_host<double>(*n, a, *lda, d, e, tau, true);
+}
+void xtb_dormtr_upper(const int* n, const int* m, const double* a, const int* lda, const double* tau, double* z,
+                      const int* ldz, double* work) {
+  (void)work;
+  ormtr_host<double>(*n, *m, a, *lda, tau, z, *ldz);
+}
+void xtb_sormtr_upper(const int* n, const int* m, const float* a, const int* lda, const float* tau, float* z,
+                      const int* ldz, float* work) {
+  (void)work;
+  ormtr_host<float>(*n, *m, a, *lda, tau, z, *ldz);
+}
+void xtbeig_dsytrd(const char* uplo, const int* n, double* a, const int* lda, double* d, double* e, double* tau,
+                   double* work, const int* lwork, int* info) {
+  sytrd_lapack<double>(uplo, n, a, lda, d, e, tau, work, lwork, info);
+}
+void xtbeig_ssytrd(const char* uplo, const int* n, float* a, const int* lda, float* d, float* e, float* tau,
+                   float* work, const int* lwork, int* info) {
+  sytrd_lapack<float>(uplo, n, a, lda, d, e, tau, work, lwork, info);
+}
+
+int xtbeig_dev_dsytrd_upper(xtbeig_handle_t handle, int n, double* a, int lda, double* d, double* e, double* tau,
+                            int full) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve(xtbeig::sytrd_workspace<double>(n) * sizeof(double) + 4096);
+    xtbeig::sytrd_upper<double>(h.stream, n, a, lda, d, e, tau, static_cast<double*>(h.arena), full != 0);
+  });
+}
+int xtbeig_dev_ssytrd_upper(xtbeig_handle_t handle, int n, float* a, int lda, float* d, float* e, float* tau,
+                            int full) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve(xtbeig::sytrd_workspace<float>(n) * sizeof(float) + 4096);
+    xtbeig::sytrd_upper<float>(h.stream, n, a, lda, d, e, tau, static_cast<float*>(h.arena), full != 0);
+  });
+}
+int xtbeig_dev_dormtr_upper(xtbeig_handle_t handle, int n, int m, const double* a, int lda, const double* tau,
+                            double* z, int ldz) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve(xtbeig::ormtr_workspace<double>(n, m) * sizeof(double) + 4096);
+    xtbeig::ormtr_upper<double>(h.stream, n, m, a, lda, tau, z, ldz, static_cast<double*>(h.arena));
+  });
+}
+int xtbeig_dev_sormtr_upper(xtbeig_handle_t handle, int n, int m, const float* a, int lda, const float* tau,
+                            float* z, int ldz) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve(xtbeig::ormtr_workspace<float>(n, m) * sizeof(float) + 4096);
+    xtbeig::ormtr_upper<float>(h.stream, n, m, a, lda, tau, z, ldz, static_cast<float*>(h.arena));
+  });
 }
 
 }  // extern "C"

@@ -122,6 +122,20 @@ void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt,
 template <typename T>
 void back_transform_band(hipStream_t s, const BackPlan& p, const T* v1, const T* t1, T* zt, int ldzt, int m, T* work);
 
+// sytrd.hip --------------------------------------------------------------------------------------------------------
+// LAPACK dsytrd('U') (nb = 32, nx = 32) of the symmetric a (n x n; full_input: both triangles hold the matrix, else
+// the upper one, mirrored first). a is kept in full storage and returned with both triangles updated (the reference's
+// full variant); its upper triangle is dsytrd('U')'s: Householder vectors above the superdiagonal, d, e, tau.
+template <typename T>
+size_t sytrd_workspace(int n);  // elements of T
+template <typename T>
+void sytrd_upper(hipStream_t s, int n, T* a, int lda, T* d, T* e, T* tau, T* work, bool full_input);
+// z = Q z (dormtr 'L', 'U', 'N') for m columns; a, tau from sytrd_upper.
+template <typename T>
+size_t ormtr_workspace(int n, int m);
+template <typename T>
+void ormtr_upper(hipStream_t s, int n, int m, const T* a, int lda, const T* tau, T* z, int ldz, T* work);
+
 // refine.hip -------------------------------------------------------------------------------------------------------
 size_t refine_workspace(int n);  // doubles; iwork n ints
 void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm, int lds, double* c, int ldc,

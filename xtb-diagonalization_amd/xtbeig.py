@@ -39,6 +39,19 @@ SIGNATURES = {
                              c_int_p, vp, c_int_p, c_int_p]),
     "xtbeig_dsygvd_mixed": (None, [c_int_p, ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, c_int_p, vp,
                                    vp, c_int_p, vp, c_int_p, c_int_p]),
+    "xtbeig_ssygvd": (None, [c_int_p, ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, c_int_p, vp, vp,
+                             c_int_p, vp, c_int_p, c_int_p]),
+    "xtb_dsytrd_upper": (None, [c_int_p, vp, c_int_p, vp, vp, vp, vp]),
+    "xtb_ssytrd_upper": (None, [c_int_p, vp, c_int_p, vp, vp, vp, vp]),
+    "xtb_dsytrd_full": (None, [c_int_p, vp, c_int_p, vp, vp, vp, vp]),
+    "xtb_dormtr_upper": (None, [c_int_p, c_int_p, vp, c_int_p, vp, vp, c_int_p, vp]),
+    "xtb_sormtr_upper": (None, [c_int_p, c_int_p, vp, c_int_p, vp, vp, c_int_p, vp]),
+    "xtbeig_dsytrd": (None, [ctypes.c_char_p, c_int_p, vp, c_int_p, vp, vp, vp, vp, c_int_p, c_int_p]),
+    "xtbeig_ssytrd": (None, [ctypes.c_char_p, c_int_p, vp, c_int_p, vp, vp, vp, vp, c_int_p, c_int_p]),
+    "xtbeig_dev_dsytrd_upper": (ci, [vp, ci, vp, ci, vp, vp, vp, ci]),
+    "xtbeig_dev_ssytrd_upper": (ci, [vp, ci, vp, ci, vp, vp, vp, ci]),
+    "xtbeig_dev_dormtr_upper": (ci, [vp, ci, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_dev_sormtr_upper": (ci, [vp, ci, ci, vp, ci, vp, vp, ci]),
     "xtbeig_create": (ci, [ctypes.POINTER(vp), ci]),
     "xtbeig_destroy": (ci, [vp]),
     "xtbeig_set_stream": (ci, [vp, vp]),
@@ -118,6 +131,71 @@ def dsygvd(h, s, mixed=False, uplo="U"):
     f(_i(1), b"V", uplo.encode(), _i(n), _ptr(a), _i(n), _ptr(b), _i(n), _ptr(w), _ptr(work), _i(lw), _ptr(iwork),
       _i(liw), ctypes.byref(info))
     return w, a, b, info.value
+
+
+def ssygvd(h, s, uplo="U"):
+    """FP32 interface of the generalized solve (xtbeig_ssygvd). Returns (w, c, u, info) in float32."""
+    a = np.array(h, dtype=np.float32, order="F", copy=True)
+    b = np.array(s, dtype=np.float32, order="F", copy=True)
+    n = a.shape[0]
+    w = np.zeros(n, dtype=np.float32)
+    lw = 1 + 6 * n + 2 * n * n
+    liw = 3 + 5 * n
+    work = np.zeros(max(lw, 1), dtype=np.float32)
+    iwork = np.zeros(max(liw, 1), dtype=np.int32)
+    info = ctypes.c_int(0)
+    lib().xtbeig_ssygvd(_i(1), b"V", uplo.encode(), _i(n), _ptr(a), _i(n), _ptr(b), _i(n), _ptr(w), _ptr(work),
+                        _i(lw), _ptr(iwork), _i(liw), ctypes.byref(info))
+    return w, a, b, info.value
+
+
+def sytrd_upper(a_in, dtype=np.float64, full=False):
+    """xtb_{d,s}sytrd_upper (full: xtb_dsytrd_full) on a copy of a_in. Returns (a, d, e, tau)."""
+    a = np.array(a_in, dtype=dtype, order="F", copy=True)
+    n = a.shape[0]
+    d = np.zeros(max(n, 1), dtype=dtype)
+    e = np.zeros(max(n, 1), dtype=dtype)
+    tau = np.zeros(max(n, 1), dtype=dtype)
+    work = np.zeros(max(32 * n, 1), dtype=dtype)
+    if full:
+        if dtype != np.float64:
+            raise ValueError("xtb_dsytrd_full is FP64 only")
+        f = lib().xtb_dsytrd_full
+    else:
+        f = lib().xtb_dsytrd_upper if dtype == np.float64 else lib().xtb_ssytrd_upper
+    f(_i(n), _ptr(a), _i(n), _ptr(d), _ptr(e), _ptr(tau), _ptr(work))
+    return a, d[:n], e[: max(n - 1, 0)], tau[: max(n - 1, 0)]
+
+
+def ormtr_upper(a, tau, z_in, dtype=np.float64):
+    """xtb_{d,s}ormtr_upper: z = Q z, Q from sytrd_upper's (a, tau)."""
+    a = np.asfortranarray(a, dtype=dtype)
+    n = a.shape[0]
+    z = np.array(z_in, dtype=dtype, order="F", copy=True)
+    m = z.shape[1]
+    t = np.zeros(max(n, 1), dtype=dtype)
+    t[: max(n - 1, 0)] = tau
+    work = np.zeros(max(32 * (2 * n + 2 * m + 32), 1), dtype=dtype)
+    f = lib().xtb_dormtr_upper if dtype == np.float64 else lib().xtb_sormtr_upper
+    f(_i(n), _i(m), _ptr(a), _i(n), _ptr(t), _ptr(z), _i(n), _ptr(work))
+    return z
+
+
+def sytrd(a_in, uplo="U", dtype=np.float64, lwork=None):
+    """LAPACK-convention xtbeig_{d,s}sytrd. Returns (a, d, e, tau, info); lwork=-1 returns the query in tau[0]."""
+    a = np.array(a_in, dtype=dtype, order="F", copy=True)
+    n = a.shape[0]
+    d = np.zeros(max(n, 1), dtype=dtype)
+    e = np.zeros(max(n, 1), dtype=dtype)
+    tau = np.zeros(max(n, 1), dtype=dtype)
+    lw = 32 * max(n, 1) if lwork is None else lwork
+    work = np.zeros(max(lw, 1), dtype=dtype)
+    info = ctypes.c_int(0)
+    f = lib().xtbeig_dsytrd if dtype == np.float64 else lib().xtbeig_ssytrd
+    f(uplo.encode(), _i(n), _ptr(a), _i(n), _ptr(d), _ptr(e), _ptr(tau), _ptr(work), _i(lw), ctypes.byref(info))
+    if lw == -1:
+        return work[0], info.value
+    return a, d[:n], e[: max(n - 1, 0)], tau[: max(n - 1, 0)], info.value
 
 
 def syevd(a_in, jobz="V", uplo="U", dtype=np.float64):
