@@ -180,6 +180,14 @@ def main():
                      "pass": workloads.passes(g)}
         log(name, res[name])
 
+    # the FP32 solve's error against the oracle before refinement (untimed): the same call with no refinement step
+    hnd.set_tuning(0, 0)
+    hnd.sygvd(hd, sd, w, c, xtbeig.FP32_REFINE)
+    hnd.synchronize()
+    unrefined = workloads.gates(h, s, w.cpu().numpy(), c.cpu().numpy().T, wref)
+    hnd.set_tuning(0, 2)
+    log("fp32 before refinement", unrefined)
+
     log("kernel roofline ...")
     roof = kernel_roofline(torch, xtbeig, hnd, h, s, band)
     log(roof)
@@ -218,6 +226,7 @@ def main():
                  "gates": fp64["gates"]},
         "fp32_refine": {"solves_per_s": mixed["solves_per_s"], "ms_per_step": mixed["ms"], "pass": mixed["pass"],
                         "gates": mixed["gates"]},
+        "fp32_before_refinement": {"gates": unrefined, "pass": workloads.passes(unrefined)},
         "speedup_fp32_refine_vs_fp64": fp64["ms"] / mixed["ms"],
         "nominal_flop_rate_TFLOPs": {"fp64": flops / (fp64["ms"] * 1e-3) / 1e12,
                                      "fp32_refine": flops / (mixed["ms"] * 1e-3) / 1e12},

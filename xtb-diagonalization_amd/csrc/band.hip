@@ -1,5 +1,5 @@
 // Stage 1 of the two-stage reduction: dense symmetric -> lower band of width b (the reference's reduceToBand with the
-// combined sy2sb update, Stages/BandForm.swift:28-142); the back-transformation through its panels is in wy.hip.
+// combined sy2sb update, Stages/BandForm.swift:28-142); the back-transformation through its panels is in backtransform.hip.
 //
 // Per panel j (columns j ..< j + b, rows r = j + b ..< n, m = n - r, k = min(b, m)):
 //   1. panel_qr_kernel: Householder QR of the m x b panel by one workgroup of 256 threads. The panel is processed

@@ -1,6 +1,6 @@
 // Stage 2 of the two-stage reduction: band (width b) -> tridiagonal by Schwarz bulge chasing, one Householder
 // reflector per bulge (the reference's chaseBulgesLower / chaseBulgeLower, Stages/BulgeChasing.swift:123-321), and
-// its reflectors for the back-transformation (wy.hip).
+// its reflectors for the back-transformation (backtransform.hip).
 //
 // Chase: one workgroup; each wavefront owns one sweep at a time (sweep s on wave s mod W) and chases its bulges down
 // the band. Bulge k of sweep s waits until sweep s - 1 has finished bulges 0 .. k + 2, published through LDS
