@@ -167,6 +167,30 @@ int xtbeig_dev_dormtr_upper(xtbeig_handle_t handle, int n, int m, const double *
 int xtbeig_dev_sormtr_upper(xtbeig_handle_t handle, int n, int m, const float *a, int lda, const float *tau,
                             float *z, int ldz);
 
+/* Fermi-smeared density matrix of batches of equal-n generalized eigensystems (device memory; problem p at base +
+ * p * stride). f_i = 1 / (1 + exp((w_i - mu) / kT)); bisect != 0: mu[p] is bisected so that sum_i f_i = nel, else
+ * mu[p] is used as given. Outputs P = C diag(f) C^T (n x n, ldp) and occ (3 n per problem: f, df/deps,
+ * log cosh((w - mu) / 2kT)), which the backward takes. The reference forms P this way from the eigenpairs
+ * (scripts/kernels/refine.f90:85-132); no reference backward exists. */
+int xtbeig_dev_density(xtbeig_handle_t handle, int batch, int n, const double *w, long long sw, const double *c,
+                       int ldc, long long sc, double kT, double nel, double *mu, int bisect, double *p, int ldp,
+                       long long sp, double *occ);
+/* Backward of the density matrix: given gbar = dL/dP (symmetric), hbar = dL/dH = C (F o G) C^T and sbar = dL/dS =
+ * -C (L o G) C^T with G = C^T gbar C, F and L the Loewner matrices of f and eps f (their derivatives on the
+ * diagonal, formed stably for any gap), plus the chemical-potential term unless mu_fixed. */
+int xtbeig_dev_density_backward(xtbeig_handle_t handle, int batch, int n, const double *w, long long sw,
+                                const double *c, int ldc, long long sc, double kT, const double *mu, int mu_fixed,
+                                const double *occ, const double *gbar, int ldg, long long sg, double *hbar, int ldhb,
+                                long long shb, double *sbar, int ldsb, long long ssb);
+/* The same backward without stored eigenvectors: each problem's eigenpairs are recomputed from (h, s) with the
+ * given precision (XTBEIG_FP64 or XTBEIG_FP32_REFINE) and mu re-bisected to nel, one problem at a time in O(n^2)
+ * device memory (for batches whose eigenvectors do not fit). */
+int xtbeig_dev_density_backward_recompute(xtbeig_handle_t handle, int batch, int n, const double *h, int ldh,
+                                          long long sh, const double *s, int lds, long long ss, int precision,
+                                          double kT, double nel, int mu_fixed, const double *gbar, int ldg,
+                                          long long sg, double *hbar, int ldhb, long long shb, double *sbar, int ldsb,
+                                          long long ssb);
+
 /* One divide-and-conquer merge (dlaed1 semantics; reference: Stages/ParallelDivideAndConquer.swift:77-124) of two
  * solved halves: d (n) ascending within rows 0 ..< n1 and n1 ..< n, q (n x n, ldq) block diagonal with their
  * eigenvectors, rho the coupling. On return d and q hold the merged eigensystem (ascending). dlaed2's outputs are

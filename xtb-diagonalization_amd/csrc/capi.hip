@@ -627,4 +627,59 @@ int xtbeig_dev_sormtr_upper(xtbeig_handle_t handle, int n, int m, const float* a
   });
 }
 
+int xtbeig_dev_density(xtbeig_handle_t handle, int batch, int n, const double* w, long long sw, const double* c,
+                       int ldc, long long sc, double kT, double nel, double* mu, int bisect, double* p, int ldp,
+                       long long sp, double* occ) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve_aux(xtbeig::density_workspace(batch, n) * sizeof(double));
+    xtbeig::density_forward(h.stream, batch, n, w, sw, c, ldc, sc, kT, nel, bisect, mu, p, ldp, sp, occ,
+                            static_cast<double*>(h.aux));
+  });
+}
+
+int xtbeig_dev_density_backward(xtbeig_handle_t handle, int batch, int n, const double* w, long long sw,
+                                const double* c, int ldc, long long sc, double kT, const double* mu, int mu_fixed,
+                                const double* occ, const double* gbar, int ldg, long long sg, double* hbar, int ldhb,
+                                long long shb, double* sbar, int ldsb, long long ssb) {
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve_aux(xtbeig::density_workspace(batch, n) * sizeof(double));
+    xtbeig::density_backward(h.stream, batch, n, w, sw, c, ldc, sc, kT, mu, mu_fixed, occ, gbar, ldg, sg, hbar, ldhb,
+                             shb, sbar, ldsb, ssb, static_cast<double*>(h.aux));
+  });
+}
+
+int xtbeig_dev_density_backward_recompute(xtbeig_handle_t handle, int batch, int n, const double* hm, int ldh,
+                                          long long sh, const double* sm, int lds, long long ss, int precision,
+                                          double kT, double nel, int mu_fixed, const double* gbar, int ldg,
+                                          long long sg, double* hbar, int ldhb, long long shb, double* sbar, int ldsb,
+                                          long long ssb) {
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_dev_density_backward_recompute: unknown precision");
+  return guarded([&] {
+    Handle& h = handle->h;
+    // one problem at a time: eigenpairs, occupations (mu bisected to nel) and the backward in O(n^2) memory
+    const size_t nn = (size_t)n * n;
+    const size_t per = nn + 4 * (size_t)n + 8;
+    h.reserve_aux((xtbeig::density_workspace(1, n) + per + 64) * sizeof(double));
+    double* work = static_cast<double*>(h.aux);
+    double* cbuf = work + xtbeig::density_workspace(1, n);
+    double* wbuf = cbuf + nn;
+    double* occ = wbuf + n;
+    double* mu = occ + 3 * (size_t)n;
+    for (int q = 0; q < batch; ++q) {
+      xtbeig::sygvd_device(h, n, hm + (int64_t)q * sh, ldh, sm + (int64_t)q * ss, lds, wbuf, cbuf, n,
+                           precision == XTBEIG_FP32_REFINE, h.band_override);
+      // the occupations need mu: a forward pass without P (p = nullptr is not allowed; P goes to the backward's
+      // first scratch matrix and is overwritten there)
+      xtbeig::density_forward(h.stream, 1, n, wbuf, n, cbuf, n, (int64_t)nn, kT, nel, 1, mu, work + nn, n,
+                              (int64_t)nn, occ, work);
+      xtbeig::density_backward(h.stream, 1, n, wbuf, n, cbuf, n, (int64_t)nn, kT, mu, mu_fixed, occ,
+                               gbar + (int64_t)q * sg, ldg, sg, hbar + (int64_t)q * shb, ldhb, shb,
+                               sbar + (int64_t)q * ssb, ldsb, ssb, work);
+    }
+  });
+}
+
 }  // extern "C"

@@ -136,6 +136,19 @@ size_t ormtr_workspace(int n, int m);
 template <typename T>
 void ormtr_upper(hipStream_t s, int n, int m, const T* a, int lda, const T* tau, T* z, int ldz, T* work);
 
+// density.hip ------------------------------------------------------------------------------------------------------
+// Batches of equal-n problems, problem p at base + p * stride. occ: 3 n per problem (f | f' | log cosh(x / 2)).
+size_t density_workspace(int batch, int n);  // doubles
+// mu (bisect: sum f = nel; else the given mu), occ, P = C diag(f) C^T.
+void density_forward(hipStream_t s, int batch, int n, const double* w, int64_t sw, const double* c, int ldc,
+                     int64_t sc, double kT, double nel, int bisect, double* mu, double* p, int ldp, int64_t sp,
+                     double* occ, double* work);
+// Hbar, Sbar from Gbar = dL/dP (mu_fixed: no re-bisection term).
+void density_backward(hipStream_t s, int batch, int n, const double* w, int64_t sw, const double* c, int ldc,
+                      int64_t sc, double kT, const double* mu, int mu_fixed, const double* occ, const double* gbar,
+                      int ldg, int64_t sg, double* hbar, int ldhb, int64_t shb, double* sbar, int ldsb, int64_t ssb,
+                      double* work);
+
 // refine.hip -------------------------------------------------------------------------------------------------------
 size_t refine_workspace(int n);  // doubles; iwork n ints
 void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm, int lds, double* c, int ldc,

@@ -39,6 +39,7 @@ Handle::Handle(int dev) : device(dev) {
 Handle::~Handle() {
   if (stream) (void)hipStreamSynchronize(stream);
   if (arena) (void)hipFree(arena);
+  if (aux) (void)hipFree(aux);
   if (dinfo) (void)hipFree(dinfo);
   if (pinned) (void)hipHostFree(pinned);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -55,6 +56,19 @@ void Handle::reserve(size_t bytes) {
   }
   XTB_HIP(hipMalloc(&arena, bytes));
   arena_bytes = bytes;
+}
+
+void Handle::reserve_aux(size_t bytes) {
+  if (bytes <= aux_bytes) return;
+  XTB_HIP(hipSetDevice(device));
+  if (aux) {
+    XTB_HIP(hipStreamSynchronize(stream));
+    XTB_HIP(hipFree(aux));
+    aux = nullptr;
+    aux_bytes = 0;
+  }
+  XTB_HIP(hipMalloc(&aux, bytes));
+  aux_bytes = bytes;
 }
 
 void* Handle::host_staging(size_t bytes) {

@@ -15,6 +15,9 @@ struct Handle {
   bool own_stream = false;
   void* arena = nullptr;
   size_t arena_bytes = 0;
+  // second device buffer, for work that wraps a solve (the solve itself carves the arena)
+  void* aux = nullptr;
+  size_t aux_bytes = 0;
   // device status words: [0] potrf info, [1] chase stall, [2] stedc info
   int* dinfo = nullptr;
   void* pinned = nullptr;
@@ -29,6 +32,7 @@ struct Handle {
   Handle(const Handle&) = delete;
   Handle& operator=(const Handle&) = delete;
   void reserve(size_t bytes);
+  void reserve_aux(size_t bytes);
   void* host_staging(size_t bytes);
   const DcPlan& plan(int n);
   const BackPlan& backplan(int n, int b);

@@ -88,6 +88,25 @@ def generalized_pair(n, seed):
     return h, s
 
 
+def density_problem(n, seed):
+    """(H, S, lambda) of the density-matrix configuration: the generalized eigenvalues are the spectrum above plus 8
+    near-degenerate pairs with gaps log-uniform in [1e-10, 1e-4]. H = U^T (Q diag(lambda) Q^T) U with S = U^T U, so
+    (H, S) has exactly the spectrum lambda."""
+    rng = np.random.default_rng(seed)
+    lam = spectrum(n, rng)
+    if n >= 40:
+        pick = rng.permutation(n)[:16]
+        gaps = 10.0 ** rng.uniform(-10.0, -4.0, 8)
+        for p in range(8):
+            lam[pick[2 * p + 1]] = lam[pick[2 * p]] + gaps[p]
+    q = householder_product(n, rng)
+    s = overlap_matrix(n, rng)
+    u = np.linalg.cholesky(s).T
+    hp = (q * lam) @ q.T
+    h = u.T @ (0.5 * (hp + hp.T)) @ u
+    return np.asfortranarray(0.5 * (h + h.T)), s, np.sort(lam)
+
+
 def gates(h, s, w, c, w_ref):
     """The three contract quantities, normalized so that each passes at <= 1."""
     n = h.shape[0]

@@ -25,6 +25,7 @@ c_dbl_p = ctypes.POINTER(ctypes.c_double)
 c_flt_p = ctypes.POINTER(ctypes.c_float)
 vp = ctypes.c_void_p
 ci = ctypes.c_int
+ll = ctypes.c_longlong
 
 # symbol -> (restype, argtypes)
 SIGNATURES = {
@@ -52,6 +53,12 @@ SIGNATURES = {
     "xtbeig_dev_ssytrd_upper": (ci, [vp, ci, vp, ci, vp, vp, vp, ci]),
     "xtbeig_dev_dormtr_upper": (ci, [vp, ci, ci, vp, ci, vp, vp, ci]),
     "xtbeig_dev_sormtr_upper": (ci, [vp, ci, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_dev_density": (ci, [vp, ci, ci, vp, ll, vp, ci, ll, ctypes.c_double, ctypes.c_double, vp, ci, vp, ci, ll,
+                                vp]),
+    "xtbeig_dev_density_backward": (ci, [vp, ci, ci, vp, ll, vp, ci, ll, ctypes.c_double, vp, ci, vp, vp, ci, ll, vp,
+                                         ci, ll, vp, ci, ll]),
+    "xtbeig_dev_density_backward_recompute": (ci, [vp, ci, ci, vp, ci, ll, vp, ci, ll, ci, ctypes.c_double,
+                                                   ctypes.c_double, ci, vp, ci, ll, vp, ci, ll, vp, ci, ll]),
     "xtbeig_create": (ci, [ctypes.POINTER(vp), ci]),
     "xtbeig_destroy": (ci, [vp]),
     "xtbeig_set_stream": (ci, [vp, vp]),
@@ -331,6 +338,44 @@ class Handle:
         import torch
         f = lib().xtbeig_dev_dback_band if z.dtype == torch.float64 else lib().xtbeig_dev_sback_band
         self._check(f(self._h, n, b, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(z.data_ptr()), n, m))
+
+    # ---- density matrix and its backward (batches of equal-n problems; tensors (batch, n, n) hold C^T etc.) ----
+    # Matrices are passed as torch tensors whose last two dimensions are a column-major n x n matrix stored
+    # row-major-transposed (tensor[p].T is the matrix), i.e. tensor[p] is the matrix's transpose in C order, the
+    # layout the rest of this module uses; w is (batch, n), mu (batch,), occ (batch, 3n).
+    def density(self, w, c, kT, nel, mu, bisect=True, p=None, occ=None):
+        import torch
+        batch, n = w.shape
+        if p is None:
+            p = torch.empty((batch, n, n), dtype=torch.float64, device=w.device)
+        if occ is None:
+            occ = torch.empty((batch, 3 * n), dtype=torch.float64, device=w.device)
+        self._check(lib().xtbeig_dev_density(self._h, batch, n, vp(w.data_ptr()), n, vp(c.data_ptr()), n, n * n,
+                                             float(kT), float(nel), vp(mu.data_ptr()), 1 if bisect else 0,
+                                             vp(p.data_ptr()), n, n * n, vp(occ.data_ptr())))
+        return p, occ
+
+    def density_backward(self, w, c, kT, mu, occ, gbar, mu_fixed=False):
+        import torch
+        batch, n = w.shape
+        hbar = torch.empty((batch, n, n), dtype=torch.float64, device=w.device)
+        sbar = torch.empty_like(hbar)
+        self._check(lib().xtbeig_dev_density_backward(self._h, batch, n, vp(w.data_ptr()), n, vp(c.data_ptr()), n,
+                                                      n * n, float(kT), vp(mu.data_ptr()), 1 if mu_fixed else 0,
+                                                      vp(occ.data_ptr()), vp(gbar.data_ptr()), n, n * n,
+                                                      vp(hbar.data_ptr()), n, n * n, vp(sbar.data_ptr()), n, n * n))
+        return hbar, sbar
+
+    def density_backward_recompute(self, h, s, kT, nel, gbar, precision=0, mu_fixed=False):
+        import torch
+        batch, n, _ = h.shape
+        hbar = torch.empty((batch, n, n), dtype=torch.float64, device=h.device)
+        sbar = torch.empty_like(hbar)
+        self._check(lib().xtbeig_dev_density_backward_recompute(
+            self._h, batch, n, vp(h.data_ptr()), n, n * n, vp(s.data_ptr()), n, n * n, precision, float(kT),
+            float(nel), 1 if mu_fixed else 0, vp(gbar.data_ptr()), n, n * n, vp(hbar.data_ptr()), n, n * n,
+            vp(sbar.data_ptr()), n, n * n))
+        return hbar, sbar
 
 
 def dc_merge(handle, d, q, n1, rho):
