@@ -167,6 +167,27 @@ int xtbeig_dev_dormtr_upper(xtbeig_handle_t handle, int n, int m, const double *
 int xtbeig_dev_sormtr_upper(xtbeig_handle_t handle, int n, int m, const float *a, int lda, const float *tau,
                             float *z, int ldz);
 
+/* SCC precision schedule, per handle (mirrors xTB's xtb_setSCCPrecision / xtb_getSCCPrecision contract, reference
+ * LOG.md:356-370 and scripts/test_sccprecision.py:79-161): fp32_rmsq = 0 runs every cycle in FP64 (the default);
+ * fp32_rmsq > 0 runs a cycle in FP32 while the caller's rms charge change is >= fp32_rmsq, for at most fp32_maxiter
+ * FP32 cycles, and in FP64 from the first cycle that is not, to the end of the sequence. fp32_refine = 1 refines
+ * the FP32 eigenpairs in FP64 (Ogita-Aishima), 0 uses them as they are. Invalid values (negative or non-finite
+ * fp32_rmsq, fp32_maxiter < 1, fp32_refine not 0 or 1, fp32_refine without FP32) return nonzero with the message in
+ * xtbeig_last_error and leave the settings unchanged. Handles are independent: two threads with their own handles
+ * never affect each other. */
+int xtbeig_set_scc_precision(xtbeig_handle_t handle, double fp32_rmsq, int fp32_maxiter, int fp32_refine);
+/* Start an SCC sequence: counters zero, FP32 allowed again. */
+int xtbeig_scc_begin(xtbeig_handle_t handle);
+/* One cycle's generalized eigensolve under the schedule (device memory, as xtbeig_dev_sygvd), rmsq the cycle's rms
+ * charge change; waits for the result. An FP32 solve that fails (chase stall, divide-and-conquer info) is redone in
+ * FP64 and the sequence stays FP64 (fp64_restart = 1). *used (may be NULL): the precision the cycle ran in. */
+int xtbeig_dev_scc_sygvd(xtbeig_handle_t handle, int n, const double *h, int ldh, const double *s, int lds, double *w,
+                         double *c, int ldc, double rmsq, int *used);
+/* The settings in effect and the sequence's counters (any pointer may be NULL). Nonzero when no cycle has run since
+ * xtbeig_scc_begin (the counters are still written). */
+int xtbeig_get_scc_precision(xtbeig_handle_t handle, double *fp32_rmsq, int *fp32_maxiter, int *fp32_refine,
+                             int *fp32_cycles, int *fp64_restart, int *scc_cycles);
+
 /* Fermi-smeared density matrix of batches of equal-n generalized eigensystems (device memory; problem p at base +
  * p * stride). f_i = 1 / (1 + exp((w_i - mu) / kT)); bisect != 0: mu[p] is bisected so that sum_i f_i = nel, else
  * mu[p] is used as given. Outputs P = C diag(f) C^T (n x n, ldp) and occ (3 n per problem: f, df/deps,

@@ -4,6 +4,7 @@
 #include "../../include/xtbeig.h"
 #include "solver.hpp"
 
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -680,6 +681,88 @@ int xtbeig_dev_density_backward_recompute(xtbeig_handle_t handle, int batch, int
                                sbar + (int64_t)q * ssb, ldsb, ssb, work);
     }
   });
+}
+
+int xtbeig_set_scc_precision(xtbeig_handle_t handle, double fp32_rmsq, int fp32_maxiter, int fp32_refine) {
+  if (!handle) return set_error("xtbeig_set_scc_precision: null handle");
+  if (!std::isfinite(fp32_rmsq) || fp32_rmsq < 0)
+    return set_error("xtbeig_set_scc_precision: fp32_rmsq must be finite and >= 0 (0: FP64 only)");
+  if (fp32_maxiter < 1) return set_error("xtbeig_set_scc_precision: fp32_maxiter must be >= 1");
+  if (fp32_refine != 0 && fp32_refine != 1) return set_error("xtbeig_set_scc_precision: fp32_refine must be 0 or 1");
+  if (fp32_refine == 1 && fp32_rmsq == 0)
+    return set_error("xtbeig_set_scc_precision: fp32_refine without FP32 (fp32_rmsq = 0)");
+  Handle& h = handle->h;
+  h.scc_fp32_rmsq = fp32_rmsq;
+  h.scc_fp32_maxiter = fp32_rmsq > 0 ? fp32_maxiter : 0;
+  h.scc_fp32_refine = fp32_refine;
+  return 0;
+}
+
+int xtbeig_scc_begin(xtbeig_handle_t handle) {
+  if (!handle) return set_error("xtbeig_scc_begin: null handle");
+  Handle& h = handle->h;
+  h.scc_fp32_cycles = h.scc_fp64_restart = h.scc_cycles = 0;
+  h.scc_fp64_latched = false;
+  return 0;
+}
+
+int xtbeig_dev_scc_sygvd(xtbeig_handle_t handle, int n, const double* hm, int ldh, const double* sm, int lds,
+                         double* w, double* c, int ldc, double rmsq, int* used) {
+  if (!handle) return set_error("xtbeig_dev_scc_sygvd: null handle");
+  if (n < 0 || ldh < (n > 1 ? n : 1) || lds < (n > 1 ? n : 1) || ldc < (n > 1 ? n : 1))
+    return set_error("xtbeig_dev_scc_sygvd: bad dimensions");
+  Handle& h = handle->h;
+  bool fp32 = h.scc_fp32_rmsq > 0 && !h.scc_fp64_latched && rmsq >= h.scc_fp32_rmsq &&
+              h.scc_fp32_cycles < h.scc_fp32_maxiter;
+  if (h.scc_fp32_rmsq > 0 && !fp32) h.scc_fp64_latched = true;  // FP64 for the rest of the sequence
+  int st[3] = {0, 0, 0};
+  int rc = guarded([&] {
+    if (fp32) {
+      const int keep = h.refine_iters;
+      h.refine_iters = h.scc_fp32_refine ? (keep > 0 ? keep : 2) : 0;
+      try {
+        xtbeig::sygvd_device(h, n, hm, ldh, sm, lds, w, c, ldc, true, h.band_override);
+      } catch (...) {
+        h.refine_iters = keep;
+        throw;
+      }
+      h.refine_iters = keep;
+      read_status(h, st);
+      if (st[1] || st[2]) {
+        // the FP32 solve failed (chase stall or D&C): redo the cycle in FP64, FP64 from here on
+        h.scc_fp64_restart = 1;
+        h.scc_fp64_latched = true;
+        fp32 = false;
+      } else if (st[0]) {
+        throw std::runtime_error("xtbeig_dev_scc_sygvd: S is not positive definite");
+      }
+    }
+    if (!fp32) {
+      xtbeig::sygvd_device(h, n, hm, ldh, sm, lds, w, c, ldc, false, h.band_override);
+      read_status(h, st);
+      if (st[0]) throw std::runtime_error("xtbeig_dev_scc_sygvd: S is not positive definite");
+      if (st[1] || st[2]) throw std::runtime_error("xtbeig_dev_scc_sygvd: FP64 eigensolve failed");
+    }
+  });
+  if (rc) return rc;
+  h.scc_cycles += 1;
+  if (fp32) h.scc_fp32_cycles += 1;
+  if (used) *used = fp32 ? XTBEIG_FP32_REFINE : XTBEIG_FP64;
+  return 0;
+}
+
+int xtbeig_get_scc_precision(xtbeig_handle_t handle, double* fp32_rmsq, int* fp32_maxiter, int* fp32_refine,
+                             int* fp32_cycles, int* fp64_restart, int* scc_cycles) {
+  if (!handle) return set_error("xtbeig_get_scc_precision: null handle");
+  Handle& h = handle->h;
+  if (fp32_rmsq) *fp32_rmsq = h.scc_fp32_rmsq;
+  if (fp32_maxiter) *fp32_maxiter = h.scc_fp32_maxiter;
+  if (fp32_refine) *fp32_refine = h.scc_fp32_refine;
+  if (fp32_cycles) *fp32_cycles = h.scc_fp32_cycles;
+  if (fp64_restart) *fp64_restart = h.scc_fp64_restart;
+  if (scc_cycles) *scc_cycles = h.scc_cycles;
+  if (h.scc_cycles == 0) return set_error("xtbeig_get_scc_precision: no SCC cycle has run since xtbeig_scc_begin");
+  return 0;
 }
 
 }  // extern "C"

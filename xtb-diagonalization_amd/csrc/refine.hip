@@ -85,6 +85,8 @@ size_t refine_workspace(int n) { return 6 * (size_t)n * n + 2 * (size_t)n + 16; 
 // One or more Ogita-Aishima steps on C (n x n, ldc), then eigenvalues w (ascending) and C's columns sorted to match.
 void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm, int lds, double* c, int ldc,
                double* w, int iterations, double* work, int* iwork) {
+  // no step: the eigenpairs stay as the FP32 solve left them (already in ascending order)
+  if (iterations <= 0) return;
   const size_t nn = (size_t)n * n;
   double* sc = work;
   double* hc = sc + nn;

@@ -24,6 +24,11 @@ struct Handle {
   size_t pinned_bytes = 0;
   int refine_iters = 2;
   int band_override = 0;
+  // SCC precision schedule: settings (fp32_rmsq 0 = FP64 only) and the counters of the current sequence
+  double scc_fp32_rmsq = 0.0;
+  int scc_fp32_maxiter = 0, scc_fp32_refine = 0;
+  int scc_fp32_cycles = 0, scc_fp64_restart = 0, scc_cycles = 0;
+  bool scc_fp64_latched = false;
   std::map<int, std::unique_ptr<DcPlan>> plans;
   std::map<std::pair<int, int>, std::unique_ptr<BackPlan>> backplans;
 

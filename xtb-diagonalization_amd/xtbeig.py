@@ -59,6 +59,11 @@ SIGNATURES = {
                                          ci, ll, vp, ci, ll]),
     "xtbeig_dev_density_backward_recompute": (ci, [vp, ci, ci, vp, ci, ll, vp, ci, ll, ci, ctypes.c_double,
                                                    ctypes.c_double, ci, vp, ci, ll, vp, ci, ll, vp, ci, ll]),
+    "xtbeig_set_scc_precision": (ci, [vp, ctypes.c_double, ci, ci]),
+    "xtbeig_scc_begin": (ci, [vp]),
+    "xtbeig_dev_scc_sygvd": (ci, [vp, ci, vp, ci, vp, ci, vp, vp, ci, ctypes.c_double, c_int_p]),
+    "xtbeig_get_scc_precision": (ci, [vp, ctypes.POINTER(ctypes.c_double), c_int_p, c_int_p, c_int_p, c_int_p,
+                                      c_int_p]),
     "xtbeig_create": (ci, [ctypes.POINTER(vp), ci]),
     "xtbeig_destroy": (ci, [vp]),
     "xtbeig_set_stream": (ci, [vp, vp]),
@@ -338,6 +343,34 @@ class Handle:
         import torch
         f = lib().xtbeig_dev_dback_band if z.dtype == torch.float64 else lib().xtbeig_dev_sback_band
         self._check(f(self._h, n, b, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(z.data_ptr()), n, m))
+
+    # ---- SCC precision schedule ----
+    def set_scc_precision(self, fp32_rmsq, fp32_maxiter=160, fp32_refine=0):
+        """Returns 0, or raises ValueError with the library's message (settings unchanged)."""
+        rc = lib().xtbeig_set_scc_precision(self._h, float(fp32_rmsq), int(fp32_maxiter), int(fp32_refine))
+        if rc:
+            raise ValueError(last_error())
+
+    def scc_begin(self):
+        self._check(lib().xtbeig_scc_begin(self._h))
+
+    def scc_sygvd(self, h, s, w, c, rmsq):
+        """One SCC cycle's solve; returns the precision used (FP64 or FP32_REFINE)."""
+        used = ctypes.c_int(-1)
+        n = h.shape[0]
+        self._check(lib().xtbeig_dev_scc_sygvd(self._h, n, vp(h.data_ptr()), h.stride(0), vp(s.data_ptr()),
+                                               s.stride(0), vp(w.data_ptr()), vp(c.data_ptr()), c.stride(0),
+                                               float(rmsq), ctypes.byref(used)))
+        return used.value
+
+    def get_scc_precision(self):
+        """(settings, counters, rc): settings (fp32_rmsq, fp32_maxiter, fp32_refine), counters dict, rc != 0 when
+        no cycle has run."""
+        r = ctypes.c_double()
+        i = [ctypes.c_int() for _ in range(5)]
+        rc = lib().xtbeig_get_scc_precision(self._h, ctypes.byref(r), *[ctypes.byref(v) for v in i])
+        return ((r.value, i[0].value, i[1].value),
+                dict(fp32_cycles=i[2].value, fp64_restart=i[3].value, scc_cycles=i[4].value), rc)
 
     # ---- density matrix and its backward (batches of equal-n problems; tensors (batch, n, n) hold C^T etc.) ----
     # Matrices are passed as torch tensors whose last two dimensions are a column-major n x n matrix stored
