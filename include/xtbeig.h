@@ -167,6 +167,34 @@ int xtbeig_dev_dormtr_upper(xtbeig_handle_t handle, int n, int m, const double *
 int xtbeig_dev_sormtr_upper(xtbeig_handle_t handle, int n, int m, const float *a, int lda, const float *tau,
                             float *z, int ldz);
 
+/* The generalized solve replayed from a HIP graph captured for (n, precision, band) on this handle (captured on the
+ * first call after one uncaptured solve; recaptured when a larger solve has since regrown the handle's workspace).
+ * Same arguments and results as xtbeig_dev_sygvd; H and S are copied into the graph's buffers and w, C out of them.
+ * For the repeated same-n solves of an SCC sequence. No reference counterpart (the reference solves on the CPU). */
+int xtbeig_dev_sygvd_graph(xtbeig_handle_t handle, int precision, int n, const double *h, int ldh, const double *s,
+                           int lds, double *w, double *c, int ldc);
+/* count independent generalized solves of uneven n (arrays of count device pointers and sizes, in host memory) on up
+ * to `slots` concurrent streams of sub-handles, longest first onto the least-loaded slot; use_graph: each solve
+ * through the slot's graph cache. Waits for completion; latency_ms (count, may be NULL) receives each solve's time
+ * on its stream. Status words of the slots are merged into the handle's (xtbeig_synchronize). The batched
+ * configuration of the north_star; no reference counterpart. */
+int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, const int *n, const double *const *h,
+                             const int *ldh, const double *const *s, const int *lds, double *const *w,
+                             double *const *c, const int *ldc, int slots, int use_graph, float *latency_ms);
+/* Band-width autotuner: for every n bucket of the handle's table that meets [nmin, nmax], times the candidate band
+ * widths (FP64: 8, 16, 32; FP32 + refinement: 16, 32) on a synthetic pair of the bucket's middle size and keeps the
+ * fastest in the table, which later solves on the handle use (an explicit xtbeig_set_tuning band overrides it).
+ * Outputs per tuned bucket (arrays of xtbeig_band_buckets() entries, may be NULL): its first n, the band, the time
+ * in ms; *count the number of buckets tuned. Reference counterpart: SBROptions.recommended's fixed table
+ * (SBROptions.swift:73-82). */
+int xtbeig_autotune(xtbeig_handle_t handle, int precision, int nmin, int nmax, int *bucket_start, int *band,
+                    float *ms, int *count);
+/* The tuned table (xtbeig_band_buckets() entries; band 0: not tuned, the built-in choice applies), to save and
+ * restore it. */
+int xtbeig_get_band_table(xtbeig_handle_t handle, int precision, int *bucket_start, int *band);
+int xtbeig_set_band_table(xtbeig_handle_t handle, int precision, const int *band);
+int xtbeig_band_buckets(void);
+
 /* SCC precision schedule, per handle (mirrors xTB's xtb_setSCCPrecision / xtb_getSCCPrecision contract, reference
  * LOG.md:356-370 and scripts/test_sccprecision.py:79-161): fp32_rmsq = 0 runs every cycle in FP64 (the default);
  * fp32_rmsq > 0 runs a cycle in FP32 while the caller's rms charge change is >= fp32_rmsq, for at most fp32_maxiter

@@ -765,4 +765,70 @@ int xtbeig_get_scc_precision(xtbeig_handle_t handle, double* fp32_rmsq, int* fp3
   return 0;
 }
 
+int xtbeig_dev_sygvd_graph(xtbeig_handle_t handle, int precision, int n, const double* hm, int ldh,
+                           const double* sm, int lds, double* w, double* c, int ldc) {
+  if (!handle) return set_error("xtbeig_dev_sygvd_graph: null handle");
+  if (n < 0 || ldh < (n > 1 ? n : 1) || lds < (n > 1 ? n : 1) || ldc < (n > 1 ? n : 1))
+    return set_error("xtbeig_dev_sygvd_graph: bad dimensions");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_dev_sygvd_graph: unknown precision");
+  return guarded([&] {
+    xtbeig::sygvd_graph(handle->h, n, hm, ldh, sm, lds, w, c, ldc, precision == XTBEIG_FP32_REFINE);
+  });
+}
+
+int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, const int* n, const double* const* hm,
+                             const int* ldh, const double* const* sm, const int* lds, double* const* w,
+                             double* const* c, const int* ldc, int slots, int use_graph, float* latency_ms) {
+  if (!handle) return set_error("xtbeig_dev_sygvd_batched: null handle");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_dev_sygvd_batched: unknown precision");
+  if (count < 0 || slots < 1 || slots > 64) return set_error("xtbeig_dev_sygvd_batched: count >= 0, 1 <= slots <= 64");
+  for (int q = 0; q < count; ++q)
+    if (n[q] < 1 || ldh[q] < n[q] || lds[q] < n[q] || ldc[q] < n[q])
+      return set_error("xtbeig_dev_sygvd_batched: bad dimensions");
+  return guarded([&] {
+    xtbeig::sygvd_batched(handle->h, count, n, hm, ldh, sm, lds, w, c, ldc, precision == XTBEIG_FP32_REFINE, slots,
+                          use_graph != 0, latency_ms);
+  });
+}
+
+int xtbeig_autotune(xtbeig_handle_t handle, int precision, int nmin, int nmax, int* bucket_start, int* band,
+                    float* ms, int* count) {
+  if (!handle) return set_error("xtbeig_autotune: null handle");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_autotune: unknown precision");
+  if (nmin < 2 || nmax < nmin || nmax > 4096) return set_error("xtbeig_autotune: need 2 <= nmin <= nmax <= 4096");
+  int cnt = 0;
+  for (int q = 0; q < Handle::NBUCKET; ++q)
+    if (!(xtbeig::bucket_edge(q + 1) - 1 < nmin || xtbeig::bucket_edge(q) > nmax)) ++cnt;
+  if (count) *count = cnt;
+  return guarded([&] {
+    xtbeig::autotune_bands(handle->h, precision == XTBEIG_FP32_REFINE, nmin, nmax, bucket_start, band, ms);
+  });
+}
+
+int xtbeig_get_band_table(xtbeig_handle_t handle, int precision, int* bucket_start, int* band) {
+  if (!handle) return set_error("xtbeig_get_band_table: null handle");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_get_band_table: unknown precision");
+  for (int q = 0; q < Handle::NBUCKET; ++q) {
+    if (bucket_start) bucket_start[q] = xtbeig::bucket_edge(q);
+    if (band) band[q] = handle->h.tuned_band[precision == XTBEIG_FP32_REFINE ? 1 : 0][q];
+  }
+  return 0;
+}
+
+int xtbeig_set_band_table(xtbeig_handle_t handle, int precision, const int* band) {
+  if (!handle || !band) return set_error("xtbeig_set_band_table: null argument");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_set_band_table: unknown precision");
+  for (int q = 0; q < Handle::NBUCKET; ++q)
+    if (band[q] < 0 || band[q] > 32) return set_error("xtbeig_set_band_table: band widths must be in [0, 32]");
+  for (int q = 0; q < Handle::NBUCKET; ++q) handle->h.tuned_band[precision == XTBEIG_FP32_REFINE ? 1 : 0][q] = band[q];
+  return 0;
+}
+
+int xtbeig_band_buckets(void) { return Handle::NBUCKET; }
+
 }  // extern "C"

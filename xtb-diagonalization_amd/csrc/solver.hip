@@ -56,6 +56,25 @@ void Handle::reserve(size_t bytes) {
   }
   XTB_HIP(hipMalloc(&arena, bytes));
   arena_bytes = bytes;
+  ++arena_generation;
+}
+
+int bucket_edge(int q) {
+  static const int edges[Handle::NBUCKET + 1] = {0, 192, 320, 512, 768, 1024, 1536, 2048, 4096, 1 << 30};
+  return edges[q];
+}
+
+int bucket_of(int n) {
+  int q = 0;
+  while (q + 1 < Handle::NBUCKET && n >= bucket_edge(q + 1)) ++q;
+  return q;
+}
+
+int Handle::band_for(int n, bool fp32) const {
+  if (band_override > 0) return band_override;
+  const int t = tuned_band[fp32 ? 1 : 0][bucket_of(n)];
+  if (t > 0) return std::min(t, std::max(2, n - 1));
+  return default_band(n, fp32);
 }
 
 void Handle::reserve_aux(size_t bytes) {
@@ -181,7 +200,7 @@ size_t syevd_bytes(int n, int b) {
 template <typename T>
 void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b) {
   if (n <= 0) return;
-  if (b <= 0) b = default_band(n, sizeof(T) == 4);
+  if (b <= 0) b = h.band_for(n, sizeof(T) == 4);
   b = std::max(1, std::min(b, std::max(1, n - 1)));
   h.reserve(syevd_bytes<T>(n, b));
   Carver c(h.arena);
@@ -221,7 +240,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
                   int ldc, bool mixed, int b) {
   if (n <= 0) return;
   hipStream_t st = h.stream;
-  if (b <= 0) b = default_band(n, mixed);
+  if (b <= 0) b = h.band_for(n, mixed);
   b = std::max(1, std::min(b, std::max(1, n - 1)));
   h.reserve(sygvd_bytes(n, b, mixed));
   Carver c(h.arena);

@@ -4,6 +4,8 @@
 #include "kernels.hpp"
 
 #include <map>
+#include <tuple>
+#include <vector>
 #include <memory>
 #include <string>
 
@@ -29,6 +31,11 @@ struct Handle {
   int scc_fp32_maxiter = 0, scc_fp32_refine = 0;
   int scc_fp32_cycles = 0, scc_fp64_restart = 0, scc_cycles = 0;
   bool scc_fp64_latched = false;
+  // autotuned band widths per n bucket (0: not tuned, the built-in table applies), [0] FP64, [1] FP32 reduction
+  static constexpr int NBUCKET = 9;
+  int tuned_band[2][NBUCKET] = {};
+  // arena reallocations so far: captured graphs remember the generation they point into
+  int arena_generation = 0;
   std::map<int, std::unique_ptr<DcPlan>> plans;
   std::map<std::pair<int, int>, std::unique_ptr<BackPlan>> backplans;
 
@@ -41,7 +48,25 @@ struct Handle {
   void* host_staging(size_t bytes);
   const DcPlan& plan(int n);
   const BackPlan& backplan(int n, int b);
+  // band width for an n x n solve: the caller's override, else the tuned table, else the built-in one
+  int band_for(int n, bool fp32) const;
+  // graph cache (batch.hip) and the concurrent sub-handles of the batched solve
+  struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    double *h = nullptr, *s = nullptr, *w = nullptr, *c = nullptr;  // staging: H, S in; w, C out (ld n)
+    int generation = -1;                                           // arena generation the graph points into
+    GraphEntry() = default;
+    GraphEntry(const GraphEntry&) = delete;
+    GraphEntry& operator=(const GraphEntry&) = delete;
+    ~GraphEntry();
+  };
+  std::map<std::tuple<int, int, int>, std::unique_ptr<GraphEntry>> graphs;
+  std::vector<std::unique_ptr<Handle>> subs;
 };
+
+// n bucket edges of the tuned table: bucket q holds bucket_edge(q) <= n < bucket_edge(q + 1)
+int bucket_of(int n);
+int bucket_edge(int q);
 
 int default_band(int n, bool fp32);
 
@@ -50,5 +75,14 @@ void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ld
 void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
                   int ldc, bool mixed, int b);
 void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz);
+
+// batch.hip: the generalized solve through a captured graph (one per (n, precision, band), inputs and outputs
+// staged through the graph's own buffers), the batched solve over concurrent sub-handles, and the band autotuner.
+void sygvd_graph(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
+                 int ldc, bool mixed);
+void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, const int* ldh,
+                   const double* const* sm, const int* lds, double* const* w, double* const* cm, const int* ldc,
+                   bool mixed, int slots, bool use_graph, float* latency_ms);
+void autotune_bands(Handle& h, bool mixed, int nmin, int nmax, int* buckets_out, int* bands_out, float* ms_out);
 
 }  // namespace xtbeig

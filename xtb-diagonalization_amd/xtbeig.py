@@ -64,6 +64,14 @@ SIGNATURES = {
     "xtbeig_dev_scc_sygvd": (ci, [vp, ci, vp, ci, vp, ci, vp, vp, ci, ctypes.c_double, c_int_p]),
     "xtbeig_get_scc_precision": (ci, [vp, ctypes.POINTER(ctypes.c_double), c_int_p, c_int_p, c_int_p, c_int_p,
                                       c_int_p]),
+    "xtbeig_dev_sygvd_graph": (ci, [vp, ci, ci, vp, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_dev_sygvd_batched": (ci, [vp, ci, ci, c_int_p, ctypes.POINTER(vp), c_int_p, ctypes.POINTER(vp), c_int_p,
+                                      ctypes.POINTER(vp), ctypes.POINTER(vp), c_int_p, ci, ci,
+                                      ctypes.POINTER(ctypes.c_float)]),
+    "xtbeig_autotune": (ci, [vp, ci, ci, ci, c_int_p, c_int_p, ctypes.POINTER(ctypes.c_float), c_int_p]),
+    "xtbeig_get_band_table": (ci, [vp, ci, c_int_p, c_int_p]),
+    "xtbeig_set_band_table": (ci, [vp, ci, c_int_p]),
+    "xtbeig_band_buckets": (ci, []),
     "xtbeig_create": (ci, [ctypes.POINTER(vp), ci]),
     "xtbeig_destroy": (ci, [vp]),
     "xtbeig_set_stream": (ci, [vp, vp]),
@@ -343,6 +351,53 @@ class Handle:
         import torch
         f = lib().xtbeig_dev_dback_band if z.dtype == torch.float64 else lib().xtbeig_dev_sback_band
         self._check(f(self._h, n, b, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(z.data_ptr()), n, m))
+
+    # ---- graphs, batches, autotuning ----
+    def sygvd_graph(self, h, s, w, c, precision=FP64):
+        n = h.shape[0]
+        self._check(lib().xtbeig_dev_sygvd_graph(self._h, precision, n, vp(h.data_ptr()), h.stride(0),
+                                                 vp(s.data_ptr()), s.stride(0), vp(w.data_ptr()), vp(c.data_ptr()),
+                                                 c.stride(0)))
+
+    def sygvd_batched(self, hs, ss, ws, cs, precision=FP64, slots=8, use_graph=False):
+        """Lists of device tensors (each h[q] n x n with ld = stride(0)); returns the per-solve latencies (ms)."""
+        cnt = len(hs)
+        arr_i = ctypes.c_int * max(cnt, 1)
+        arr_p = vp * max(cnt, 1)
+        ns = arr_i(*[t.shape[0] for t in hs])
+        ldh = arr_i(*[t.stride(0) for t in hs])
+        lds = arr_i(*[t.stride(0) for t in ss])
+        ldc = arr_i(*[t.stride(0) for t in cs])
+        ph = arr_p(*[t.data_ptr() for t in hs])
+        ps = arr_p(*[t.data_ptr() for t in ss])
+        pw = arr_p(*[t.data_ptr() for t in ws])
+        pc = arr_p(*[t.data_ptr() for t in cs])
+        lat = (ctypes.c_float * max(cnt, 1))()
+        self._check(lib().xtbeig_dev_sygvd_batched(self._h, precision, cnt, ns, ph, ldh, ps, lds, pw, pc, ldc,
+                                                   slots, 1 if use_graph else 0, lat))
+        return [lat[q] for q in range(cnt)]
+
+    def autotune(self, precision, nmin, nmax):
+        """Returns [(bucket_start_n, band, ms)] for the buckets tuned."""
+        nb = lib().xtbeig_band_buckets()
+        st = (ctypes.c_int * nb)()
+        bd = (ctypes.c_int * nb)()
+        ms = (ctypes.c_float * nb)()
+        cnt = ctypes.c_int(0)
+        self._check(lib().xtbeig_autotune(self._h, precision, nmin, nmax, st, bd, ms, ctypes.byref(cnt)))
+        return [(st[q], bd[q], ms[q]) for q in range(cnt.value)]
+
+    def band_table(self, precision):
+        nb = lib().xtbeig_band_buckets()
+        st = (ctypes.c_int * nb)()
+        bd = (ctypes.c_int * nb)()
+        self._check(lib().xtbeig_get_band_table(self._h, precision, st, bd))
+        return list(st), list(bd)
+
+    def set_band_table(self, precision, bands):
+        nb = lib().xtbeig_band_buckets()
+        bd = (ctypes.c_int * nb)(*bands)
+        self._check(lib().xtbeig_set_band_table(self._h, precision, bd))
 
     # ---- SCC precision schedule ----
     def set_scc_precision(self, fp32_rmsq, fp32_maxiter=160, fp32_refine=0):
