@@ -155,6 +155,20 @@ def dsygvd(h, s):
     return w, a, info.value
 
 
+def dsyevd_values(h):
+    """Eigenvalues of the symmetric h by the host LAPACK's dsyevd('N', 'U')."""
+    a = np.array(h, dtype=np.float64, order="F")
+    n = a.shape[0]
+    w = np.zeros(n)
+    lw = 2 * n + 1
+    liw = 1
+    work = np.zeros(lw)
+    iwork = np.zeros(liw, dtype=np.int32)
+    info = ctypes.c_int(0)
+    fn("dsyevd")(b"N", b"U", _i(n), _p(a), _i(n), _p(w), _p(work), _i(lw), _p(iwork), _i(liw), ctypes.byref(info))
+    return w, info.value
+
+
 def dpotrf_upper(s):
     b = np.array(s, dtype=np.float64, order="F")
     n = b.shape[0]
