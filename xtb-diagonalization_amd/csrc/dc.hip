@@ -115,39 +115,140 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
   int* ROT_N = ROT_P + n;
   double* ROT_C = reinterpret_cast<double*>(ROT_N + n + (n & 1));
   double* ROT_S = ROT_C + n;
-  __shared__ int sh_k, sh_nrot, sh_ctot[4], sh_early;
-  __shared__ double sh_rho;
-  const int t = threadIdx.x;
+  __shared__ int sh_k, sh_nrot, sh_ctot[4], sh_early, sh_serial;
+  __shared__ double sh_rho, sh_tol;
+  __shared__ double red_v[256];
+  __shared__ int red_i[256], scan[4][257];
+  const int t = threadIdx.x, nt = blockDim.x;
   const double* q = q_all + idx(off, off, ldq);
-  for (int i = t; i < n; i += blockDim.x) {
-    D[i] = d_all[off + i];
-    Z[i] = (i < n1) ? q[idx(n1 - 1, i, ldq)] : q[idx(n1, i, ldq)];
-    INDXQ[i] = (i < n1) ? i + 1 : i - n1 + 1;
+  const double rho_in = rho_pre[node_pre_index[blockIdx.x]];
+  const double tt = 1.0 / sqrt(2.0);
+  // z from the halves' boundary rows (sign of rho folded into the second half), scaled by 1 / sqrt(2); the halves
+  // are sorted, so dlaed2's INDXQ is the identity and DLAMDA = D before the merge
+  for (int i = t; i < n; i += nt) {
+    const double di = d_all[off + i];
+    double zi = (i < n1) ? q[idx(n1 - 1, i, ldq)] : q[idx(n1, i, ldq)];
+    if (rho_in < 0 && i >= n1) zi = -zi;
+    D[i] = di;
+    DL[i] = di;
+    Z[i] = zi * tt;
+    INDXQ[i] = i + 1;
   }
   __syncthreads();
-  if (t == 0) {
-    double rho = rho_pre[node_pre_index[blockIdx.x]];
-    if (rho < 0) for (int i = n1; i < n; ++i) Z[i] = -Z[i];
-    const double tt = 1.0 / sqrt(2.0);
-    for (int i = 0; i < n; ++i) Z[i] *= tt;
-    rho = fabs(2 * rho);
-    for (int i = n1; i < n; ++i) INDXQ[i] += n1;
-    for (int i = 0; i < n; ++i) DL[i] = D[INDXQ[i] - 1];
-    lamrg(n1, n2, DL, 1, 1, INDXC);
-    for (int i = 0; i < n; ++i) INDX[i] = INDXQ[INDXC[i] - 1];
-    int imax = 0, jmax = 0;
-    for (int i = 1; i < n; ++i) {
-      if (fabs(Z[i]) > fabs(Z[imax])) imax = i;
-      if (fabs(D[i]) > fabs(D[jmax])) jmax = i;
-    }
-    const double tol = 8 * EPS * fmax(fabs(D[jmax]), fabs(Z[imax]));
-    int k = 0, nrot = 0;
-    sh_early = 0;
-    if (rho * fabs(Z[imax]) <= tol) {
-      sh_early = 1;
+  const double rho = fabs(2 * rho_in);
+  // dlamrg of the two ascending halves by ranks: on ties the first half's element comes first
+  for (int i = t; i < n; i += nt) {
+    int r;
+    const double v = DL[i];
+    if (i < n1) {
+      int lo = 0, hi = n2;  // # second-half elements < v
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (DL[n1 + mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      r = i + lo;
     } else {
+      int lo = 0, hi = n1;  // # first-half elements <= v
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (DL[mid] <= v) lo = mid + 1;
+        else hi = mid;
+      }
+      r = (i - n1) + lo;
+    }
+    INDXC[r] = i + 1;
+    INDX[r] = i + 1;
+  }
+  // idamax of z and of d (first index of the largest magnitude)
+  {
+    double bz = -1, bd = -1;
+    int iz = 0, id = 0;
+    for (int i = t; i < n; i += nt) {
+      if (fabs(Z[i]) > bz) { bz = fabs(Z[i]); iz = i; }
+      if (fabs(D[i]) > bd) { bd = fabs(D[i]); id = i; }
+    }
+    red_v[t] = bz;
+    red_i[t] = iz;
+    __syncthreads();
+    for (int st = nt / 2; st > 0; st >>= 1) {
+      if (t < st && (red_v[t + st] > red_v[t] || (red_v[t + st] == red_v[t] && red_i[t + st] < red_i[t]))) {
+        red_v[t] = red_v[t + st];
+        red_i[t] = red_i[t + st];
+      }
+      __syncthreads();
+    }
+    const int imax = red_i[0];
+    __syncthreads();
+    red_v[t] = bd;
+    red_i[t] = id;
+    __syncthreads();
+    for (int st = nt / 2; st > 0; st >>= 1) {
+      if (t < st && (red_v[t + st] > red_v[t] || (red_v[t + st] == red_v[t] && red_i[t + st] < red_i[t]))) {
+        red_v[t] = red_v[t + st];
+        red_i[t] = red_i[t + st];
+      }
+      __syncthreads();
+    }
+    const int jmax = red_i[0];
+    if (t == 0) {
+      sh_tol = 8 * EPS * fmax(fabs(D[jmax]), fabs(Z[imax]));
+      sh_early = (rho * fabs(Z[imax]) <= sh_tol) ? 1 : 0;
+      sh_rho = rho;
+      sh_serial = 0;
+    }
+    __syncthreads();
+  }
+  const double tol = sh_tol;
+  // small z (sorted position j) into ROT_P as flags, and whether any two consecutive large-z poles pass dlaed2's
+  // close-pole test on the original values: if none does, the scan makes no rotation and its result follows from the
+  // flags alone (a rotation changes only the values of the pair it merges, so the first rotation of the serial scan
+  // is at the first pair that passes here).
+  int* SMALL = ROT_P;
+  if (!sh_early) {
+    const int per = (n + nt - 1) / nt, j0 = min(n, t * per), j1 = min(n, j0 + per);
+    int last = -1;
+    for (int j = j0; j < j1; ++j) {
+      const int sm = (rho * fabs(Z[INDX[j] - 1]) <= tol) ? 1 : 0;
+      SMALL[j] = sm;
+      if (!sm) last = j;
+    }
+    scan[0][t] = last;
+    __syncthreads();
+    if (t == 0) {
+      int run = -1;
+      for (int u = 0; u < nt; ++u) {
+        const int v = scan[0][u];
+        scan[0][u] = run;  // exclusive: the last large-z position before chunk u
+        if (v >= 0) run = v;
+      }
+    }
+    __syncthreads();
+    int prev = scan[0][t];
+    bool close = false;
+    for (int j = j0; j < j1; ++j) {
+      if (!SMALL[j]) {
+        if (prev >= 0) {
+          const int pj = INDX[prev], nj = INDX[j];
+          double s2 = Z[pj - 1], c2 = Z[nj - 1];
+          const double tau = lapy2(c2, s2);
+          const double tdiff = D[nj - 1] - D[pj - 1];
+          c2 = c2 / tau;
+          s2 = -s2 / tau;
+          if (fabs(tdiff * c2 * s2) <= tol) close = true;
+        }
+        prev = j;
+      }
+    }
+    if (close) atomicOr(&sh_serial, 1);
+    __syncthreads();
+  }
+  if (!sh_early && sh_serial) {
+    // dlaed2's scan, serial: close poles are merged by Givens rotations, which change D and Z as the scan goes
+    if (t == 0) {
       for (int i = 0; i < n1; ++i) COLTYP[i] = 1;
       for (int i = n1; i < n; ++i) COLTYP[i] = 3;
+      int k = 0, nrot = 0;
       int k2 = n + 1, j = 1, pj = 0;
       // first non-deflated column (one exists: z[imax] is not small)
       for (; j <= n; ++j) {
@@ -209,24 +310,79 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
       DL[k - 1] = D[pj - 1];
       W[k - 1] = Z[pj - 1];
       INDXP[k - 1] = pj;
-      int ctot[4] = {0, 0, 0, 0};
-      for (int jj = 0; jj < n; ++jj) ctot[COLTYP[jj] - 1]++;
-      int psm[4];
-      psm[0] = 1; psm[1] = 1 + ctot[0]; psm[2] = psm[1] + ctot[1]; psm[3] = psm[2] + ctot[2];
-      k = n - ctot[3];
-      for (int jj = 1; jj <= n; ++jj) {
-        const int js = INDXP[jj - 1], ct = COLTYP[js - 1];
-        INDX[psm[ct - 1] - 1] = js;
-        INDXC[psm[ct - 1] - 1] = jj;
-        psm[ct - 1]++;
-      }
-      for (int c = 0; c < 4; ++c) sh_ctot[c] = ctot[c];
+      sh_nrot = nrot;
     }
-    sh_k = k;
-    sh_nrot = nrot;
-    sh_rho = rho;
+    __syncthreads();
+  } else if (!sh_early) {
+    // no rotation: large-z poles keep their sorted order at 0 ..< K, small-z ones fill n - 1 down to K in the order
+    // met (ranks from per-chunk counts)
+    const int per = (n + nt - 1) / nt, j0 = min(n, t * per), j1 = min(n, j0 + per);
+    int nl = 0;
+    for (int j = j0; j < j1; ++j) nl += !SMALL[j];
+    scan[1][t] = nl;
+    __syncthreads();
+    if (t == 0) {
+      int run = 0;
+      for (int u = 0; u < nt; ++u) {
+        const int v = scan[1][u];
+        scan[1][u] = run;
+        run += v;
+      }
+      sh_nrot = 0;
+    }
+    __syncthreads();
+    int kl = scan[1][t], ks = j0 - scan[1][t];  // large / small met before this chunk
+    for (int j = j0; j < j1; ++j) {
+      const int nj = INDX[j];
+      if (!SMALL[j]) {
+        DL[kl] = D[nj - 1];
+        W[kl] = Z[nj - 1];
+        INDXP[kl] = nj;
+        ++kl;
+      } else {
+        INDXP[n - 1 - ks] = nj;
+        ++ks;
+      }
+    }
+    for (int i = t; i < n; i += nt) COLTYP[i] = (i < n1) ? 1 : 3;
+    __syncthreads();
+    for (int j = j0; j < j1; ++j)
+      if (SMALL[j]) COLTYP[INDX[j] - 1] = 4;
+    __syncthreads();
   }
-  __syncthreads();
+  if (!sh_early) {
+    // the four column groups: INDX / INDXC by type, stable (dlaed2's PSM pass), from per-chunk counts
+    const int per = (n + nt - 1) / nt, j0 = min(n, t * per), j1 = min(n, j0 + per);
+    int cnt4[4] = {0, 0, 0, 0};
+    for (int j = j0; j < j1; ++j) cnt4[COLTYP[INDXP[j] - 1] - 1]++;
+    for (int c = 0; c < 4; ++c) scan[c][t] = cnt4[c];
+    __syncthreads();
+    if (t < 4) {
+      int run = 0;
+      for (int u = 0; u < nt; ++u) {
+        const int v = scan[t][u];
+        scan[t][u] = run;
+        run += v;
+      }
+      scan[t][nt] = run;
+    }
+    __syncthreads();
+    const int c0 = scan[0][nt], c1 = scan[1][nt], c2 = scan[2][nt];
+    const int base[4] = {0, c0, c0 + c1, c0 + c1 + c2};
+    int pos[4];
+    for (int c = 0; c < 4; ++c) pos[c] = base[c] + scan[c][t];
+    for (int j = j0; j < j1; ++j) {
+      const int js = INDXP[j], ct = COLTYP[js - 1] - 1;
+      INDX[pos[ct]] = js;
+      INDXC[pos[ct]] = j + 1;
+      pos[ct]++;
+    }
+    if (t == 0) {
+      for (int c = 0; c < 4; ++c) sh_ctot[c] = scan[c][nt];
+      sh_k = n - scan[3][nt];
+    }
+    __syncthreads();
+  }
   if (sh_early) {
     // K = 0: every column deflates, in sorted order (Q2 = Q(:, INDX), d = D(INDX))
     if (t == 0) {
@@ -309,6 +465,8 @@ __global__ void q2_gather_kernel(const Node* nodes, const NodeOut* out, const in
 }
 
 // One lane per root. S (the K x K eigenvector matrix of the rank-one problem, ld sz) is written here only for K <= 2.
+constexpr int SEC_G = 8;  // lanes per secular root: each sums every SEC_G-th pole
+
 __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const NodeOut* out, const double* rho_lvl,
                                                      const double* __restrict__ dlamda_all,
                                                      const double* __restrict__ w_all, const int* __restrict__ indxc_all,
@@ -316,26 +474,30 @@ __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const No
                                                      double* __restrict__ tau_all, double* __restrict__ s_all,
                                                      int* info) {
   extern __shared__ double poles[];  // dlamda (k), then z (k): every root reads all of them each iteration
+  constexpr int RPB = 64 / SEC_G;    // roots per workgroup
   const Node nd = nodes[blockIdx.y];
   const int k = out[blockIdx.y].k;
-  if (blockIdx.x * 64 >= k) return;
+  if (blockIdx.x * RPB >= k) return;
   for (int i = threadIdx.x; i < k; i += 64) {
     poles[i] = dlamda_all[nd.off + i];
     poles[k + i] = w_all[nd.off + i];
   }
   __syncthreads();
-  const int j = blockIdx.x * 64 + threadIdx.x;
-  if (j >= k) return;
+  const int j = blockIdx.x * RPB + threadIdx.x / SEC_G, gl = threadIdx.x % SEC_G;
+  if (j >= k) return;  // whole groups leave together
   const double* dl = poles;
   const double* w = poles + k;
   const double rho = rho_lvl[blockIdx.y];
   double* S = s_all + nd.q2off;
   if (k == 1) {
-    d_all[nd.off] = dl[0] + rho * w[0] * w[0];
-    S[0] = 1;
+    if (gl == 0) {
+      d_all[nd.off] = dl[0] + rho * w[0] * w[0];
+      S[0] = 1;
+    }
     return;
   }
   if (k == 2) {
+    if (gl != 0) return;
     double v[2], lam;
     const double dd[2] = {dl[0], dl[1]}, zz[2] = {w[0], w[1]};
     secular::laed5(j, dd, zz, rho, v, lam);
@@ -347,7 +509,8 @@ __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const No
   }
   int orig = 0;
   double tau = 0;
-  const int bad = secular::laed4(k, j, dl, w, rho, orig, tau);
+  const int bad = secular::laed4<SEC_G>(k, j, dl, w, rho, orig, tau, gl);
+  if (gl != 0) return;
   if (bad) atomicCAS(info, 0, nd.off + j + 1);
   orig_all[nd.off + j] = orig;
   tau_all[nd.off + j] = tau;
@@ -601,7 +764,8 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
     XTB_HIP(hipMemcpyAsync(probe->rho, W.rho_lvl, sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   allow_lds(secular_kernel, 2 * sizeof(double) * maxsz);
-  hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 2 * sizeof(double) * maxsz, s, lvl, W.outs,
+  hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64 / SEC_G), cnt), dim3(64), 2 * sizeof(double) * maxsz, s, lvl,
+                     W.outs,
                      W.rho_lvl, W.dlamda, W.w, W.indxc, d, W.orig, W.tau, W.sbuf, info);
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 0, s, lvl, W.outs, W.dlamda, W.w, W.orig,

@@ -157,32 +157,52 @@ __device__ inline void laed5(int i, const double d[2], const double z[2], double
   v[1] /= nrm;
 }
 
-// Sums over the poles for the current shift: psi (poles 0 .. ilo - 1) and phi (poles ihi .. k - 1, summed from the
-// top, as the Fortran loops run), their derivatives and the error bound's accumulation.
+// Sums over the poles for the current shift: psi (poles 0 .. ilo - 1) and phi (poles ihi .. k - 1), their
+// derivatives and the error bound's accumulation. G lanes share one root (lane gl of the group takes poles
+// gl, gl + G, ...; the group's sums are combined so that every lane holds the same bits). The Fortran loop adds the
+// running psi after each pole (and the running phi, from the top) into the error bound: that is the weighted sum
+// sum_j term_j (ilo - j) (and sum_j term_j (j - ihi + 1)), formed here directly.
 struct Sums { double psi, dpsi, phi, dphi, erretm; };
-__device__ inline Sums pole_sums(int k, int ilo, int ihi, const double* d, const double* z, double dorg, double tau) {
-  Sums r{0, 0, 0, 0, 0};
-  for (int j = 0; j < ilo; ++j) {
+template <int G>
+__device__ inline double gsum(double v) {
+  if constexpr (G == 1) return v;
+  else return group_sum_lanes<G>(v);
+}
+template <int G>
+__device__ inline Sums pole_sums(int k, int ilo, int ihi, const double* d, const double* z, double dorg, double tau,
+                                 int gl) {
+  double psi = 0, dpsi = 0, phi = 0, dphi = 0, e1 = 0, e2 = 0;
+  for (int j = gl; j < ilo; j += G) {
     const double dj = (d[j] - dorg) - tau;
     const double t = z[j] / dj;
-    r.psi += z[j] * t;
-    r.dpsi += t * t;
-    r.erretm += r.psi;
+    const double term = z[j] * t;
+    psi += term;
+    dpsi += t * t;
+    e1 += term * (double)(ilo - j);
   }
-  r.erretm = fabs(r.erretm);
-  for (int j = k - 1; j >= ihi; --j) {
+  for (int j = ihi + gl; j < k; j += G) {
     const double dj = (d[j] - dorg) - tau;
     const double t = z[j] / dj;
-    r.phi += z[j] * t;
-    r.dphi += t * t;
-    r.erretm += r.phi;
+    const double term = z[j] * t;
+    phi += term;
+    dphi += t * t;
+    e2 += term * (double)(j - ihi + 1);
   }
+  Sums r;
+  r.psi = gsum<G>(psi);
+  r.dpsi = gsum<G>(dpsi);
+  r.phi = gsum<G>(phi);
+  r.dphi = gsum<G>(dphi);
+  r.erretm = fabs(gsum<G>(e1)) + gsum<G>(e2);
   return r;
 }
 
-// dlaed4 for root i (0-based) of the k >= 3 problem. Outputs the origin pole index and tau with
-// lambda = d[orig] + tau. Returns info (0: converged).
-__device__ inline int laed4(int k, int i, const double* d, const double* z, double rho, int& orig, double& tau) {
+// dlaed4 for root i (0-based) of the k >= 3 problem, on the G lanes of a group (gl: lane within the group; every
+// lane returns the same result). Outputs the origin pole index and tau with lambda = d[orig] + tau. Returns info
+// (0: converged).
+template <int G>
+__device__ inline int laed4(int k, int i, const double* d, const double* z, double rho, int& orig, double& tau,
+                            int gl) {
   const double rhoinv = 1 / rho;
   if (i == k - 1) {
     // the largest root, origin d[k - 1]
@@ -191,7 +211,8 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
     const double dorg = d[n - 1];
     const double midpt = rho / 2;
     double psi = 0;
-    for (int j = 0; j < n - 2; ++j) psi += z[j] * z[j] / ((d[j] - dorg) - midpt);
+    for (int j = gl; j < n - 2; j += G) psi += z[j] * z[j] / ((d[j] - dorg) - midpt);
+    psi = gsum<G>(psi);
     double c = rhoinv + psi;
     double w = c + z[ii] * z[ii] / ((d[ii] - dorg) - midpt) + z[n - 1] * z[n - 1] / ((d[n - 1] - dorg) - midpt);
     double dltlb, dltub;
@@ -219,7 +240,7 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
     }
     for (int niter = 1; niter <= MAXIT + 1; ++niter) {
       // psi over poles 0 .. n - 2, phi from pole n - 1 alone
-      Sums sm = pole_sums(n, ii + 1, n, d, z, dorg, tau);
+      Sums sm = pole_sums<G>(n, ii + 1, n, d, z, dorg, tau, gl);
       const double dn = (d[n - 1] - dorg) - tau, dnm1 = (d[n - 2] - dorg) - tau;
       const double t = z[n - 1] / dn;
       const double phi = z[n - 1] * t, dphi = t * t;
@@ -248,8 +269,10 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
   const double del = d[ip1] - d[i];
   const double midpt = del / 2;
   double psi = 0, phi = 0;
-  for (int j = 0; j < i; ++j) psi += z[j] * z[j] / ((d[j] - d[i]) - midpt);
-  for (int j = k - 1; j >= i + 2; --j) phi += z[j] * z[j] / ((d[j] - d[i]) - midpt);
+  for (int j = gl; j < i; j += G) psi += z[j] * z[j] / ((d[j] - d[i]) - midpt);
+  for (int j = i + 2 + gl; j < k; j += G) phi += z[j] * z[j] / ((d[j] - d[i]) - midpt);
+  psi = gsum<G>(psi);
+  phi = gsum<G>(phi);
   double c = rhoinv + psi + phi;
   double w = c + z[i] * z[i] / ((d[i] - d[i]) - midpt) + z[ip1] * z[ip1] / ((d[ip1] - d[i]) - midpt);
   bool orgati;
@@ -275,7 +298,7 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
   const double dorg = d[orig];
   const int ii = orig, iim1 = ii - 1, iip1 = ii + 1;
   auto delta = [&](int j, double tt) { return (d[j] - dorg) - tt; };
-  Sums sm = pole_sums(k, ii, iip1, d, z, dorg, tau);
+  Sums sm = pole_sums<G>(k, ii, iip1, d, z, dorg, tau, gl);
   w = rhoinv + sm.phi + sm.psi;
   bool swtch3 = orgati ? (w < 0) : (w > 0);
   if (ii == 0 || ii == k - 1) swtch3 = false;
@@ -349,7 +372,7 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
     if (tn > dltub || tn < dltlb) eta = (w < 0) ? (dltub - tau) / 2 : (dltlb - tau) / 2;
     const double prew = w;
     tau += eta;
-    sm = pole_sums(k, ii, iip1, d, z, dorg, tau);
+    sm = pole_sums<G>(k, ii, iip1, d, z, dorg, tau, gl);
     temp = z[ii] / delta(ii, tau);
     dw = sm.dpsi + sm.dphi + temp * temp;
     temp = z[ii] * temp;
