@@ -243,7 +243,7 @@ static void sygvd_host(bool mixed, const int* itype, const char* jobz, const cha
       // b on exit: the Cholesky factor in the referenced triangle (U, or L = U^T), the other triangle untouched,
       // as LAPACK's dsygvd leaves it
       xtbeig::copy_matrix<double, double>(h.stream, n, n, ds.as<double>(), n, ds2.as<double>(), n);
-      h.reserve((size_t)n * 64 * sizeof(double) + 4096);
+      h.reserve(xtbeig::potrf_workspace(n) * sizeof(double) + 4096);
       xtbeig::potrf_inverse(h.stream, n, ds2.as<double>(), n, du.as<double>(), n, dx.as<double>(), n,
                             static_cast<double*>(h.arena), h.dinfo);
       if (ul == 'L') xtbeig::transpose_matrix<double, double>(h.stream, n, n, du.as<double>(), n, dx.as<double>(), n);
@@ -401,7 +401,7 @@ int xtbeig_dev_potrf_inverse(xtbeig_handle_t handle, int n, double* s, int lds, 
                              int ldx) {
   return guarded([&] {
     Handle& h = handle->h;
-    h.reserve((size_t)n * 64 * sizeof(double) + 4096);
+    h.reserve(xtbeig::potrf_workspace(n) * sizeof(double) + 4096);
     xtbeig::potrf_inverse(h.stream, n, s, lds, u, ldu, x, ldx, static_cast<double*>(h.arena), h.dinfo);
   });
 }

@@ -56,8 +56,8 @@ __device__ inline void lamrg(int n1, int n2, const double* a, int s1, int s2, in
 }
 
 // Scale by the max norm, apply every tear (pre-order, parent before children), Q = I.
-__global__ void prepare_kernel(int n, double* d, double* e, const Node* nodes_pre, int nnodes, double* rho,
-                               double* scale) {
+__global__ __launch_bounds__(1024) void prepare_kernel(int n, double* d, double* e, const int* __restrict__ split_pre,
+                                                       double* rho, double* scale) {
   __shared__ double red[16];
   const int t = threadIdx.x;
   double m = 0;
@@ -70,18 +70,24 @@ __global__ void prepare_kernel(int n, double* d, double* e, const Node* nodes_pr
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) nrm = fmax(nrm, red[w]);
   if (nrm == 0) nrm = 1;
   if (t == 0) *scale = nrm;
-  for (int i = t; i < n; i += blockDim.x) d[i] /= nrm;
   for (int i = t; i < n - 1; i += blockDim.x) e[i] /= nrm;
   __syncthreads();
-  if (t == 0) {
-    for (int p = 0; p < nnodes; ++p) {
-      const Node nd = nodes_pre[p];
-      const int m1 = nd.off + nd.n1;
-      const double r = e[m1 - 1];
-      rho[p] = r;
-      d[m1 - 1] -= fabs(r);
-      d[m1] -= fabs(r);
+  // every tear at row m (merge p in pre-order) subtracts |e[m - 1]| from d[m - 1] and d[m]: row i meets the tears at
+  // i (as d[m]) and at i + 1 (as d[m - 1]), applied in pre-order, as the tree's recursion applies them
+  for (int i = t; i < n; i += blockDim.x) {
+    double di = d[i] / nrm;
+    const int pa = (i >= 1) ? split_pre[i] : -1, pb = (i + 1 <= n - 1) ? split_pre[i + 1] : -1;
+    const double va = (pa >= 0) ? fabs(e[i - 1]) : 0.0, vb = (pb >= 0) ? fabs(e[i]) : 0.0;
+    if (pa >= 0 && pb >= 0) {
+      if (pa < pb) { di -= va; di -= vb; }
+      else { di -= vb; di -= va; }
+    } else if (pa >= 0) {
+      di -= va;
+    } else if (pb >= 0) {
+      di -= vb;
     }
+    d[i] = di;
+    if (pa >= 0) rho[pa] = e[i - 1];
   }
 }
 
@@ -521,21 +527,32 @@ __device__ inline double delta_of(const double* dl, int i, int orig, double tau)
 
 // Gu-Eisenstat: w_i = sign(z_i) sqrt(-delta(i, i) prod_{j != i} delta(i, j) / (d_i - d_j)), same product order as
 // dlaed3 (j ascending).
+template <int G>
+__device__ inline double group_prod(double v) {
+  if (G >= 2) v *= dpp_mov<DPP_XOR1>(v);
+  if (G >= 4) v *= dpp_mov<DPP_XOR2>(v);
+  if (G >= 8) v *= dpp_mov<DPP_HALF_MIRROR>(v);
+  return v;
+}
+constexpr int GU_G = 8;  // lanes per z_i: each multiplies every GU_G-th factor
+
 __global__ void gu_kernel(const Node* nodes, const NodeOut* out, const double* __restrict__ dlamda_all,
                           const double* __restrict__ w_all, const int* __restrict__ orig_all,
                           const double* __restrict__ tau_all, double* __restrict__ wnew_all) {
   const Node nd = nodes[blockIdx.y];
   const int k = out[blockIdx.y].k;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k <= 2 || i >= k) return;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / GU_G, gl = threadIdx.x % GU_G;
+  if (k <= 2 || i >= k) return;  // whole groups leave together
   const double* dl = dlamda_all + nd.off;
   const int* og = orig_all + nd.off;
   const double* ta = tau_all + nd.off;
-  double prod = delta_of(dl, i, og[i], ta[i]);
-  for (int j = 0; j < k; ++j) {
+  double prod = (gl == 0) ? delta_of(dl, i, og[i], ta[i]) : 1.0;
+  for (int j = gl; j < k; j += GU_G) {
     if (j == i) continue;
     prod *= delta_of(dl, i, og[j], ta[j]) / (dl[i] - dl[j]);
   }
+  prod = group_prod<GU_G>(prod);
+  if (gl != 0) return;
   const double mag = sqrt(-prod);
   wnew_all[nd.off + i] = (w_all[nd.off + i] < 0) ? -mag : mag;
 }
@@ -596,17 +613,42 @@ __global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, cons
 }
 
 // dlamrg of the K merged (ascending) and n - K deflated (descending) eigenvalues.
-__global__ void lamrg_kernel(int count, const Node* nodes, const NodeOut* out, const double* __restrict__ d_all,
-                             int* __restrict__ perm_all) {
-  const int node = blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= count) return;
-  const Node nd = nodes[node];
-  const int k = out[node].k;
+// dlamrg(k, sz - k, d, 1, -1) per merge (one workgroup each): the K new eigenvalues ascending, the deflated ones
+// stored descending (read backwards), merged by ranks; on ties the first list's element comes first, as dlamrg.
+// K = 0 (every column deflated at once) leaves them ascending: the identity.
+__global__ __launch_bounds__(256) void lamrg_kernel(int count, const Node* nodes, const NodeOut* out,
+                                                    const double* __restrict__ d_all, int* __restrict__ perm_all) {
+  const Node nd = nodes[blockIdx.x];
+  const int k = out[blockIdx.x].k, sz = nd.sz, n2 = sz - k;
+  const double* a = d_all + nd.off;
   int* perm = perm_all + nd.off;
+  (void)count;
   if (k == 0) {
-    for (int i = 0; i < nd.sz; ++i) perm[i] = i + 1;
-  } else {
-    lamrg(k, nd.sz - k, d_all + nd.off, 1, -1, perm);
+    // every column deflated at once: they are already ascending
+    for (int i = threadIdx.x; i < sz; i += blockDim.x) perm[i] = i + 1;
+    return;
+  }
+  for (int i = threadIdx.x; i < sz; i += blockDim.x) {
+    if (i < k) {
+      const double v = a[i];
+      int lo = 0, hi = n2;  // # second-list elements (b_q = a[sz - 1 - q]) < v
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[sz - 1 - mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      perm[i + lo] = i + 1;
+    } else {
+      const int q = sz - 1 - i;  // traversal position in the second list
+      const double v = a[i];
+      int lo = 0, hi = k;  // # first-list elements <= v
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1;
+        else hi = mid;
+      }
+      perm[q + lo] = i + 1;
+    }
   }
 }
 
@@ -685,6 +727,10 @@ DcPlan::DcPlan(int n_) : n(n_) {
     }
     lvls.push_back(lv);
   }
+  std::vector<int> split_pre(n + 1, -1);
+  for (size_t p = 0; p < pre.size(); ++p) split_pre[pre[p].off + pre[p].n1] = (int)p;
+  XTB_HIP(hipMalloc(&split_pre_dev, sizeof(int) * split_pre.size()));
+  XTB_HIP(hipMemcpy(split_pre_dev, split_pre.data(), sizeof(int) * split_pre.size(), hipMemcpyHostToDevice));
   XTB_HIP(hipMalloc(&nodes_dev, sizeof(Node) * nodes.size()));
   XTB_HIP(hipMalloc(&pre_index_dev, sizeof(int) * pre_index.size()));
   XTB_HIP(hipMemcpy(nodes_dev, nodes.data(), sizeof(Node) * nodes.size(), hipMemcpyHostToDevice));
@@ -694,6 +740,7 @@ DcPlan::DcPlan(int n_) : n(n_) {
 DcPlan::~DcPlan() {
   if (nodes_dev) (void)hipFree(nodes_dev);
   if (pre_index_dev) (void)hipFree(pre_index_dev);
+  if (split_pre_dev) (void)hipFree(split_pre_dev);
 }
 
 // Workspace (doubles): dlamda, w, tau, wnew (n each), dtmp (2n), rho_pre, rho_lvl (n each), scale (16), rotation
@@ -768,7 +815,7 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
                      W.outs,
                      W.rho_lvl, W.dlamda, W.w, W.indxc, d, W.orig, W.tau, W.sbuf, info);
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz, 64), cnt), dim3(64), 0, s, lvl, W.outs, W.dlamda, W.w, W.orig,
+  hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz * GU_G, 64), cnt), dim3(64), 0, s, lvl, W.outs, W.dlamda, W.w, W.orig,
                      W.tau, W.wnew);
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(maxsz, 4), cnt), dim3(256), 0, s, lvl, W.outs, W.dlamda, W.orig,
@@ -777,7 +824,7 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
   hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), 2 * cnt), dim3(256), 0, s, lvl,
                      W.outs, W.q2, W.sbuf, z, ldz);
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(lamrg_kernel, dim3(cdiv(cnt, 256)), dim3(256), 0, s, cnt, lvl, W.outs, d, W.perm);
+  hipLaunchKernelGGL(lamrg_kernel, dim3(cnt), dim3(256), 0, s, cnt, lvl, W.outs, d, W.perm);
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(gather_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, W.outs, W.perm, z, ldz,
                      W.q2, d, W.sbuf, W.dtmp);
@@ -800,7 +847,7 @@ void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, i
   const DcWork W = carve(n, work, iwork);
   const Node* nodes = plan.nodes_dev;
   set_identity(s, n, z, ldz);
-  hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(256), 0, s, n, d, e, nodes, plan.nmerge, W.rho_pre, W.scale);
+  hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(1024), 0, s, n, d, e, plan.split_pre_dev, W.rho_pre, W.scale);
   XTB_LAUNCH_CHECK();
   for (int L = (int)plan.lvls.size() - 1; L >= 0; --L) {
     const DcPlan::Level& lv = plan.lvls[L];

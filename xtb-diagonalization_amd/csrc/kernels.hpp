@@ -34,7 +34,8 @@ void symmetrize_pad(hipStream_t s, int n, const double* src, int lds, double* ds
 // chol.hip ---------------------------------------------------------------------------------------------------------
 // S = U^T U (upper Cholesky) of the symmetric positive definite a (n x n, full storage, destroyed); writes U (zero
 // below the diagonal) and X = U^{-1} (zero below). info (device int): 0, or the LAPACK potrf index j > 0 of the
-// first non-positive pivot. tmp: n * nb elements.
+// first non-positive pivot. tmp: potrf_workspace(n) doubles.
+size_t potrf_workspace(int n);
 void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu, double* x, int ldx, double* tmp,
                    int* info);
 
@@ -69,6 +70,7 @@ struct DcPlan {
   std::vector<Level> lvls;  // by depth, root first
   DcNode* nodes_dev = nullptr;
   int* pre_index_dev = nullptr;
+  int* split_pre_dev = nullptr;  // n + 1: pre-order index of the merge that tears at row m, or -1
   explicit DcPlan(int n);
   ~DcPlan();
   DcPlan(const DcPlan&) = delete;

@@ -15,41 +15,25 @@ namespace xtbeig {
 
 namespace {
 
-// R <- I - R (R = C^T S C on entry), lambda_i = A_ii / (C^T S C)_ii; stats[0] = max |R_ij|, stats[1] = max |lambda|.
+// R <- I - R (R = C^T S C on entry), lambda_i = A_ii / (C^T S C)_ii.
 __global__ void oa_prepare_kernel(int n, double* __restrict__ r, int ldr, const double* __restrict__ a, int lda,
-                                  double* __restrict__ lam, unsigned long long* __restrict__ stats) {
+                                  double* __restrict__ lam) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  double m = 0;
-  if (i < n) {
-    const double v = r[idx(i, j, ldr)];
-    if (i == j) lam[i] = a[idx(i, i, lda)] / v;
-    const double rv = (i == j ? 1.0 : 0.0) - v;
-    r[idx(i, j, ldr)] = rv;
-    m = fabs(rv);
-  }
-  m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) atomicMax(&stats[0], __double_as_longlong(m));  // non-negative doubles order as ints
-  if (i < n && i == j) {
-    const double l = fabs(a[idx(i, i, lda)] / (1.0 - r[idx(i, j, ldr)]));
-    atomicMax(&stats[1], __double_as_longlong(l));
-  }
+  if (i >= n) return;
+  const double v = r[idx(i, j, ldr)];
+  if (i == j) lam[i] = a[idx(i, i, lda)] / v;
+  r[idx(i, j, ldr)] = (i == j ? 1.0 : 0.0) - v;
 }
 
 __global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, const double* __restrict__ a, int lda,
-                                 const double* __restrict__ lam, const unsigned long long* __restrict__ stats,
-                                 double* __restrict__ e, int lde) {
+                                 const double* __restrict__ lam, double* __restrict__ e, int lde) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= n) return;
-  (void)stats;
   const double rij = r[idx(i, j, ldr)];
   const double gap = lam[j] - lam[i];
   double v = (a[idx(i, j, lda)] + lam[j] * rij) / gap;
   if (i == j || !(fabs(v) <= 0.25)) v = 0.5 * rij;
   e[idx(i, j, lde)] = v;
-}
-
-__global__ void zero_stats_kernel(unsigned long long* stats) {
-  if (threadIdx.x < 2) stats[threadIdx.x] = 0;
 }
 
 // rank of lambda_i in ascending order (ties by index)
@@ -95,18 +79,15 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   double* e = a + nn;
   double* cnew = e + nn;
   double* lam = cnew + nn;
-  unsigned long long* stats = reinterpret_cast<unsigned long long*>(lam + n + n);
   const dim3 eg(cdiv(n, 256), n);
   for (int it = 0; it < iterations; ++it) {
     gemm<double>(s, 'N', 'N', n, n, n, 1.0, sm, lds, c, ldc, 0.0, sc, n);   // S C
     gemm<double>(s, 'N', 'N', n, n, n, 1.0, h, ldh, c, ldc, 0.0, hc, n);    // H C
     gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, sc, n, 0.0, r, n);      // C^T S C
     gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, hc, n, 0.0, a, n);      // C^T H C
-    hipLaunchKernelGGL(zero_stats_kernel, dim3(1), dim3(64), 0, s, stats);
+    hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam);
     XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, stats);
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, stats, e, n);
+    hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n);
     XTB_LAUNCH_CHECK();
     copy_matrix<double, double>(s, n, n, c, ldc, cnew, n);
     gemm<double>(s, 'N', 'N', n, n, n, 1.0, c, ldc, e, n, 1.0, cnew, n);    // C + C E

@@ -216,7 +216,7 @@ size_t sygvd_bytes(int n, int b, bool mixed) {
   c.take<double>(nn);  // S copy
   c.take<double>(nn);  // U
   c.take<double>(nn);  // X
-  c.take<double>((size_t)n * 64);
+  c.take<double>(potrf_workspace(n));
   if (!mixed) {
     c.take<double>(nn);  // H X
     c.take<double>(nn);  // H'
@@ -248,7 +248,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   double* scopy = c.take<double>(nn);
   double* u = c.take<double>(nn);
   double* x = c.take<double>(nn);
-  double* tmp = c.take<double>((size_t)n * 64);
+  double* tmp = c.take<double>(potrf_workspace(n));
   copy_matrix<double, double>(st, n, n, sm, lds, scopy, n);
   potrf_inverse(st, n, scopy, n, u, n, x, n, tmp, h.dinfo + 0);
   if (!mixed) {
