@@ -135,8 +135,9 @@ int xtbeig_dev_sgemm(xtbeig_handle_t handle, char transa, char transb, int m, in
 int xtbeig_dev_potrf_inverse(xtbeig_handle_t handle, int n, double *s, int lds, double *u, int ldu, double *x,
                              int ldx);
 /* Dense -> band (reference: Stages/BandForm.swift:28-142): a (full symmetric, destroyed) -> lower band of width b
- * in band storage ab (element (i, j), 0 <= i - j < 2b, at i + (2b - 1) j; xtbeig_band_elems(n, b) elements), the
- * panels' Householder vectors v1 (n x n) and T factors t1 (b x n). */
+ * in band storage ab (element (i, j), 0 <= i - j < 2b, at i + (2b - 1) j; xtbeig_band_elems(n, b) elements, which
+ * include b padding columns past n that the chase clears and uses), the panels' Householder vectors v1 (n x n) and
+ * T factors t1 (b x n). */
 long long xtbeig_band_elems(int n, int b);
 int xtbeig_dev_dreduce_to_band(xtbeig_handle_t handle, int n, int b, double *a, int lda, double *v1, double *t1,
                                double *ab);

@@ -217,6 +217,208 @@ __device__ inline bool chase_bulge_reg(int n, int b, const Band<T>& A, T* __rest
   return true;
 }
 
+// Householder scalars for x = (alpha, x_1 ..): beta, sq = sqrt(tau) and s2 = sqrt(tau) / (alpha - beta), so that
+// w = (sq, s2 x_1, ..) = sqrt(tau) v. With d = beta - alpha (|d| = |alpha| + |beta|, d beta > 0): tau = d / beta,
+// sqrt(tau) = |d| / sqrt(d beta) and sqrt(tau) / (alpha - beta) = -sign(d) / sqrt(d beta), one square root and one
+// reciprocal square root and no division. The hardware FP32 forms (1 ulp) run while every square stays normal; a
+// tail below that range takes the division form in FP64.
+__device__ inline float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double fast_sqrt(double x) { return sqrt(x); }
+__device__ inline double fast_rsqrt(double x) { return rsqrt(x); }
+template <typename T>
+__device__ inline void larfg_scalars(T alpha, T xn2, T& beta, T& sq, T& s2) {
+  constexpr T tiny = sizeof(T) == 4 ? T(1.17549435e-38f) : T(2.2250738585072014e-308);
+  if (xn2 >= tiny) {
+    beta = -copysign(fast_sqrt(alpha * alpha + xn2), alpha);
+    const T d = beta - alpha;
+    const T r = fast_rsqrt(d * beta);
+    sq = fabs(d) * r;
+    s2 = -copysign(r, d);
+  } else if (xn2 != T(0)) {
+    const double a = alpha, bd = -copysign(sqrt(a * a + (double)xn2), a);
+    const double sqd = sqrt((bd - a) / bd);
+    beta = T(bd);
+    sq = T(sqd);
+    s2 = T(sqd / (a - bd));
+  } else {
+    beta = alpha;
+    sq = T(0);
+    s2 = T(0);
+  }
+}
+
+// Sum of the two lanes l and l ^ B (the two halves of a column's rows).
+template <int B, typename T>
+__device__ inline T pair_sum(T v) {
+  if constexpr (B == 8) return v + dpp_mov<DPP_ROR8>(v);
+  else return v + __shfl_xor(v, B, 64);
+}
+
+// Lane-per-column form for b = B (8 or 16): H * B lanes carry one bulge. Lane (h, cl) owns column rs - b + cl of the
+// left block, row re + cl of the block below and row rs + cl of the diagonal block, over the U = B / H rows (columns)
+// u = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms and, for H = 2, one exchange with
+// the partner lane. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is a sum
+// over the B lanes of one half, and q is exchanged through LDS. One wavefront advances 64 / (H B) sweeps in lockstep,
+// each group on its own sweep and at its own bulge. The band is read and written as if the matrix continued past n
+// with zeros (band_elems(n, b) covers b extra columns, and the kernel clears every stored element outside the matrix
+// before the chase), so a bulge at the end of the band takes the same branch-free path as any other: the rows past n
+// give x = 0, w = 0 there and leave the zeros in place.
+template <typename T, int B, int H>
+__device__ __forceinline__ bool chase_bulge_lanes(int n, const Band<T>& A, T* __restrict__ v2, int ldv2, int s, int k,
+                                                  int cl, int h, T* __restrict__ qs) {
+  constexpr int U = B / H;
+  constexpr int LD = 2 * B - 1;
+  const int rs = s + 1 + k * B;
+  if (rs >= n) return false;
+  const int re = rs + B;
+  const int c = (k == 0) ? s : rs - B;
+  // left block: columns rs - B ..< rs; for the first bulge of a sweep only column s is nonzero (the columns before it
+  // are already tridiagonal), and every lane reads that column
+  const int lcol = (k == 0) ? s : c + cl;
+  const int u0 = h * U;
+  T* const xp = &A(rs, c);                // x = A[R, c] at xp[u]
+  T* const lp = &A(rs + u0, lcol);        // left block column at lp[j]
+  T* const bp = &A(re + cl, rs + u0);     // block-below row at bp[j LD]
+  T* const dr = &A(rs + cl, rs + u0);     // diagonal block row rs + cl, columns rs + u <= rs + cl, at dr[j LD]
+  T* const dc = &A(rs + u0, rs + cl);     // diagonal block column rs + cl, rows rs + u >= rs + cl, at dc[j]
+  T L[U], Bl[U], D[U], w[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    w[j] = xp[u0 + j];
+    L[j] = lp[j];
+    Bl[j] = bp[j * LD];
+    // both reads stay inside the band storage (for u > cl, dr[j LD] is another stored element)
+    const T r = dr[j * LD], q = dc[j];
+    D[j] = (u0 + j <= cl) ? r : q;
+  }
+  const T alpha = xp[0];
+  const T xc = xp[cl];  // this lane's element of x, for its own w
+  T xn2 = (u0 == 0) ? T(0) : w[0] * w[0];
+#pragma unroll
+  for (int j = 1; j < U; ++j) xn2 += w[j] * w[j];
+  if (H == 2) xn2 = pair_sum<B>(xn2);
+  T beta, sq, s2;
+  larfg_scalars(alpha, xn2, beta, sq, s2);  // tau = 0 gives w = 0: every update below leaves its block unchanged
+#pragma unroll
+  for (int j = 0; j < U; ++j) w[j] = (j == 0 && u0 == 0) ? sq : s2 * w[j];
+  const T wc = (cl == 0) ? sq : s2 * xc;
+  if (h == 0 && rs + cl < n) v2[(int64_t)(rs + cl) + (int64_t)ldv2 * s] = wc;
+  T dl = 0, y = 0, p = 0;
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    dl += w[j] * L[j];  // left block: w^T A[R, lcol]
+    y += Bl[j] * w[j];  // block below: A[re + cl, R] w
+    p += D[j] * w[j];   // diagonal block: (A_RR w)[cl]
+  }
+  if (H == 2) {
+    dl = pair_sum<B>(dl);
+    y = pair_sum<B>(y);
+    p = pair_sum<B>(p);
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    L[j] -= w[j] * dl;
+    Bl[j] -= y * w[j];
+  }
+  const T half = group_sum_lanes<B>(wc * p) * T(0.5);
+  const T qc = p - half * wc;
+  qs[cl] = qc;  // both halves store the same value
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int j = 0; j < U; ++j) D[j] -= wc * qs[u0 + j] + qc * w[j];
+  __builtin_amdgcn_wave_barrier();
+  if (k > 0 || cl == 0) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) lp[j] = (cl == 0) ? ((j == 0 && u0 == 0) ? beta : T(0)) : L[j];
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    bp[j * LD] = Bl[j];
+    if (u0 + j <= cl) dr[j * LD] = D[j];  // the lower triangle of the diagonal block, one lane per row
+  }
+  __threadfence_block();
+  return true;
+}
+
+// Band storage positions outside the matrix (rows >= n in the last columns' slots, the b extra columns) -> 0.
+template <typename T, int B>
+__device__ inline void clear_band_padding(int n, T* ab, int64_t nel) {
+  const int64_t first = (int64_t)2 * B * max(0, n - 2 * B);
+  for (int64_t q = first + threadIdx.x; q < nel; q += blockDim.x) {
+    const int64_t j = q / (2 * B), i = j + q % (2 * B);
+    if (i >= n) ab[q] = T(0);
+  }
+}
+
+template <int B, int H, bool LDS, typename T>
+inline size_t chase_lanes_lds(int n, int waves) {
+  const size_t groups = (size_t)waves * (64 / (H * B));
+  return sizeof(T) * (groups * B + (LDS ? (size_t)band_elems(n, B) : 0));
+}
+
+// dynamic LDS: the band (when LDS-resident) first, then one q slot of B elements per group
+template <typename T, int B, int H, bool LDS, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __restrict__ ab_global, T* __restrict__ v2,
+                                                                 int ldv2, T* __restrict__ d, T* __restrict__ e,
+                                                                 int* __restrict__ status) {
+  constexpr int GS = H * B;         // lanes per group
+  constexpr int SPW = 64 / GS;      // sweeps per wavefront
+  constexpr int NG = WAVES * SPW;   // sweep groups: group g takes sweeps g, g + NG, ...
+  extern __shared__ char smem_raw[];
+  __shared__ int prog[NG];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int gl = lane % GS, cl = gl % B, h = gl / B, g = wid * SPW + lane / GS;
+  const int64_t nel = band_elems(n, B);
+  T* const sm = reinterpret_cast<T*>(smem_raw);
+  T* const qs = sm + (LDS ? nel : 0) + g * B;  // this group's q exchange slot
+  T* ab = ab_global;
+  if (LDS) {
+    for (int64_t q = t; q < nel; q += blockDim.x) sm[q] = ab_global[q];
+    ab = sm;
+    __syncthreads();
+  }
+  clear_band_padding<T, B>(n, ab, nel);
+  if (t < NG) prog[t] = -1;
+  __syncthreads();
+  const Band<T> A{ab, band_ld(B)};
+  const int sweeps = max(0, n - 2);
+  const int KB = (n + B - 1) / B + 8;  // progress code: s * KB + bulges done (monotonic per group)
+  const int pg = (g + NG - 1) % NG;    // the group of sweep s - 1
+  bool failed = false;
+  int s = g, k = 0;
+  long spins = 0;
+  while (__any(s < sweeps)) {
+    bool ready = s < sweeps;
+    if (ready && s > 0) ready = ld_acquire(&prog[pg]) >= (s - 1) * KB + (k + 3);
+    if (ready) {
+      spins = 0;
+      if (chase_bulge_lanes<T, B, H>(n, A, v2, ldv2, s, k, cl, h, qs)) {
+        ++k;
+        if (gl == 0) st_release(&prog[g], s * KB + k);
+      } else {
+        if (gl == 0) st_release(&prog[g], s * KB + (KB - 1));
+        s += NG;
+        k = 0;
+      }
+    } else if (s < sweeps && ++spins > (1L << 24)) {
+      failed = true;
+      s = sweeps;
+    }
+    if (!__any(ready)) __builtin_amdgcn_s_sleep(1);
+  }
+  if (failed && gl == 0) atomicExch(status, 1);
+  __syncthreads();
+  for (int i = t; i < n; i += blockDim.x) {
+    d[i] = A(i, i);
+    if (i + 1 < n) e[i] = A(i + 1, i);
+  }
+  if (LDS) {
+    __syncthreads();
+    for (int64_t q = t; q < nel; q += blockDim.x) ab_global[q] = ab[q];
+  }
+}
+
 template <typename T, int B, bool LDS, int CHASE_WAVES>
 __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T* __restrict__ ab_global,
                                                                  T* __restrict__ v2, int ldv2, T* __restrict__ d,
@@ -294,11 +496,30 @@ void launch_chase(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* st
   XTB_LAUNCH_CHECK();
 }
 
+template <typename T, int B, int H>
+void launch_chase_lanes(hipStream_t s, int n, T* ab, T* v2, T* d, T* e, int* status) {
+  constexpr int WAVES = 8;
+  constexpr int NG = WAVES * (64 / (H * B));
+  const size_t lds = chase_lanes_lds<B, H, true, T>(n, WAVES);
+  if (lds + sizeof(int) * NG <= 160 * 1024) {
+    allow_lds(chase_lanes_kernel<T, B, H, true, WAVES>, lds);
+    hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, true, WAVES>), dim3(1), dim3(64 * WAVES), lds, s, n, ab, v2, n, d,
+                       e, status);
+  } else {
+    const size_t qlds = chase_lanes_lds<B, H, false, T>(n, WAVES);
+    hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, false, WAVES>), dim3(1), dim3(64 * WAVES), qlds, s, n, ab, v2, n,
+                       d, e, status);
+  }
+  XTB_LAUNCH_CHECK();
+}
+
 template <typename T>
 void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* status) {
   if (b > 64) throw std::runtime_error("chase_bulges: band width must be <= 64");
-  // register-resident bulges for b <= 32 (one template per power-of-two bound), the scalar form beyond
-  if (b <= 8) launch_chase<T, 8>(s, n, b, ab, v2, d, e, status);
+  // lane-per-column bulges for b <= 16, register-resident wavefront bulges for b <= 32, the scalar form beyond
+  if (b == 8 && n > 2) launch_chase_lanes<T, 8, 2>(s, n, ab, v2, d, e, status);
+  else if (b == 16 && n > 2) launch_chase_lanes<T, 16, 2>(s, n, ab, v2, d, e, status);
+  else if (b <= 8) launch_chase<T, 8>(s, n, b, ab, v2, d, e, status);
   else if (b <= 16) launch_chase<T, 16>(s, n, b, ab, v2, d, e, status);
   else if (b <= 32) launch_chase<T, 32>(s, n, b, ab, v2, d, e, status);
   else launch_chase<T, 0>(s, n, b, ab, v2, d, e, status);

@@ -56,6 +56,7 @@ constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1, 0, 3, 2]
 constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2, 3, 0, 1]
 constexpr int DPP_HALF_MIRROR = 0x141;  // lane i <-> 7 - i within 8
 constexpr int DPP_MIRROR = 0x140;       // lane i <-> 15 - i within 16
+constexpr int DPP_ROR8 = 0x128;         // lane i <-> i ^ 8 within 16 (row rotate by 8)
 
 // Sum over aligned groups of G lanes (G a power of two <= 64); every lane of a group receives its group's total.
 // The mirror steps pair whole sub-groups once those are uniform.

@@ -50,9 +50,10 @@ template <typename T>
 void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T* ab, T* work);
 
 // chase.hip --------------------------------------------------------------------------------------------------------
-// Band storage of the chase: element (i, j), 0 <= i - j <= 2b - 1, at i + band_ld(b) j.
+// Band storage of the chase: element (i, j), 0 <= i - j <= 2b - 1, at i + band_ld(b) j, for columns j < n + b (the
+// b columns past the matrix, and the rows past n in every column's slot, are zero padding the chase may read).
 __host__ __device__ inline int band_ld(int b) { return b > 1 ? 2 * b - 1 : 1; }
-__host__ __device__ inline int64_t band_elems(int n, int b) { return (int64_t)band_ld(b) * n + n + 1; }
+__host__ __device__ inline int64_t band_elems(int n, int b) { return (int64_t)band_ld(b) * (n + b) + (n + b) + 1; }
 // Band (ab, leading dimension 2b, (2b + 1) n elements) -> tridiagonal d, e; reflectors w into v2 (n x n, column s
 // holds sweep s, rows rs ..< re).
 template <typename T>
