@@ -60,7 +60,8 @@ def test_potrf_inverse(handle, torch_cuda):
     assert np.max(np.abs(x @ u - np.eye(n))) < 1e-12
 
 
-@pytest.mark.parametrize("prec,b,n", [("d", 32, 260), ("d", 8, 131), ("s", 16, 200), ("d", 32, 33), ("d", 4, 9)])
+@pytest.mark.parametrize("prec,b,n", [("d", 32, 260), ("d", 8, 131), ("s", 16, 200), ("d", 32, 33), ("d", 4, 9),
+                                      ("d", 16, 700), ("s", 8, 1300), ("d", 8, 1100), ("d", 16, 1700)])
 def test_reduce_to_band(handle, torch_cuda, prec, b, n):
     torch = torch_cuda
     a, lam = workloads.standard_matrix(n, 11)

@@ -15,6 +15,7 @@
 // After the last panel the lower band is copied into the chase's band storage (element (i, j), 0 <= i - j <= 2b - 1,
 // at i + (2b - 1) j; see chase.hip).
 #include "common.hpp"
+#include "gemm_tile.hpp"
 #include "kernels.hpp"
 
 namespace xtbeig {
@@ -158,6 +159,401 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
       }
     __syncthreads();  // v1 and tau of this sub-panel visible to the next one
   }
+}
+
+// Butterfly reduce-scatter of NB values per lane (NB = 8 or 16) over the 64 lanes of a wavefront: each exchange
+// step halves the values a lane keeps (the half selected by one lane bit) and adds the partner's copy of that half;
+// the last steps sum the one remaining value over the lanes that hold the same index. Lane l returns the wavefront
+// total of value index (l >> 2) & 15 (NB = 16) or (l >> 3) & 7 (NB = 8). Pairings: lane bits 5 and 4 by xor, bit 3
+// by rotating a row of 16 by 8, bit 2 by the mirror within 8 lanes (it also flips bits 1 and 0, which are not yet
+// decided), bits 1 and 0 by quad permutes.
+template <typename T, int H>
+__device__ inline void bfly_split(const T (&v)[2 * H], bool up, T (&out)[H], int step) {
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const T send = up ? v[j] : v[j + H];
+    T recv;
+    if (step == 32) recv = xor_lanes<32>(send);
+    else if (step == 16) recv = xor_lanes<16>(send);
+    else if (step == 8) recv = dpp_mov<DPP_ROR8>(send);
+    else recv = dpp_mov<DPP_HALF_MIRROR>(send);
+    out[j] = (up ? v[j + H] : v[j]) + recv;
+  }
+}
+
+template <typename T, int NB>
+__device__ inline T bfly_reduce(const T (&v)[NB]) {
+  const int lane = threadIdx.x & 63;
+  static_assert(NB == 8 || NB == 16, "bfly_reduce: NB must be 8 or 16");
+  T a[NB / 2], b[NB / 4], c[NB / 8];
+  bfly_split<T, NB / 2>(v, lane & 32, a, 32);
+  bfly_split<T, NB / 4>(a, lane & 16, b, 16);
+  bfly_split<T, NB / 8>(b, lane & 8, c, 8);
+  T r;
+  if constexpr (NB == 16) {
+    T d[1];
+    bfly_split<T, 1>(c, lane & 4, d, 4);
+    r = d[0];
+  } else {
+    r = c[0];
+    r += dpp_mov<DPP_HALF_MIRROR>(r);
+  }
+  r += dpp_mov<DPP_XOR2>(r);
+  r += dpp_mov<DPP_XOR1>(r);
+  return r;
+}
+
+// Householder QR of the m x NB panel p (ld lda; NB = 8 or 16, m <= 256 R) by one workgroup of 256 threads, with the
+// panel's block-reflector data: row i = t + 256 r lives in registers of thread t. The column loop is a real loop
+// (the body is the same for every column, so the kernel's code stays small enough for the instruction cache): the
+// register array rotates by one column per step, so that the current column is always at position 0, the columns
+// still to be reduced follow it and the finished reflectors v_0 .. v_{c-1} sit at the end. Column c takes one block
+// reduction of NB sums over the rows i > c of x_c[i] times position j: at j = 0 the reflector's tail norm, at an
+// unfinished column's position the part of v^T x_j below row c (row c itself comes through LDS), at v_l's position
+// sum_{i > c} v_l[i] x_c[i], so that G(l, c) = v_l^T v_c = v_l[c] + scal_c * that sum comes with the same reduction.
+// T (I - V T V^T = H_1 ... H_k, forward columnwise: T(0:c, c) = -tau_c T(0:c, 0:c) G(0:c, c), T(c, c) = tau_c) is
+// built in LDS one column per step. Outputs: R over p's upper triangle (the part below it is left as it was: only the
+// band, diagonals 0 .. b, is read afterwards), V (unit diagonal, zeros above) into v1 (ld ldv1), tau, and T into tt
+// (ld ldt, zeros below).
+template <typename T, int NB, int R>
+__global__ __launch_bounds__(256) void panel_factor_kernel(int m, T* __restrict__ p, int lda, T* __restrict__ v1,
+                                                           int ldv1, T* __restrict__ tau, T* __restrict__ tt,
+                                                           int ldt) {
+  constexpr int WAVES = 4, SPREAD = 64 / NB;  // lanes holding each reduced value
+  __shared__ T red[2][WAVES][NB];
+  __shared__ T rowb[2][NB];
+  __shared__ T Ts[NB][NB + 1];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int k = min(m, NB);
+  // T's strictly lower part is read (times a zero mask) by the branch-free T update: it must hold zeros, not whatever
+  // an earlier kernel left in LDS (0 * NaN is NaN). The first column's barrier orders these stores before any read.
+  for (int e = t; e < NB * (NB + 1); e += 256) (&Ts[0][0])[e] = T(0);
+  T x[R][NB];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = t + 256 * r;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[r][c] = (i < m) ? p[idx(i, c, lda)] : T(0);
+  }
+  // unrolled by 4: the rotation's register moves are needed only where the unrolled body loops back
+#pragma unroll 4
+  for (int c = 0; c < NB; ++c) {
+    T newcol[R];
+    if (c < k) {
+      const int par = c & 1;
+      T part[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) part[j] = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T xc = (r > 0 || t > c) ? x[r][0] : T(0);  // rows past m hold zeros
+#pragma unroll
+        for (int j = 0; j < NB; ++j) part[j] += xc * x[r][j];
+      }
+      if (t == c) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) rowb[par][j] = x[0][j];
+      }
+      const T wsum = bfly_reduce<T, NB>(part);
+      if (lane % SPREAD == 0) red[par][wid][lane / SPREAD] = wsum;
+      __syncthreads();
+      T tot[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) tot[j] = (red[par][0][j] + red[par][1][j]) + (red[par][2][j] + red[par][3][j]);
+      const T alpha = rowb[par][0], xn2 = tot[0];
+      T beta = alpha, tq = 0, scal = 0;
+      if (xn2 != T(0)) {
+        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+        tq = (beta - alpha) / beta;
+        scal = T(1) / (alpha - beta);
+      }
+      // g[j] = v_l^T v_c at a finished column's position j = NB - c + l, v^T x_j / ... at an unfinished one's
+      T g[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) g[j] = rowb[par][j] + scal * tot[j];
+      {
+        // T(t, c) = -tau_c sum_{l = t}^{c - 1} T(t, l) G(l, c), branch-free so that the LDS reads overlap
+        const int tr = min(t, NB - 1);
+        T sum = 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          const int l = j - NB + c;
+          const T gm = (l >= t) ? g[j] : T(0);  // the mask on g keeps the LDS read unconditional
+          sum += Ts[tr][max(l, 0)] * gm;
+        }
+        if (t < c) Ts[t][c] = -tq * sum;
+      }
+      if (t == c) {
+        Ts[c][c] = tq;
+        tau[c] = tq;
+      }
+      T f[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) f[j] = (j >= 1 && j < NB - c) ? tq * g[j] : T(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r == 0 && t == c) {
+#pragma unroll
+          for (int j = 1; j < NB; ++j) x[0][j] -= f[j];
+          newcol[0] = beta;
+        } else if (r > 0 || t > c) {
+          const T vi = x[r][0] * scal;
+#pragma unroll
+          for (int j = 1; j < NB; ++j) x[r][j] -= vi * f[j];
+          newcol[r] = vi;
+        } else {
+          newcol[r] = x[r][0];  // R(i, c) above the diagonal
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; ++r) newcol[r] = x[r][0];  // columns past k (m < NB) are not reflected
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+#pragma unroll
+      for (int j = 0; j + 1 < NB; ++j) x[r][j] = x[r][j + 1];
+      x[r][NB - 1] = newcol[r];
+    }
+  }
+  __syncthreads();  // T complete
+  if (t < k * k) {
+    const int r = t % k, c = t / k;
+    tt[r + (int64_t)c * ldt] = (r <= c) ? Ts[r][c] : T(0);
+  }
+  if (t < NB && t < m) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (t <= c) p[idx(t, c, lda)] = x[0][c];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = t + 256 * r;
+    if (i >= m) continue;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c < k) v1[idx(i, c, ldv1)] = (i > c) ? x[r][c] : (i == c ? T(1) : T(0));
+  }
+}
+
+// X = A2 V T for the panel's trailing block A2 (m x m, ld lda) and its reflectors V (m x k, ld ldv), in two launches
+// without any inter-workgroup synchronization: panel_xpart_kernel forms the split-K partials A2(rows, K split)
+// V(K split, :) on the matrix cores (grid: row blocks of 64 x K splits of XKC; A2's block and V's rows are staged in
+// LDS with all loads in flight at once; the product is taken as X^T = V^T A2^T so that lanes run along rows), and
+// panel_xfin_kernel (one workgroup per row block) sums them in split order, applies T, stores X and the block's
+// part of V^T X. Y = (V T)^T X = T^T (V^T X) is completed by the update kernel.
+template <typename T> struct XSplit;
+template <> struct XSplit<float> { static constexpr int K = 128; };
+template <> struct XSplit<double> { static constexpr int K = 64; };
+
+template <typename T>
+__global__ __launch_bounds__(256) void panel_xpart_kernel(int m, int k, const T* __restrict__ a2, int lda,
+                                                          const T* __restrict__ v, int ldv, T* __restrict__ xpart) {
+  using namespace gemm_detail;
+  using acc_t = typename Mfma<T>::acc_t;
+  constexpr int KC = XSplit<T>::K;
+  __shared__ T As[KC][64 + 1];
+  __shared__ T Vs[KC][16 + 1];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int i0 = blockIdx.x * 64, ks = blockIdx.y, l0 = ks * KC, kl = min(KC, m - l0);
+  constexpr int NA = 64 * KC / 256, NV = 16 * KC / 256;
+  T ra[NA], rv[NV];  // every load in flight before the first LDS store
+#pragma unroll
+  for (int q = 0; q < NA; ++q) {
+    const int e = t + 256 * q, i = e % 64, kk = e / 64;
+    ra[q] = (i0 + i < m && kk < kl) ? a2[idx(i0 + i, l0 + kk, lda)] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const int e = t + 256 * q, kk = e % KC, c = e / KC;
+    rv[q] = (kk < kl && c < k) ? v[idx(l0 + kk, c, ldv)] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < NA; ++q) {
+    const int e = t + 256 * q;
+    As[e / 64][e % 64] = ra[q];
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const int e = t + 256 * q;
+    Vs[e % KC][e / KC] = rv[q];
+  }
+  __syncthreads();
+  acc_t acc = {0, 0, 0, 0};
+  const int il = wid * 16 + (lane & 15);
+#pragma unroll 8
+  for (int s0 = 0; s0 < KC; s0 += 4) {
+    const int kk = s0 + (lane >> 4);
+    acc = Mfma<T>::op(Vs[kk][lane & 15], As[kk][il], acc);  // A(c, kk) = V(kk, c), B(kk, i) = A2(i, kk)
+  }
+  const int i = i0 + il;
+  T* xp = xpart + (size_t)ks * m * k;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int c = Mfma<T>::row(lane, r);
+    if (i < m && c < k) xp[i + (size_t)c * m] = acc[r];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void panel_xfin_kernel(int m, int k, const T* __restrict__ xpart, int nks,
+                                                         const T* __restrict__ tt, int ldt, const T* __restrict__ v,
+                                                         int ldv, T* __restrict__ x, T* __restrict__ ypart) {
+  __shared__ T Xs[64][17];
+  __shared__ T Vr[64][17];
+  __shared__ T Ts[16][17];
+  const int t = threadIdx.x, rb = blockIdx.x, i0 = rb * 64;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q, r = e % 64, c = e / 64;
+    const bool ok = i0 + r < m && c < k;
+    T sum = 0;
+    if (ok) {
+#pragma unroll 4
+      for (int s2 = 0; s2 < nks; ++s2) sum += xpart[(size_t)s2 * m * k + (i0 + r) + (size_t)c * m];
+    }
+    Xs[r][c] = sum;
+    Vr[r][c] = ok ? v[idx(i0 + r, c, ldv)] : T(0);
+  }
+  {
+    const int r = t % 16, c = t / 16;
+    Ts[r][c] = (r < k && c < k && r <= c) ? tt[r + (size_t)c * ldt] : T(0);
+  }
+  __syncthreads();
+  T xv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q, r = e % 64, c = e / 64;
+    T sum = 0;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) sum += Xs[r][l] * Ts[l][c];
+    xv[q] = sum;
+    if (i0 + r < m && c < k) x[(i0 + r) + (size_t)c * m] = sum;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q;
+    Xs[e % 64][e / 64] = xv[q];
+  }
+  __syncthreads();
+  const int a = t % 16, c = t / 16;
+  T sum = 0;
+#pragma unroll 8
+  for (int r = 0; r < 64; ++r) sum += Vr[r][a] * Xs[r][c];
+  ypart[(size_t)rb * 256 + t] = sum;  // (V^T X)(a, c) over this block's rows
+}
+
+// A2 -= V W^T + W V^T with W = X - V Y / 2 and Y = T^T (V^T X) (the row blocks' parts of V^T X summed in block
+// order), one 64 x 64 tile of A2 per workgroup on the matrix cores (computed as its transpose, so that lanes run
+// along A2's rows). The tile's rows I and columns J stage [V(I) | W(I)] and [W(J) | V(J)] (zero past k) in LDS;
+// K = 32.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_update_kernel(int m, int k, T* __restrict__ a2, int lda,
+                                                           const T* __restrict__ v, int ldv, const T* __restrict__ x,
+                                                           const T* __restrict__ ypart, int nrb,
+                                                           const T* __restrict__ tt, int ldt) {
+  using namespace gemm_detail;
+  using acc_t = typename Mfma<T>::acc_t;
+  __shared__ T Ai[64][33];
+  __shared__ T Bj[64][33];
+  __shared__ T Ys[16][17];
+  __shared__ T Ss[16][17];
+  __shared__ T Ts[16][17];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int I0 = blockIdx.x * 64, J0 = blockIdx.y * 64;
+  const int wm = wid & 1, wn = wid >> 1;
+  // this thread's 16 elements of the A2 tile, in the accumulator layout, loaded first so that their latency overlaps
+  // everything below
+  T old[2][2][4];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int i = I0 + wn * 32 + bj * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = J0 + wm * 32 + bi * 16 + Mfma<T>::row(lane, r);
+        old[bi][bj][r] = (i < m && j < m) ? a2[idx(i, j, lda)] : T(0);
+      }
+    }
+  {
+    const int r = t % 16, c = t / 16;
+    T sum = 0;
+#pragma unroll 8
+    for (int q = 0; q < nrb; ++q) sum += ypart[(size_t)q * 256 + t];
+    Ss[r][c] = sum;
+    Ts[r][c] = (r < k && c < k && r <= c) ? tt[r + (size_t)c * ldt] : T(0);
+  }
+  __syncthreads();
+  {
+    const int r = t % 16, c = t / 16;
+    T sum = 0;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) sum += Ts[l][r] * Ss[l][c];
+    Ys[r][c] = (r < k && c < k) ? sum : T(0);
+  }
+  for (int e = t; e < 64 * 16; e += 256) {
+    const int r = e % 64, c = e / 64;
+    const bool oi = I0 + r < m && c < k, oj = J0 + r < m && c < k;
+    Ai[r][c] = oi ? v[idx(I0 + r, c, ldv)] : T(0);
+    Ai[r][16 + c] = oi ? x[(I0 + r) + (size_t)c * m] : T(0);
+    Bj[r][16 + c] = oj ? v[idx(J0 + r, c, ldv)] : T(0);
+    Bj[r][c] = oj ? x[(J0 + r) + (size_t)c * m] : T(0);
+  }
+  __syncthreads();
+  T wi[4], wj[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q, r = e % 64, c = e / 64;
+    T si = 0, sj = 0;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      si += Ai[r][l] * Ys[l][c];
+      sj += Bj[r][16 + l] * Ys[l][c];
+    }
+    wi[q] = Ai[r][16 + c] - T(0.5) * si;
+    wj[q] = Bj[r][c] - T(0.5) * sj;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q, r = e % 64, c = e / 64;
+    Ai[r][16 + c] = wi[q];
+    Bj[r][c] = wj[q];
+  }
+  __syncthreads();
+  // C^T(j, i) = sum_kk Bj[j][kk] Ai[i][kk]: wavefront (wm, wn) takes j in 32 wm .., i in 32 wn ..
+  acc_t acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int kk = 0; kk < 32; kk += 4) {
+    const int kr = kk + (lane >> 4), c = lane & 15;
+    T av[2], bv[2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) av[bi] = Bj[wm * 32 + bi * 16 + c][kr];
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) bv[bj] = Ai[wn * 32 + bj * 16 + c][kr];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = Mfma<T>::op(av[bi], bv[bj], acc[bi][bj]);
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int i = I0 + wn * 32 + bj * 16 + (lane & 15);
+      if (i >= m) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = J0 + wm * 32 + bi * 16 + Mfma<T>::row(lane, r);
+        if (j < m) a2[idx(i, j, lda)] = old[bi][bj][r] - acc[bi][bj][r];
+      }
+    }
 }
 
 // The per-panel reductions over the m rows run on cdiv(m, PR) workgroups, PR rows each; k x k partial products per
@@ -334,10 +730,56 @@ constexpr int SPLIT_ROWS = 128;  // K per split of the X = A2 VT product
 
 template <typename T>
 size_t band_workspace(int n, int b) {
-  // tau (b) | buf [V W V] (3 n b) | VT (n b) | split-K partials of X (ceil(n / SPLIT_ROWS) n b) | k x k partials
-  // of the row blocks (ceil(n / PR) 1024)
-  return (size_t)b + 4 * (size_t)n * b + (size_t)((n + SPLIT_ROWS - 1) / SPLIT_ROWS) * n * b +
-         (size_t)((n + PR - 1) / PR) * 1024 + 64;
+  // tau (b) | buf [V W V] (3 n b) | VT (n b) | split-K partials of X (ceil(n / 64) n b: the fused path splits K by
+  // 64 (FP64) or 128 (FP32), the general path by SPLIT_ROWS) | k x k partials of the row blocks (ceil(n / PR) 1024)
+  return (size_t)b + 4 * (size_t)n * b + (size_t)((n + 63) / 64) * n * b + (size_t)((n + PR - 1) / PR) * 1024 + 64;
+}
+
+template <typename T, int NB, int R>
+void launch_factor(hipStream_t s, int m, T* p, int lda, T* v1, int ldv1, T* tau, T* tt, int ldt) {
+  hipLaunchKernelGGL((panel_factor_kernel<T, NB, R>), dim3(1), dim3(256), 0, s, m, p, lda, v1, ldv1, tau, tt, ldt);
+  XTB_LAUNCH_CHECK();
+}
+
+// Fused panel factorization (QR, T and VT in one workgroup) for panel widths 8 and 16 and m <= 2048; false when the
+// panel needs the general path.
+template <typename T>
+bool panel_factor(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, T* tau, T* tt, int ldt) {
+  const int r = (m + 255) / 256;
+  if (nb == 8) {
+    switch (r) {
+      case 0: case 1: launch_factor<T, 8, 1>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 2: launch_factor<T, 8, 2>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 3: launch_factor<T, 8, 3>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 4: launch_factor<T, 8, 4>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 5: launch_factor<T, 8, 5>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 6: launch_factor<T, 8, 6>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 7: launch_factor<T, 8, 7>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 8: launch_factor<T, 8, 8>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      default: return false;
+    }
+  }
+  if (nb == 16) {
+    switch (r) {
+      case 0: case 1: launch_factor<T, 16, 1>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 2: launch_factor<T, 16, 2>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 3: launch_factor<T, 16, 3>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 4: launch_factor<T, 16, 4>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 5: launch_factor<T, 16, 5>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      case 6: launch_factor<T, 16, 6>(s, m, p, lda, v1, ldv1, tau, tt, ldt); return true;
+      // FP64 with more than 6 rows per thread: the register array no longer fits; the general path takes it
+      case 7:
+        if constexpr (sizeof(T) == 8) return false;
+        else launch_factor<T, 16, 7>(s, m, p, lda, v1, ldv1, tau, tt, ldt);
+        return true;
+      case 8:
+        if constexpr (sizeof(T) == 8) return false;
+        else launch_factor<T, 16, 8>(s, m, p, lda, v1, ldv1, tau, tt, ldt);
+        return true;
+      default: return false;
+    }
+  }
+  return false;
 }
 
 template <typename T>
@@ -347,23 +789,39 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
   T* buf = tau + b;
   T* vt = buf + 3 * (size_t)n * b;
   T* xpart = vt + (size_t)n * b;
-  T* kpart = xpart + (size_t)((n + SPLIT_ROWS - 1) / SPLIT_ROWS) * n * b;
+  T* kpart = xpart + (size_t)((n + 63) / 64) * n * b;
   // v1 is zero outside the panels' V, so that consecutive panels read as one block [V_a V_b ...] from row r_a
   set_zero<T>(s, n, n, v1, n);
   for (int j = 0; j < n - b - 1; j += b) {
     const int r = j + b, m = n - r, k = std::min(b, m);
     T* panel = a + idx(r, j, lda);
     T* v = v1 + idx(r, j, n);
-    launch_panel_qr<T>(s, m, b, panel, lda, v, n, buf, tau);
     T* tt = t1 + idx(0, j, b);
     T* a2 = a + idx(r, r, lda);
     T* x = buf + (size_t)m * k;  // slot 1 of [V | W | V]
     const int nblk = cdiv(m, PR);
-    hipLaunchKernelGGL(panel_gram_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, kpart);      // V^T V parts
-    XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(panel_tvt_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, tau, kpart, nblk, tt, b,
-                       vt);  // T, VT = V T
-    XTB_LAUNCH_CHECK();
+    if (panel_factor<T>(s, m, b, panel, lda, v, n, tau, tt, b)) {
+      // X = A2 V T (split-K partials, then their sum with T applied and the row blocks' parts of V^T X), then
+      // A2 -= V W^T + W V^T with Y and W formed per tile
+      T* xb = buf;
+      const int nks = cdiv(m, XSplit<T>::K), nrb = cdiv(m, 64);
+      hipLaunchKernelGGL(panel_xpart_kernel<T>, dim3(nrb, nks), dim3(256), 0, s, m, k, a2, lda, v, n, xpart);
+      XTB_LAUNCH_CHECK();
+      hipLaunchKernelGGL(panel_xfin_kernel<T>, dim3(nrb), dim3(256), 0, s, m, k, xpart, nks, tt, b, v, n, xb, kpart);
+      XTB_LAUNCH_CHECK();
+      hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb, nrb), dim3(256), 0, s, m, k, a2, lda, v, n, xb, kpart, nrb,
+                         tt, b);
+      XTB_LAUNCH_CHECK();
+      continue;
+    }
+    {
+      launch_panel_qr<T>(s, m, b, panel, lda, v, n, buf, tau);
+      hipLaunchKernelGGL(panel_gram_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, kpart);  // V^T V parts
+      XTB_LAUNCH_CHECK();
+      hipLaunchKernelGGL(panel_tvt_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, tau, kpart, nblk, tt, b,
+                         vt);  // T, VT = V T
+      XTB_LAUNCH_CHECK();
+    }
     const int nsplit = gemm_splitk<T>(s, m, k, m, a2, lda, vt, m, xpart, SPLIT_ROWS);  // A2 VT in parts
     hipLaunchKernelGGL(panel_xy_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, vt, xpart, nsplit, x, kpart);  // X, Y
     XTB_LAUNCH_CHECK();

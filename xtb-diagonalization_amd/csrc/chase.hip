@@ -248,17 +248,21 @@ __device__ inline void larfg_scalars(T alpha, T xn2, T& beta, T& sq, T& s2) {
   }
 }
 
-// Sum of the two lanes l and l ^ B (the two halves of a column's rows).
-template <int B, typename T>
-__device__ inline T pair_sum(T v) {
-  if constexpr (B == 8) return v + dpp_mov<DPP_ROR8>(v);
-  else return v + __shfl_xor(v, B, 64);
+// Sum over the H lanes l ^ (m B), m < H (the parts of a column's rows held by one group).
+template <int B, int H, typename T>
+__device__ inline T part_sum(T v) {
+  if constexpr (H >= 2) {
+    if constexpr (B == 8) v += dpp_mov<DPP_ROR8>(v);
+    else v += __shfl_xor(v, B, 64);
+  }
+  if constexpr (H >= 4) v += __shfl_xor(v, 2 * B, 64);
+  return v;
 }
 
 // Lane-per-column form for b = B (8 or 16): H * B lanes carry one bulge. Lane (h, cl) owns column rs - b + cl of the
 // left block, row re + cl of the block below and row rs + cl of the diagonal block, over the U = B / H rows (columns)
-// u = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms and, for H = 2, one exchange with
-// the partner lane. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is a sum
+// u = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms followed, for H > 1, by a sum
+// over the group's H parts. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is a sum
 // over the B lanes of one half, and q is exchanged through LDS. One wavefront advances 64 / (H B) sweeps in lockstep,
 // each group on its own sweep and at its own bulge. The band is read and written as if the matrix continued past n
 // with zeros (band_elems(n, b) covers b extra columns, and the kernel clears every stored element outside the matrix
@@ -297,7 +301,7 @@ __device__ __forceinline__ bool chase_bulge_lanes(int n, const Band<T>& A, T* __
   T xn2 = (u0 == 0) ? T(0) : w[0] * w[0];
 #pragma unroll
   for (int j = 1; j < U; ++j) xn2 += w[j] * w[j];
-  if (H == 2) xn2 = pair_sum<B>(xn2);
+  if (H > 1) xn2 = part_sum<B, H>(xn2);
   T beta, sq, s2;
   larfg_scalars(alpha, xn2, beta, sq, s2);  // tau = 0 gives w = 0: every update below leaves its block unchanged
 #pragma unroll
@@ -311,10 +315,10 @@ __device__ __forceinline__ bool chase_bulge_lanes(int n, const Band<T>& A, T* __
     y += Bl[j] * w[j];  // block below: A[re + cl, R] w
     p += D[j] * w[j];   // diagonal block: (A_RR w)[cl]
   }
-  if (H == 2) {
-    dl = pair_sum<B>(dl);
-    y = pair_sum<B>(y);
-    p = pair_sum<B>(p);
+  if (H > 1) {
+    dl = part_sum<B, H>(dl);
+    y = part_sum<B, H>(y);
+    p = part_sum<B, H>(p);
   }
 #pragma unroll
   for (int j = 0; j < U; ++j) {
@@ -323,7 +327,7 @@ __device__ __forceinline__ bool chase_bulge_lanes(int n, const Band<T>& A, T* __
   }
   const T half = group_sum_lanes<B>(wc * p) * T(0.5);
   const T qc = p - half * wc;
-  qs[cl] = qc;  // both halves store the same value
+  qs[cl] = qc;  // every part stores the same value
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int j = 0; j < U; ++j) D[j] -= wc * qs[u0 + j] + qc * w[j];
@@ -496,9 +500,8 @@ void launch_chase(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* st
   XTB_LAUNCH_CHECK();
 }
 
-template <typename T, int B, int H>
+template <typename T, int B, int H, int WAVES = 8>
 void launch_chase_lanes(hipStream_t s, int n, T* ab, T* v2, T* d, T* e, int* status) {
-  constexpr int WAVES = 8;
   constexpr int NG = WAVES * (64 / (H * B));
   const size_t lds = chase_lanes_lds<B, H, true, T>(n, WAVES);
   if (lds + sizeof(int) * NG <= 160 * 1024) {

@@ -58,6 +58,33 @@ constexpr int DPP_HALF_MIRROR = 0x141;  // lane i <-> 7 - i within 8
 constexpr int DPP_MIRROR = 0x140;       // lane i <-> 15 - i within 16
 constexpr int DPP_ROR8 = 0x128;         // lane i <-> i ^ 8 within 16 (row rotate by 8)
 
+// The value of lane l ^ D (D = 16 or 32) through the gfx950 row and half swaps: v_permlane16_swap exchanges rows 1
+// and 3 of its first operand with rows 0 and 2 of its second, v_permlane32_swap the upper half of the first with the
+// lower half of the second; with both operands the same value, a lane finds its partner's value in the result that
+// took the other row (half).
+template <int D>
+__device__ inline unsigned xor_lanes_u32(unsigned v) {
+  static_assert(D == 16 || D == 32, "xor_lanes: D must be 16 or 32");
+  const int lane = threadIdx.x & 63;
+  if constexpr (D == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? r[0] : r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? r[0] : r[1];
+  }
+}
+template <int D>
+__device__ inline float xor_lanes(float v) {
+  return __uint_as_float(xor_lanes_u32<D>(__float_as_uint(v)));
+}
+template <int D>
+__device__ inline double xor_lanes(double v) {
+  const unsigned lo = xor_lanes_u32<D>((unsigned)__double2loint(v));
+  const unsigned hi = xor_lanes_u32<D>((unsigned)__double2hiint(v));
+  return __hiloint2double((int)hi, (int)lo);
+}
+
 // Sum over aligned groups of G lanes (G a power of two <= 64); every lane of a group receives its group's total.
 // The mirror steps pair whole sub-groups once those are uniform.
 template <int G, typename T>

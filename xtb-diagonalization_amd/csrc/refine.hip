@@ -7,7 +7,12 @@
 // The four products and the update run as FP64 GEMMs on the matrix cores. A pair is treated as a cluster (E_ij =
 // R_ij / 2: its subspace converges, the basis inside it is not rotated) when its Newton correction would not be
 // small, |E_ij| > 1/4, which covers exactly and nearly degenerate pairs (the gap below what the current accuracy
-// resolves); as the residual shrinks, near-degenerate pairs separate.
+// resolves); as the residual shrinks, near-degenerate pairs separate. The diagonal carries the second-order terms of
+// the new Gram matrix, diag((I + E)^T (I - R) (I + E)) = 1 - R_ii + 2 E_ii + sum_k E_ki^2 - 2 sum_k R_ik E_ki + O(3):
+//     E_ii = (R_ii - sum_{k != i} E_ki^2 + 2 sum_{k != i} R_ik E_ki) / 2,
+// so that a column rotated by angle theta within a close pair keeps its S-norm to O(theta^3) instead of O(theta^2)
+// (with E_ii = R_ii / 2 alone, two steps from FP32 leave the norms of such pairs off by theta^4, about 1e-9 for a gap
+// of 2e-5).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -34,6 +39,25 @@ __global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, c
   double v = (a[idx(i, j, lda)] + lam[j] * rij) / gap;
   if (i == j || !(fabs(v) <= 0.25)) v = 0.5 * rij;
   e[idx(i, j, lde)] = v;
+}
+
+// E_ii from column i of E and R (one workgroup per column; R symmetric).
+__global__ __launch_bounds__(256) void oa_diag_kernel(int n, const double* __restrict__ r, int ldr,
+                                                      double* __restrict__ e, int lde) {
+  __shared__ double red[16];
+  const int i = blockIdx.x, t = threadIdx.x;
+  const double* rc = r + (int64_t)i * ldr;
+  const double* ec = e + (int64_t)i * lde;
+  double s1 = 0, s2 = 0;
+  for (int k = t; k < n; k += blockDim.x) {
+    if (k == i) continue;
+    const double ek = ec[k];
+    s1 += ek * ek;
+    s2 += rc[k] * ek;
+  }
+  s1 = block_sum(s1, red);
+  s2 = block_sum(s2, red);
+  if (t == 0) e[idx(i, i, lde)] = 0.5 * (rc[i] - s1 + 2.0 * s2);
 }
 
 // rank of lambda_i in ascending order (ties by index)
@@ -88,6 +112,8 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
     hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam);
     XTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(oa_diag_kernel, dim3(n), dim3(256), 0, s, n, r, n, e, n);
     XTB_LAUNCH_CHECK();
     copy_matrix<double, double>(s, n, n, c, ldc, cnew, n);
     gemm<double>(s, 'N', 'N', n, n, n, 1.0, c, ldc, e, n, 1.0, cnew, n);    // C + C E
