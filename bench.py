@@ -26,10 +26,10 @@ SEED = 1001  # configuration 1
 MI355X_FP64_TFLOPS = 78.6
 MI355X_FP32_TFLOPS = 157.3
 MI355X_HBM_GBS = 8000.0
-# HBM bytes per chase launch (FP32, n = 1204, b = 16) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this command,
-# each scaled by the same counter's reading for a 256 MiB copy (--pmc-calibration): fetch 81.19 KiB x 2.00, write
-# 3014.69 KiB x 1.00.
-TRAFFIC_BYTES = 3253312
+# HBM bytes per chase launch (FP32, n = 1204, b = 8) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this command,
+# each scaled by the same counter's reading for a 256 MiB copy (--pmc-calibration): fetch 41.875 KiB x 2.00, write
+# 2939.94 KiB x 1.00.
+TRAFFIC_BYTES = 3096256
 
 
 def log(*a):
@@ -103,7 +103,7 @@ def kernel_roofline(torch, xtbeig, hnd, h, s, b):
     ms = sum(times) / len(times)
     bytes_per_launch = 2 * (2 * b + 1) * n * n * 4
     achieved = bytes_per_launch / (ms * 1e-3) / 1e9
-    return {"kernel": f"chase_kernel<float, 16, LDS> (bulge chase, n={n}, b={b})", "bound": "hbm",
+    return {"kernel": f"chase_lanes_kernel<float, {b}, 2, LDS> (bulge chase, n={n}, b={b})", "bound": "hbm",
             "achieved": achieved, "peak": MI355X_HBM_GBS, "unit": "GB/s", "frac": achieved / MI355X_HBM_GBS,
             "traffic": TRAFFIC_BYTES, "ms_per_launch": ms, "bytes_per_launch": bytes_per_launch}
 

@@ -60,6 +60,20 @@ def test_potrf_inverse(handle, torch_cuda):
     assert np.max(np.abs(x @ u - np.eye(n))) < 1e-12
 
 
+@pytest.mark.parametrize("bad", [0, 70, 263])
+def test_potrf_not_positive_definite(handle, torch_cuda, bad):
+    # LAPACK's dpotrf convention: info = order of the first leading minor that is not positive definite
+    torch = torch_cuda
+    n = 300
+    s = workloads.overlap_matrix(n, np.random.default_rng(3))
+    s[bad, bad] = -1.0
+    S = dev(torch, s)
+    U = torch.zeros_like(S)
+    X = torch.zeros_like(S)
+    handle.potrf_inverse(S, U, X)
+    assert handle.synchronize()[0] == bad + 1  # reading the status also clears it
+
+
 @pytest.mark.parametrize("prec,b,n", [("d", 32, 260), ("d", 8, 131), ("s", 16, 200), ("d", 32, 33), ("d", 4, 9),
                                       ("d", 16, 700), ("s", 8, 1300), ("d", 8, 1100), ("d", 16, 1700)])
 def test_reduce_to_band(handle, torch_cuda, prec, b, n):

@@ -528,6 +528,16 @@ void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* st
   else launch_chase<T, 0>(s, n, b, ab, v2, d, e, status);
 }
 
+bool chase_in_lds(int n, int b, bool fp32) {
+  constexpr int WAVES = 8;
+  size_t need;
+  if (b == 8) need = fp32 ? chase_lanes_lds<8, 2, true, float>(n, WAVES) : chase_lanes_lds<8, 2, true, double>(n, WAVES);
+  else if (b == 16)
+    need = fp32 ? chase_lanes_lds<16, 2, true, float>(n, WAVES) : chase_lanes_lds<16, 2, true, double>(n, WAVES);
+  else return false;
+  return need + sizeof(int) * WAVES * (64 / (2 * b)) <= 160 * 1024;  // as launch_chase_lanes decides
+}
+
 template void chase_bulges<double>(hipStream_t, int, int, double*, double*, double*, double*, int*);
 template void chase_bulges<float>(hipStream_t, int, int, float*, float*, float*, float*, int*);
 

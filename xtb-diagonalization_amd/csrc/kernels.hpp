@@ -58,6 +58,8 @@ __host__ __device__ inline int64_t band_elems(int n, int b) { return (int64_t)ba
 // holds sweep s, rows rs ..< re).
 template <typename T>
 void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* flags);
+// Whether the chase of an n x n band of width b (8 or 16) keeps the band in LDS.
+bool chase_in_lds(int n, int b, bool fp32);
 
 // dc.hip -----------------------------------------------------------------------------------------------------------
 // One merge of the divide-and-conquer tree: rows/columns off ..< off + sz, torn after n1; q2off: its Q2 / S offset.

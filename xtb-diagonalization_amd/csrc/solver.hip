@@ -16,15 +16,12 @@
 
 namespace xtbeig {
 
-// Band width per n: the widest band whose chase storage ((2b - 1) n elements plus the per-wave vectors) fits the
-// 160 KiB LDS of one CU, so the bulge chase runs LDS-resident: FP32 b = 16 up to n = 1280 and b = 8 up to 2600, FP64
-// b = 8 up to n = 1290; beyond that b = 16 with the band in global memory.
+// Band width per n: 8 while the chase of that band is LDS-resident (FP64 up to n = 1255, FP32 up to about 2500), else
+// 16. At n = 1204 the band of 8 is the faster in both precisions: the chase is cheaper per sweep, and the twice as
+// many dense-to-band panels cost less than that saves.
 int default_band(int n, bool fp32) {
   if (n <= 3) return 1;
-  // b = 16 in both precisions: the FP32 band of 8 leaves the refined C^T S C - I above 1e-12 n at n = 1204, and the
-  // FP64 band of 16 is the faster one there.
-  (void)fp32;
-  const int b = 16;
+  const int b = chase_in_lds(n, 8, fp32) ? 8 : 16;
   return std::min(b, std::max(2, n - 1));
 }
 
