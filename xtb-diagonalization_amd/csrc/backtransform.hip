@@ -34,7 +34,10 @@ constexpr int BULGE_WAVES = 8;  // wavefronts per workgroup
 constexpr int GROUP_K = 64;     // reflectors per merged panel block
 constexpr int SPLIT_KC = 128;   // rows per chunk of the split product W = zt V
 
-template <typename T, int B>
+// REV = false: Z <- Q2 Z (zt <- zt Q2^T), the sequence described above. REV = true: Z <- Q2^T Z (zt <- zt Q2), the
+// same sequence run backwards: stages from the last to the first, each block's reflectors from its first sweep to
+// its last.
+template <typename T, int B, bool REV>
 __global__ __launch_bounds__(64 * BULGE_WAVES) void bulge_apply_kernel(int n, int b, int m,
                                                                        const BulgeBlock* __restrict__ blocks,
                                                                        const int* __restrict__ stage_off,
@@ -50,7 +53,8 @@ __global__ __launch_bounds__(64 * BULGE_WAVES) void bulge_apply_kernel(int n, in
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int col = blockIdx.x * 64 + lane;  // < ldzt; columns past m are padding
   T* ws = wsh[wid];
-  for (int st = 0; st < nstages; ++st) {
+  for (int si = 0; si < nstages; ++si) {
+    const int st = REV ? nstages - 1 - si : si;
     const int beg = stage_off[st], end = stage_off[st + 1];
     for (int q = beg + wid; q < end; q += BULGE_WAVES) {
       const BulgeBlock bk = blocks[order[q]];
@@ -71,7 +75,8 @@ __global__ __launch_bounds__(64 * BULGE_WAVES) void bulge_apply_kernel(int n, in
 #pragma unroll
       for (int i = 0; i < H; ++i) z[i] = zt[(int64_t)(r0 + i) * ldzt + col];
 #pragma unroll
-      for (int j = BULGE_G - 1; j >= 0; --j) {
+      for (int jj = 0; jj < BULGE_G; ++jj) {
+        const int j = REV ? jj : BULGE_G - 1 - jj;
         T wr[B];
 #pragma unroll
         for (int r = 0; r < B; ++r) wr[r] = ws[j * WS + r];
@@ -211,9 +216,13 @@ __global__ __launch_bounds__(256) void group_combine_kernel(int m, int K, int sp
 }
 
 template <typename T, int B>
-void launch_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m) {
-  hipLaunchKernelGGL((bulge_apply_kernel<T, B>), dim3(cdiv(m, 64)), dim3(64 * BULGE_WAVES), 0, s, p.n, p.b, m,
-                     p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt);
+void launch_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m, bool transposed) {
+  if (transposed)
+    hipLaunchKernelGGL((bulge_apply_kernel<T, B, true>), dim3(cdiv(m, 64)), dim3(64 * BULGE_WAVES), 0, s, p.n, p.b, m,
+                       p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt);
+  else
+    hipLaunchKernelGGL((bulge_apply_kernel<T, B, false>), dim3(cdiv(m, 64)), dim3(64 * BULGE_WAVES), 0, s, p.n, p.b,
+                       m, p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt);
   XTB_LAUNCH_CHECK();
 }
 
@@ -289,14 +298,14 @@ size_t back_band_workspace(int n, int b, int m) {
 }
 
 template <typename T>
-void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m) {
+void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m, bool transposed) {
   if (p.nbulge == 0 || m <= 0) return;
   if (ldzt < zt_ld(m)) throw std::runtime_error("bulge back-transform: zt leading dimension below zt_ld(m)");
   switch (bulge_width_class(p.b)) {
-    case 4: launch_bulges<T, 4>(s, p, v2, zt, ldzt, m); break;
-    case 8: launch_bulges<T, 8>(s, p, v2, zt, ldzt, m); break;
-    case 16: launch_bulges<T, 16>(s, p, v2, zt, ldzt, m); break;
-    default: launch_bulges<T, 32>(s, p, v2, zt, ldzt, m); break;
+    case 4: launch_bulges<T, 4>(s, p, v2, zt, ldzt, m, transposed); break;
+    case 8: launch_bulges<T, 8>(s, p, v2, zt, ldzt, m, transposed); break;
+    case 16: launch_bulges<T, 16>(s, p, v2, zt, ldzt, m, transposed); break;
+    default: launch_bulges<T, 32>(s, p, v2, zt, ldzt, m, transposed); break;
   }
 }
 
@@ -328,8 +337,8 @@ void back_transform_band(hipStream_t s, const BackPlan& p, const T* v1, const T*
 
 template size_t back_band_workspace<double>(int, int, int);
 template size_t back_band_workspace<float>(int, int, int);
-template void back_transform_bulges<double>(hipStream_t, const BackPlan&, const double*, double*, int, int);
-template void back_transform_bulges<float>(hipStream_t, const BackPlan&, const float*, float*, int, int);
+template void back_transform_bulges<double>(hipStream_t, const BackPlan&, const double*, double*, int, int, bool);
+template void back_transform_bulges<float>(hipStream_t, const BackPlan&, const float*, float*, int, int, bool);
 template void back_transform_band<double>(hipStream_t, const BackPlan&, const double*, const double*, double*, int,
                                           int, double*);
 template void back_transform_band<float>(hipStream_t, const BackPlan&, const float*, const float*, float*, int, int,

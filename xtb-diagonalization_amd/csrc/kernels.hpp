@@ -120,9 +120,11 @@ size_t back_band_workspace(int n, int b, int m);  // elements of T
 // the last zt_rows(n) - n of them zero (the bulge kernel's windows run into them).
 int zt_ld(int m);
 int zt_rows(int n);
-// zt <- zt Q2^T, i.e. Z <- Q2 Z (v2: the chase's reflectors w, H = I - w w^T, column s = sweep s).
+// zt <- zt Q2^T, i.e. Z <- Q2 Z (v2: the chase's reflectors w, H = I - w w^T, column s = sweep s); transposed:
+// zt <- zt Q2, i.e. Z <- Q2^T Z.
 template <typename T>
-void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m);
+void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m,
+                           bool transposed = false);
 // zt <- zt Q1^T, i.e. Z <- Q1 Z (v1, t1: the panels of reduce_to_band; v1 zero outside the panels' V).
 template <typename T>
 void back_transform_band(hipStream_t s, const BackPlan& p, const T* v1, const T* t1, T* zt, int ldzt, int m, T* work);

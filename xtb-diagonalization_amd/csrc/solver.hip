@@ -29,12 +29,21 @@ Handle::Handle(int dev) : device(dev) {
   XTB_HIP(hipSetDevice(device));
   XTB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   own_stream = true;
+  XTB_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  XTB_HIP(hipEventCreateWithFlags(&ev_band, hipEventDisableTiming));
+  XTB_HIP(hipEventCreateWithFlags(&ev_chase, hipEventDisableTiming));
+  XTB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   XTB_HIP(hipMalloc(&dinfo, 64 * sizeof(int)));
   XTB_HIP(hipMemset(dinfo, 0, 64 * sizeof(int)));
 }
 
 Handle::~Handle() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (stream2) (void)hipStreamSynchronize(stream2);
+  if (ev_band) (void)hipEventDestroy(ev_band);
+  if (ev_chase) (void)hipEventDestroy(ev_chase);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (stream2) (void)hipStreamDestroy(stream2);
   if (arena) (void)hipFree(arena);
   if (aux) (void)hipFree(aux);
   if (dinfo) (void)hipFree(dinfo);
@@ -133,7 +142,7 @@ struct Carver {
 
 template <typename T>
 struct StdBuffers {
-  T *v1, *t1, *ab, *bwork, *v2, *dT, *eT, *btwork, *zt;
+  T *v1, *t1, *ab, *bwork, *v2, *dT, *eT, *btwork, *zt, *qt;
   double *d64, *e64, *z64, *dcwork;
   int* dciwork;
 };
@@ -151,39 +160,60 @@ StdBuffers<T> carve_std(Carver& c, int n, int b) {
   s.eT = c.take<T>(n);
   s.btwork = c.take<T>(back_band_workspace<T>(n, b, n));
   s.zt = c.take<T>((size_t)zt_ld(n) * zt_rows(n));
+  s.qt = c.take<T>((size_t)zt_ld(n) * zt_rows(n));
   s.d64 = c.take<double>(n);
   s.e64 = c.take<double>(n);
-  s.z64 = std::is_same<T, double>::value ? nullptr : c.take<double>(nn);
+  s.z64 = c.take<double>(nn);
   s.dcwork = c.take<double>(dc_workspace_doubles(n));
   s.dciwork = c.take<int>(dc_workspace_ints(n));
   return s;
 }
 
 // Eigenpairs of the symmetric a (n x n, full storage, destroyed): w ascending (FP64), z (n x n).
+// z = Q1 Q2 Zdc: Q1 from the dense-to-band panels, Q2 from the chase, Zdc the tridiagonal's eigenvectors. M = Q1 Q2
+// is formed on the handle's second stream while the chase and the divide and conquer run on the first: Q1 as the band
+// back-transformation of the identity (as soon as the band is there), then M = Q1 Q2 by the bulge reflectors applied
+// from the right (once the chase is done); the first stream then needs one product, z = M Zdc.
 template <typename T>
 void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, double* w, T* z, int ldz) {
-  hipStream_t st = h.stream;
+  hipStream_t st = h.stream, st2 = h.stream2;
   if (n == 1) {
     copy_matrix<T, double>(st, 1, 1, a, lda, w, 1);
     set_identity(st, 1, z, ldz);
     return;
   }
+  const int ldzt = zt_ld(n), rows = zt_rows(n);
+  const BackPlan& bp = h.backplan(n, b);
   reduce_to_band<T>(st, n, b, a, lda, s.v1, s.t1, s.ab, s.bwork);
+  XTB_HIP(hipEventRecord(h.ev_band, st));
+  XTB_HIP(hipStreamWaitEvent(st2, h.ev_band, 0));
+  // second stream: zt = (Q1 I)^T, qt = Q1 (both ldzt x rows, the rows past n zero)
+  set_zero<T>(st2, ldzt, rows, s.zt, ldzt);
+  set_identity<T>(st2, n, s.zt, ldzt);
+  back_transform_band<T>(st2, bp, s.v1, s.t1, s.zt, ldzt, n, s.btwork);
+  set_zero<T>(st2, ldzt, rows - n, s.qt + (size_t)n * ldzt, ldzt);
+  transpose_matrix<T, T>(st2, n, n, s.zt, ldzt, s.qt, ldzt);
+  // first stream: the chase, then the tridiagonal's eigenpairs
   chase_bulges<T>(st, n, b, s.ab, s.v2, s.dT, s.eT, h.dinfo + 1);
+  XTB_HIP(hipEventRecord(h.ev_chase, st));
   copy_matrix<T, double>(st, n, 1, s.dT, n, s.d64, n);
   copy_matrix<T, double>(st, n - 1, 1, s.eT, n, s.e64, n);
-  double* z64 = std::is_same<T, double>::value ? reinterpret_cast<double*>(z) : s.z64;
-  const int ldz64 = std::is_same<T, double>::value ? ldz : n;
-  stedc(st, h.plan(n), s.d64, s.e64, z64, ldz64, s.dcwork, s.dciwork, h.dinfo + 2);
+  stedc(st, h.plan(n), s.d64, s.e64, s.z64, n, s.dcwork, s.dciwork, h.dinfo + 2);
   copy_matrix<double, double>(st, n, 1, s.d64, n, w, n);
-  // The back-transformations run on zt = Z^T (padded: zt_ld(n) x zt_rows(n), rows past n zero).
-  const int ldzt = zt_ld(n);
-  set_zero<T>(st, ldzt, zt_rows(n) - n, s.zt + (size_t)n * ldzt, ldzt);
-  transpose_matrix<double, T>(st, n, n, z64, ldz64, s.zt, ldzt);
-  const BackPlan& bp = h.backplan(n, b);
-  back_transform_bulges<T>(st, bp, s.v2, s.zt, ldzt, n);
-  back_transform_band<T>(st, bp, s.v1, s.t1, s.zt, ldzt, n, s.btwork);
-  transpose_matrix<T, T>(st, n, n, s.zt, ldzt, z, ldz);
+  // second stream: qt <- qt Q2 = Q1 Q2 (qt read as a column-major matrix with leading dimension ldzt)
+  XTB_HIP(hipStreamWaitEvent(st2, h.ev_chase, 0));
+  back_transform_bulges<T>(st2, bp, s.v2, s.qt, ldzt, n, true);
+  XTB_HIP(hipEventRecord(h.ev_join, st2));
+  XTB_HIP(hipStreamWaitEvent(st, h.ev_join, 0));
+  // z = M Zdc (Zdc rounded to T in zt, which the second stream no longer needs)
+  const T* zdc;
+  if (std::is_same<T, double>::value) {
+    zdc = reinterpret_cast<const T*>(s.z64);
+  } else {
+    copy_matrix<double, T>(st, n, n, s.z64, n, s.zt, n);
+    zdc = s.zt;
+  }
+  gemm<T>(st, 'N', 'N', n, n, n, T(1), s.qt, ldzt, zdc, n, T(0), z, ldz);
 }
 
 template <typename T>

@@ -15,6 +15,10 @@ struct Handle {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // the solve's second stream (forming the back-transformation's orthogonal factor while the chase and the divide
+  // and conquer run) and the events that fork it off and join it back
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_band = nullptr, ev_chase = nullptr, ev_join = nullptr;
   void* arena = nullptr;
   size_t arena_bytes = 0;
   // second device buffer, for work that wraps a solve (the solve itself carves the arena)
