@@ -104,7 +104,9 @@ def test_reduce_to_band(handle, torch_cuda, prec, b, n):
 
 
 @pytest.mark.parametrize("prec,b,n", [("d", 32, 260), ("d", 8, 131), ("s", 16, 200), ("d", 3, 40), ("d", 16, 300),
-                                      ("s", 8, 517), ("s", 32, 100)])
+                                      ("s", 8, 517), ("s", 32, 100),
+                                      # the lane-per-column chase at a few bulges per sweep and at b = n - 1
+                                      ("d", 8, 9), ("s", 8, 11), ("d", 16, 17), ("d", 16, 40), ("s", 16, 33)])
 def test_chase(handle, torch_cuda, prec, b, n):
     torch = torch_cuda
     rng = np.random.default_rng(5)

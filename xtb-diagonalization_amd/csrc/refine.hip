@@ -88,7 +88,7 @@ __global__ void scatter_columns_kernel(int n, int m, const int* __restrict__ ran
 
 }  // namespace
 
-size_t refine_workspace(int n) { return 6 * (size_t)n * n + 2 * (size_t)n + 16; }
+size_t refine_workspace(int n) { return 8 * (size_t)n * n + 2 * (size_t)n + 16; }
 
 // One or more Ogita-Aishima steps on C (n x n, ldc), then eigenvalues w (ascending) and C's columns sorted to match.
 void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm, int lds, double* c, int ldc,
@@ -102,13 +102,14 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   double* a = r + nn;
   double* e = a + nn;
   double* cnew = e + nn;
-  double* lam = cnew + nn;
+  double* shm = cnew + nn;  // [S | H], each n x n (ld n): the paired products run as one batched launch
+  double* lam = shm + 2 * nn;
   const dim3 eg(cdiv(n, 256), n);
+  copy_matrix<double, double>(s, n, n, sm, lds, shm, n);
+  copy_matrix<double, double>(s, n, n, h, ldh, shm + nn, n);
   for (int it = 0; it < iterations; ++it) {
-    gemm<double>(s, 'N', 'N', n, n, n, 1.0, sm, lds, c, ldc, 0.0, sc, n);   // S C
-    gemm<double>(s, 'N', 'N', n, n, n, 1.0, h, ldh, c, ldc, 0.0, hc, n);    // H C
-    gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, sc, n, 0.0, r, n);      // C^T S C
-    gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, hc, n, 0.0, a, n);      // C^T H C
+    gemm<double>(s, 'N', 'N', n, n, n, 1.0, shm, n, c, ldc, 0.0, sc, n, 2, nn, 0, nn);  // S C, H C
+    gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, sc, n, 0.0, r, n, 2, 0, nn, nn);    // C^T S C, C^T H C
     hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam);
     XTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n);
