@@ -33,6 +33,7 @@ constexpr int BULGE_G = 32;     // sweeps per bulge block
 constexpr int BULGE_WAVES = 8;  // wavefronts per workgroup
 constexpr int GROUP_K = 64;     // reflectors per merged panel block
 constexpr int SPLIT_KC = 128;   // rows per chunk of the split product W = zt V
+constexpr size_t LDS_BYTES = 160 * 1024;  // LDS per workgroup on gfx950
 
 // REV = false: Z <- Q2 Z (zt <- zt Q2^T), the sequence described above. REV = true: Z <- Q2^T Z (zt <- zt Q2), the
 // same sequence run backwards: stages from the last to the first, each block's reflectors from its first sweep to
@@ -99,6 +100,147 @@ __global__ __launch_bounds__(64 * BULGE_WAVES) void bulge_apply_kernel(int n, in
       __builtin_amdgcn_wave_barrier();  // ws is rewritten by the next block
     }
     __syncthreads();
+  }
+}
+
+// The same sequence with the strip zt(0 ..< rows, C columns) resident in LDS, for sizes where it fits: one workgroup
+// per C columns (lane = column within a group of C lanes, 64 / C groups per wavefront), so a 1200-column zt spreads
+// over 150 workgroups instead of 19. Each lane group is a slot that takes one block per round of a stage; its window
+// moves between LDS and registers, so a stage costs the block's reflector chain and one barrier, not a round trip to
+// memory. The reflector entries of the slot's next block are loaded into registers while the current block computes
+// (the block descriptors two blocks ahead), and staged into the slot's LDS area when that block starts.
+template <typename T, int B, bool REV, int C, int W>
+__global__ __launch_bounds__(64 * W) void bulge_strip_kernel(int n, int b, int rows, int rs, int nstages,
+                                                             const BulgeBlock* __restrict__ sorted,
+                                                             const int* __restrict__ stage_off,
+                                                             const T* __restrict__ v2, int ldv2,
+                                                             T* __restrict__ zt, int ldzt) {
+  constexpr int G = BULGE_G, H = G + B - 1, WE = G * B;
+  constexpr int GPW = 64 / C, SLOTS = W * GPW;
+  constexpr int SS = WE + 16 / (int)sizeof(T);  // slot stride: the slots of one wavefront start 4 banks apart
+  constexpr int PE = WE / C;                     // reflector entries a lane stages per block
+  extern __shared__ __align__(16) char smem_raw[];
+  T* slots = reinterpret_cast<T*>(smem_raw);
+  T* strip = slots + SLOTS * SS;  // column c of the strip at strip[c rs + row]
+  int* soff = reinterpret_cast<int*>(strip + C * rs);
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int grp = lane / C, cl = lane % C;
+  const int slot = wid * GPW + grp;
+  const int col0 = blockIdx.x * C;
+  T* ws = slots + slot * SS;
+  T* zs = strip + cl * rs;
+  for (int e = t; e < rows * C; e += 64 * W) {
+    const int r = e / C, c = e % C;
+    strip[c * rs + r] = zt[(int64_t)r * ldzt + col0 + c];
+  }
+  for (int e = t; e <= nstages; e += 64 * W) soff[e] = stage_off[e];
+  __syncthreads();
+
+  // Tasks: (stage si in processing order, round rd); the slot's block of a task is q = beg + rd SLOTS + slot when
+  // q < end. si == nstages marks the end of the sequence.
+  auto advance = [&](int& si, int& rd) {
+    if (si >= nstages) return;
+    const int st = REV ? nstages - 1 - si : si;
+    if ((rd + 1) * SLOTS < soff[st + 1] - soff[st]) {
+      ++rd;
+    } else {
+      ++si;
+      rd = 0;
+    }
+  };
+  auto block_of = [&](int si, int rd) {
+    BulgeBlock bk{0, 0, 0, 0};
+    if (si < nstages) {
+      const int st = REV ? nstages - 1 - si : si;
+      const int q = soff[st] + rd * SLOTS + slot;
+      if (q < soff[st + 1]) bk = sorted[q];
+    }
+    return bk;
+  };
+  auto fetch = [&](const BulgeBlock& bk, T (&pre)[PE]) {
+#pragma unroll
+    for (int u = 0; u < PE; ++u) {
+      const int e = cl + C * u, j = e / B, r = e % B;
+      const int rsw = bk.row0 + j;
+      T x = 0;
+      if (j < bk.count && r < min(b, n - rsw)) x = v2[(int64_t)(rsw + r) + (int64_t)ldv2 * (bk.s0 + j)];
+      pre[u] = x;
+    }
+  };
+
+  int si_a = 0, rd_a = 0;  // current task
+  int si_b = si_a, rd_b = rd_a;
+  advance(si_b, rd_b);     // next
+  int si_c = si_b, rd_c = rd_b;
+  advance(si_c, rd_c);     // the one after
+  BulgeBlock cur = block_of(si_a, rd_a);
+  BulgeBlock nxt = block_of(si_b, rd_b);
+  T pre[PE];
+  fetch(cur, pre);
+  while (si_a < nstages) {
+    // stage the current block's reflectors (zero for absent sweeps and idle slots)
+#pragma unroll
+    for (int u = 0; u < PE; ++u) ws[cl + C * u] = pre[u];
+    __builtin_amdgcn_wave_barrier();
+    const BulgeBlock bk = cur;
+    // the descriptor two tasks ahead, then the next block's entries
+    const BulgeBlock nn = block_of(si_c, rd_c);
+    fetch(nxt, pre);
+    if (bk.count > 0) {
+      const int r0 = bk.row0;
+      T z[H];
+#pragma unroll
+      for (int i = 0; i < H; ++i) z[i] = zs[r0 + i];
+      T wa[B];
+      {
+        const int j = REV ? 0 : G - 1;
+#pragma unroll
+        for (int r = 0; r < B; ++r) wa[r] = ws[j * B + r];
+      }
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) {
+        const int j = REV ? jj : G - 1 - jj;
+        T wb[B];
+        if (jj + 1 < G) {
+          const int jn = REV ? j + 1 : j - 1;
+#pragma unroll
+          for (int r = 0; r < B; ++r) wb[r] = ws[jn * B + r];
+        }
+        T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+#pragma unroll
+        for (int r = 0; r < B; r += 4) {
+          d0 += wa[r] * z[j + r];
+          d1 += wa[r + 1] * z[j + r + 1];
+          d2 += wa[r + 2] * z[j + r + 2];
+          d3 += wa[r + 3] * z[j + r + 3];
+        }
+        const T dot = (d0 + d1) + (d2 + d3);
+#pragma unroll
+        for (int r = 0; r < B; ++r) z[j + r] -= wa[r] * dot;
+        // compiler-only fence: reflector reads stay one reflector ahead of the arithmetic
+        __asm__ volatile("" ::: "memory");
+        if (jj + 1 < G) {
+#pragma unroll
+          for (int r = 0; r < B; ++r) wa[r] = wb[r];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < H; ++i) zs[r0 + i] = z[i];
+    }
+    __builtin_amdgcn_wave_barrier();  // the slot's reflectors are rewritten by the next block
+    const bool stage_end = si_b != si_a;
+    cur = nxt;
+    nxt = nn;
+    si_a = si_b;
+    rd_a = rd_b;
+    si_b = si_c;
+    rd_b = rd_c;
+    advance(si_c, rd_c);
+    if (stage_end) __syncthreads();
+  }
+  for (int e = t; e < n * C; e += 64 * W) {
+    const int r = e / C, c = e % C;
+    zt[(int64_t)r * ldzt + col0 + c] = strip[c * rs + r];
   }
 }
 
@@ -215,8 +357,33 @@ __global__ __launch_bounds__(256) void group_combine_kernel(int m, int K, int sp
   }
 }
 
+// The LDS-resident variant with C columns per workgroup and W wavefronts, when its LDS fits (false: not launched).
+template <typename T, int B, bool REV, int C, int W>
+bool launch_strip(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m) {
+  constexpr int SS = BULGE_G * B + 16 / (int)sizeof(T);
+  const int rs = p.rows | 1;  // odd column stride: the C lanes of a group read distinct banks
+  const size_t lds = sizeof(T) * ((size_t)W * (64 / C) * SS + (size_t)C * rs) + sizeof(int) * (p.nstages + 1);
+  if (lds > LDS_BYTES) return false;
+  allow_lds(bulge_strip_kernel<T, B, REV, C, W>, lds);
+  hipLaunchKernelGGL((bulge_strip_kernel<T, B, REV, C, W>), dim3(cdiv(m, C)), dim3(64 * W), lds, s, p.n, p.b, p.rows,
+                     rs, p.nstages, p.sorted_dev, p.stage_off_dev, v2, p.n, zt, ldzt);
+  XTB_LAUNCH_CHECK();
+  return true;
+}
+
+template <typename T, int B, bool REV>
+bool launch_strips(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m) {
+  if constexpr (B > 16) {
+    return false;
+  } else {
+    return launch_strip<T, B, REV, 8, 4>(s, p, v2, zt, ldzt, m) || launch_strip<T, B, REV, 8, 2>(s, p, v2, zt, ldzt, m);
+  }
+}
+
 template <typename T, int B>
 void launch_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt, int m, bool transposed) {
+  if (transposed ? launch_strips<T, B, true>(s, p, v2, zt, ldzt, m) : launch_strips<T, B, false>(s, p, v2, zt, ldzt, m))
+    return;
   if (transposed)
     hipLaunchKernelGGL((bulge_apply_kernel<T, B, true>), dim3(cdiv(m, 64)), dim3(64 * BULGE_WAVES), 0, s, p.n, p.b, m,
                        p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt);
@@ -269,10 +436,17 @@ BackPlan::BackPlan(int n_, int b_) : n(n_), b(b_), g(BULGE_G) {
     }
     std::vector<int> fill(off.begin(), off.end() - 1);
     for (int q = 0; q < nbulge; ++q) order[fill[stage[q] - 1]++] = q;
+    std::vector<BulgeBlock> sorted(nbulge);
+    for (int q = 0; q < nbulge; ++q) {
+      sorted[q] = bl[order[q]];
+      rows = std::max(rows, bl[q].row0 + bl[q].height);
+    }
     XTB_HIP(hipMalloc(&blocks_dev, sizeof(BulgeBlock) * nbulge));
+    XTB_HIP(hipMalloc(&sorted_dev, sizeof(BulgeBlock) * nbulge));
     XTB_HIP(hipMalloc(&stage_off_dev, sizeof(int) * (nstages + 1)));
     XTB_HIP(hipMalloc(&order_dev, sizeof(int) * nbulge));
     XTB_HIP(hipMemcpy(blocks_dev, bl.data(), sizeof(BulgeBlock) * nbulge, hipMemcpyHostToDevice));
+    XTB_HIP(hipMemcpy(sorted_dev, sorted.data(), sizeof(BulgeBlock) * nbulge, hipMemcpyHostToDevice));
     XTB_HIP(hipMemcpy(stage_off_dev, off.data(), sizeof(int) * (nstages + 1), hipMemcpyHostToDevice));
     XTB_HIP(hipMemcpy(order_dev, order.data(), sizeof(int) * nbulge, hipMemcpyHostToDevice));
   }
@@ -283,6 +457,7 @@ BackPlan::BackPlan(int n_, int b_) : n(n_), b(b_), g(BULGE_G) {
 
 BackPlan::~BackPlan() {
   if (blocks_dev) (void)hipFree(blocks_dev);
+  if (sorted_dev) (void)hipFree(sorted_dev);
   if (stage_off_dev) (void)hipFree(stage_off_dev);
   if (order_dev) (void)hipFree(order_dev);
 }

@@ -107,6 +107,8 @@ struct BackPlan {
   BulgeBlock* blocks_dev = nullptr;
   int* stage_off_dev = nullptr;  // nstages + 1
   int* order_dev = nullptr;      // block indices by stage
+  BulgeBlock* sorted_dev = nullptr;  // the blocks in stage order (blocks_dev[order_dev[q]] at q)
+  int rows = 0;                      // rows the blocks' windows reach (at most n + 63)
   // band panels j = 0, b, ..., last, merged ppg at a time into groups of up to ppg * b reflectors
   int last = -1, ppg = 1, ngroups = 0;
   BackPlan(int n, int b);

@@ -196,14 +196,15 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   // first stream: the chase, then the tridiagonal's eigenpairs
   chase_bulges<T>(st, n, b, s.ab, s.v2, s.dT, s.eT, h.dinfo + 1);
   XTB_HIP(hipEventRecord(h.ev_chase, st));
+  // second stream: qt <- qt Q2 = Q1 Q2 (qt read as a column-major matrix with leading dimension ldzt), enqueued
+  // before the divide and conquer so that it starts as soon as the chase ends
+  XTB_HIP(hipStreamWaitEvent(st2, h.ev_chase, 0));
+  back_transform_bulges<T>(st2, bp, s.v2, s.qt, ldzt, n, true);
+  XTB_HIP(hipEventRecord(h.ev_join, st2));
   copy_matrix<T, double>(st, n, 1, s.dT, n, s.d64, n);
   copy_matrix<T, double>(st, n - 1, 1, s.eT, n, s.e64, n);
   stedc(st, h.plan(n), s.d64, s.e64, s.z64, n, s.dcwork, s.dciwork, h.dinfo + 2);
   copy_matrix<double, double>(st, n, 1, s.d64, n, w, n);
-  // second stream: qt <- qt Q2 = Q1 Q2 (qt read as a column-major matrix with leading dimension ldzt)
-  XTB_HIP(hipStreamWaitEvent(st2, h.ev_chase, 0));
-  back_transform_bulges<T>(st2, bp, s.v2, s.qt, ldzt, n, true);
-  XTB_HIP(hipEventRecord(h.ev_join, st2));
   XTB_HIP(hipStreamWaitEvent(st, h.ev_join, 0));
   // z = M Zdc (Zdc rounded to T in zt, which the second stream no longer needs)
   const T* zdc;
