@@ -91,9 +91,16 @@ def test_reduce_to_band(handle, torch_cuda, prec, b, n):
     wb = np.linalg.eigvalsh(band)
     tol = 1e-13 if prec == "d" else 3e-6
     assert np.max(np.abs(wb - lam)) < tol
-    # same algorithm as the oracle's restatement of BandForm.swift: same band to rounding
+    # same algorithm as the oracle's restatement of BandForm.swift: same band to rounding (in FP32 up to the sign of
+    # a reflector whose leading element is within rounding of zero, which flips a row and column of the band: the
+    # magnitudes are compared, and Q1^T A Q1 = band below pins the signs)
     oband, _, _ = sbr.reduce_to_band(a, b)
-    assert np.max(np.abs(band - oband)) < (1e-12 if prec == "d" else 2e-5) * n
+    if prec == "d":
+        assert np.max(np.abs(band - oband)) < 1e-12 * n
+    else:
+        dm = np.abs(np.abs(band) - np.abs(oband))
+        assert np.max(dm) < 2e-5 * n, (np.unravel_index(np.argmax(dm), dm.shape), np.max(dm),
+                                       np.max(np.abs(band - oband)))
     # back-transformation through the panels: Q1 = (back_band of I), Q1^T A Q1 = band
     Z = torch.eye(n, dtype=dt, device="cuda")
     handle.back_band(n, b, V1, T1, Z, n)

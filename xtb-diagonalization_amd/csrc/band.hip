@@ -215,10 +215,11 @@ __device__ inline T bfly_reduce(const T (&v)[NB]) {
 // built in LDS one column per step. Outputs: R over p's upper triangle (the part below it is left as it was: only the
 // band, diagonals 0 .. b, is read afterwards), V (unit diagonal, zeros above) into v1 (ld ldv1), tau, and T into tt
 // (ld ldt, zeros below).
+// The factorization proper, on the panel already in registers (x[r][c] = p(t + 256 r, c), zero past m); shared by
+// panel_factor_kernel and panel_update_factor_kernel.
 template <typename T, int NB, int R>
-__global__ __launch_bounds__(256) void panel_factor_kernel(int m, T* __restrict__ p, int lda, T* __restrict__ v1,
-                                                           int ldv1, T* __restrict__ tau, T* __restrict__ tt,
-                                                           int ldt) {
+__device__ __forceinline__ void factor_panel(int m, T (&x)[R][NB], T* __restrict__ p, int lda, T* __restrict__ v1,
+                                             int ldv1, T* __restrict__ tau, T* __restrict__ tt, int ldt) {
   constexpr int WAVES = 4, SPREAD = 64 / NB;  // lanes holding each reduced value
   __shared__ T red[2][WAVES][NB];
   __shared__ T rowb[2][NB];
@@ -228,13 +229,6 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(int m, T* __restrict_
   // T's strictly lower part is read (times a zero mask) by the branch-free T update: it must hold zeros, not whatever
   // an earlier kernel left in LDS (0 * NaN is NaN). The first column's barrier orders these stores before any read.
   for (int e = t; e < NB * (NB + 1); e += 256) (&Ts[0][0])[e] = T(0);
-  T x[R][NB];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int i = t + 256 * r;
-#pragma unroll
-    for (int c = 0; c < NB; ++c) x[r][c] = (i < m) ? p[idx(i, c, lda)] : T(0);
-  }
   // unrolled by 4: the rotation's register moves are needed only where the unrolled body loops back
 #pragma unroll 4
   for (int c = 0; c < NB; ++c) {
@@ -334,6 +328,21 @@ __global__ __launch_bounds__(256) void panel_factor_kernel(int m, T* __restrict_
     for (int c = 0; c < NB; ++c)
       if (c < k) v1[idx(i, c, ldv1)] = (i > c) ? x[r][c] : (i == c ? T(1) : T(0));
   }
+}
+
+template <typename T, int NB, int R>
+__global__ __launch_bounds__(256) void panel_factor_kernel(int m, T* __restrict__ p, int lda, T* __restrict__ v1,
+                                                           int ldv1, T* __restrict__ tau, T* __restrict__ tt,
+                                                           int ldt) {
+  const int t = threadIdx.x;
+  T x[R][NB];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = t + 256 * r;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[r][c] = (i < m) ? p[idx(i, c, lda)] : T(0);
+  }
+  factor_panel<T, NB, R>(m, x, p, lda, v1, ldv1, tau, tt, ldt);
 }
 
 // X = A2 V T for the panel's trailing block A2 (m x m, ld lda) and its reflectors V (m x k, ld ldv), in two launches
@@ -449,19 +458,51 @@ __global__ __launch_bounds__(256) void panel_xfin_kernel(int m, int k, const T* 
 // along A2's rows). The tile's rows I and columns J stage [V(I) | W(I)] and [W(J) | V(J)] (zero past k) in LDS;
 // K = 32.
 template <typename T>
-__global__ __launch_bounds__(256) void panel_update_kernel(int m, int k, T* __restrict__ a2, int lda,
-                                                           const T* __restrict__ v, int ldv, const T* __restrict__ x,
-                                                           const T* __restrict__ ypart, int nrb,
-                                                           const T* __restrict__ tt, int ldt) {
+struct UpdateLds {
+  T Ai[64][33];
+  T Bj[64][33];
+  T Ys[16][17];
+  T Ss[16][17];
+  T Ts[16][17];
+};
+
+// Y = T^T (V^T X) into L.Ys (k x k, zero elsewhere), from the row blocks' parts and T; ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void form_y(int k, const T* __restrict__ ypart, int nrb, const T* __restrict__ tt,
+                                       int ldt, UpdateLds<T>& L) {
+  const int t = threadIdx.x;
+  {
+    const int r = t % 16, c = t / 16;
+    T sum = 0;
+#pragma unroll 8
+    for (int q = 0; q < nrb; ++q) sum += ypart[(size_t)q * 256 + t];
+    L.Ss[r][c] = sum;
+    L.Ts[r][c] = (r < k && c < k && r <= c) ? tt[r + (size_t)c * ldt] : T(0);
+  }
+  __syncthreads();
+  {
+    const int r = t % 16, c = t / 16;
+    T sum = 0;
+#pragma unroll
+    for (int l = 0; l < 16; ++l) sum += L.Ts[l][r] * L.Ss[l][c];
+    L.Ys[r][c] = (r < k && c < k) ? sum : T(0);
+  }
+  __syncthreads();
+}
+
+// One 64 x 64 tile (rows I0.., columns J0..) of the update. Elements (i, j) with i >= skip and j < skip are left as
+// they are (the fused next-panel factorization takes those).
+template <typename T>
+__device__ __forceinline__ void update_tile(int m, int k, T* __restrict__ a2, int lda, const T* __restrict__ v,
+                                            int ldv, const T* __restrict__ x, const T* __restrict__ ypart, int nrb,
+                                            const T* __restrict__ tt, int ldt, int I0, int J0, int skip,
+                                            UpdateLds<T>& L) {
   using namespace gemm_detail;
   using acc_t = typename Mfma<T>::acc_t;
-  __shared__ T Ai[64][33];
-  __shared__ T Bj[64][33];
-  __shared__ T Ys[16][17];
-  __shared__ T Ss[16][17];
-  __shared__ T Ts[16][17];
+  auto& Ai = L.Ai;
+  auto& Bj = L.Bj;
+  auto& Ys = L.Ys;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  const int I0 = blockIdx.x * 64, J0 = blockIdx.y * 64;
   const int wm = wid & 1, wn = wid >> 1;
   // this thread's 16 elements of the A2 tile, in the accumulator layout, loaded first so that their latency overlaps
   // everything below
@@ -477,22 +518,7 @@ __global__ __launch_bounds__(256) void panel_update_kernel(int m, int k, T* __re
         old[bi][bj][r] = (i < m && j < m) ? a2[idx(i, j, lda)] : T(0);
       }
     }
-  {
-    const int r = t % 16, c = t / 16;
-    T sum = 0;
-#pragma unroll 8
-    for (int q = 0; q < nrb; ++q) sum += ypart[(size_t)q * 256 + t];
-    Ss[r][c] = sum;
-    Ts[r][c] = (r < k && c < k && r <= c) ? tt[r + (size_t)c * ldt] : T(0);
-  }
-  __syncthreads();
-  {
-    const int r = t % 16, c = t / 16;
-    T sum = 0;
-#pragma unroll
-    for (int l = 0; l < 16; ++l) sum += Ts[l][r] * Ss[l][c];
-    Ys[r][c] = (r < k && c < k) ? sum : T(0);
-  }
+  form_y<T>(k, ypart, nrb, tt, ldt, L);
   for (int e = t; e < 64 * 16; e += 256) {
     const int r = e % 64, c = e / 64;
     const bool oi = I0 + r < m && c < k, oj = J0 + r < m && c < k;
@@ -551,9 +577,85 @@ __global__ __launch_bounds__(256) void panel_update_kernel(int m, int k, T* __re
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int j = J0 + wm * 32 + bi * 16 + Mfma<T>::row(lane, r);
-        if (j < m) a2[idx(i, j, lda)] = old[bi][bj][r] - acc[bi][bj][r];
+        if (j < m && (j >= skip || i < skip)) a2[idx(i, j, lda)] = old[bi][bj][r] - acc[bi][bj][r];
       }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void panel_update_kernel(int m, int k, T* __restrict__ a2, int lda,
+                                                           const T* __restrict__ v, int ldv, const T* __restrict__ x,
+                                                           const T* __restrict__ ypart, int nrb,
+                                                           const T* __restrict__ tt, int ldt) {
+  __shared__ UpdateLds<T> L;
+  update_tile<T>(m, k, a2, lda, v, ldv, x, ypart, nrb, tt, ldt, blockIdx.x * 64, blockIdx.y * 64, 0, L);
+}
+
+// The update of this panel (as panel_update_kernel, workgroups 1 ..) fused with the factorization of the next panel
+// (workgroup 0): the next panel is A2(NB .., 0 ..< NB), and workgroup 0 applies this panel's update to just those
+// m - NB rows and NB columns itself, A2(i, c) - sum_l V(i, l) W(c, l) + W(i, l) V(c, l) with W = X - V Y / 2, holding
+// the result in registers for the factorization, while the other workgroups update the rest of A2 and leave those
+// elements alone. No workgroup waits for another; workgroup 0 is dispatched first, so the factorization runs beside
+// the bulk of the update instead of after it.
+template <typename T, int NB, int R>
+__global__ __launch_bounds__(256) void panel_update_factor_kernel(int m, T* __restrict__ a2, int lda,
+                                                                  const T* __restrict__ v, int ldv,
+                                                                  const T* __restrict__ x,
+                                                                  const T* __restrict__ ypart, int nrb,
+                                                                  const T* __restrict__ tt, int ldt,
+                                                                  T* __restrict__ v1n, int ldv1,
+                                                                  T* __restrict__ taun, T* __restrict__ ttn,
+                                                                  int ldtn) {
+  __shared__ UpdateLds<T> L;
+  const int t = threadIdx.x;
+  if (blockIdx.x > 0) {
+    const int q = blockIdx.x - 1;
+    update_tile<T>(m, NB, a2, lda, v, ldv, x, ypart, nrb, tt, ldt, (q % nrb) * 64, (q / nrb) * 64, NB, L);
+    return;
+  }
+  const int mn = m - NB;  // the next panel's rows
+  form_y<T>(NB, ypart, nrb, tt, ldt, L);
+  // V(c, :) and W(c, :) of the panel's columns c < NB, in Ai[c][0 ..< NB] and Ai[c][16 ..]
+  if (t < NB * NB) {
+    const int c = t % NB, l = t / NB;
+    T w = x[c + (size_t)l * m];
+#pragma unroll
+    for (int l2 = 0; l2 < NB; ++l2) w -= T(0.5) * v[idx(c, l2, ldv)] * L.Ys[l2][l];
+    L.Ai[c][l] = v[idx(c, l, ldv)];
+    L.Ai[c][16 + l] = w;
+  }
+  __syncthreads();
+  T xr[R][NB];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = NB + t + 256 * r;  // row of A2
+    if (t + 256 * r < mn) {
+      T vi[NB], wi[NB];
+#pragma unroll
+      for (int l = 0; l < NB; ++l) {
+        vi[l] = v[idx(i, l, ldv)];
+        wi[l] = x[i + (size_t)l * m];
+      }
+#pragma unroll
+      for (int l = 0; l < NB; ++l) {
+        T sy = 0;
+#pragma unroll
+        for (int l2 = 0; l2 < NB; ++l2) sy += vi[l2] * L.Ys[l2][l];
+        wi[l] -= T(0.5) * sy;
+      }
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        T u = 0;
+#pragma unroll
+        for (int l = 0; l < NB; ++l) u += vi[l] * L.Ai[c][16 + l] + wi[l] * L.Ai[c][l];
+        xr[r][c] = a2[idx(i, c, lda)] - u;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) xr[r][c] = T(0);
+    }
+  }
+  factor_panel<T, NB, R>(mn, xr, a2 + idx(NB, 0, lda), lda, v1n, ldv1, taun, ttn, ldtn);
 }
 
 // The per-panel reductions over the m rows run on cdiv(m, PR) workgroups, PR rows each; k x k partial products per
@@ -782,6 +884,57 @@ bool panel_factor(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, 
   return false;
 }
 
+template <typename T, int NB, int R>
+void launch_update_factor(hipStream_t s, int m, T* a2, int lda, const T* v, int ldv, const T* x, const T* kpart,
+                          int nrb, const T* tt, int ldt, T* v1n, int ldv1, T* tau, T* ttn) {
+  hipLaunchKernelGGL((panel_update_factor_kernel<T, NB, R>), dim3(1 + nrb * nrb), dim3(256), 0, s, m, a2, lda, v, ldv,
+                     x, kpart, nrb, tt, ldt, v1n, ldv1, tau, ttn, NB);
+  XTB_LAUNCH_CHECK();
+}
+
+// This panel's update fused with the next panel's factorization (band widths 8 and 16, the next panel's m - b rows
+// within panel_factor's limits); false when the two must run separately.
+template <typename T>
+bool update_factor(hipStream_t s, int m, int b, T* a2, int lda, const T* v, int ldv, const T* x, const T* kpart,
+                   int nrb, const T* tt, T* v1n, int ldv1, T* tau, T* ttn) {
+  const int r = cdiv(m - b, 256);
+#define XTB_UF(NB, R) launch_update_factor<T, NB, R>(s, m, a2, lda, v, ldv, x, kpart, nrb, tt, b, v1n, ldv1, tau, ttn)
+  if (b == 8) {
+    switch (r) {
+      case 0: case 1: XTB_UF(8, 1); return true;
+      case 2: XTB_UF(8, 2); return true;
+      case 3: XTB_UF(8, 3); return true;
+      case 4: XTB_UF(8, 4); return true;
+      case 5: XTB_UF(8, 5); return true;
+      case 6: XTB_UF(8, 6); return true;
+      case 7: XTB_UF(8, 7); return true;
+      case 8: XTB_UF(8, 8); return true;
+      default: return false;
+    }
+  }
+  if (b == 16) {
+    switch (r) {
+      case 0: case 1: XTB_UF(16, 1); return true;
+      case 2: XTB_UF(16, 2); return true;
+      case 3: XTB_UF(16, 3); return true;
+      case 4: XTB_UF(16, 4); return true;
+      case 5: XTB_UF(16, 5); return true;
+      case 6: XTB_UF(16, 6); return true;
+      case 7:
+        if constexpr (sizeof(T) == 8) return false;
+        else XTB_UF(16, 7);
+        return true;
+      case 8:
+        if constexpr (sizeof(T) == 8) return false;
+        else XTB_UF(16, 8);
+        return true;
+      default: return false;
+    }
+  }
+#undef XTB_UF
+  return false;
+}
+
 template <typename T>
 void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T* ab, T* work) {
   if (b > 32 || b < 1) throw std::runtime_error("reduce_to_band: band width must be in [1, 32]");
@@ -792,6 +945,7 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
   T* kpart = xpart + (size_t)((n + 63) / 64) * n * b;
   // v1 is zero outside the panels' V, so that consecutive panels read as one block [V_a V_b ...] from row r_a
   set_zero<T>(s, n, n, v1, n);
+  bool factored = false;  // this panel was factored by the previous panel's fused update
   for (int j = 0; j < n - b - 1; j += b) {
     const int r = j + b, m = n - r, k = std::min(b, m);
     T* panel = a + idx(r, j, lda);
@@ -800,20 +954,27 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
     T* a2 = a + idx(r, r, lda);
     T* x = buf + (size_t)m * k;  // slot 1 of [V | W | V]
     const int nblk = cdiv(m, PR);
-    if (panel_factor<T>(s, m, b, panel, lda, v, n, tau, tt, b)) {
+    if (factored || panel_factor<T>(s, m, b, panel, lda, v, n, tau, tt, b)) {
       // X = A2 V T (split-K partials, then their sum with T applied and the row blocks' parts of V^T X), then
-      // A2 -= V W^T + W V^T with Y and W formed per tile
+      // A2 -= V W^T + W V^T with Y and W formed per tile, with the next panel's factorization when it can ride along
       T* xb = buf;
       const int nks = cdiv(m, XSplit<T>::K), nrb = cdiv(m, 64);
       hipLaunchKernelGGL(panel_xpart_kernel<T>, dim3(nrb, nks), dim3(256), 0, s, m, k, a2, lda, v, n, xpart);
       XTB_LAUNCH_CHECK();
       hipLaunchKernelGGL(panel_xfin_kernel<T>, dim3(nrb), dim3(256), 0, s, m, k, xpart, nks, tt, b, v, n, xb, kpart);
       XTB_LAUNCH_CHECK();
-      hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb, nrb), dim3(256), 0, s, m, k, a2, lda, v, n, xb, kpart, nrb,
-                         tt, b);
-      XTB_LAUNCH_CHECK();
+      const int jn = j + b;
+      factored = jn < n - b - 1 && k == b &&
+                 update_factor<T>(s, m, b, a2, lda, v, n, xb, kpart, nrb, tt, v1 + idx(jn + b, jn, n), n, tau,
+                                  t1 + idx(0, jn, b));
+      if (!factored) {
+        hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb, nrb), dim3(256), 0, s, m, k, a2, lda, v, n, xb, kpart,
+                           nrb, tt, b);
+        XTB_LAUNCH_CHECK();
+      }
       continue;
     }
+    factored = false;
     {
       launch_panel_qr<T>(s, m, b, panel, lda, v, n, buf, tau);
       hipLaunchKernelGGL(panel_gram_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, kpart);  // V^T V parts
