@@ -1,10 +1,12 @@
 // Cholesky factor of the overlap S = U^T U and its inverse X = U^{-1}, FP64 (the generalized solve keeps S's factor
 // in FP64 in both precisions). Blocked right-looking potrf('U') with 64-column blocks: the diagonal block is factored
-// and inverted by one workgroup (4 x 4 sub-blocks per thread in registers), the row panel is U_kk^{-T} A_k* as a product with the inverted block,
+// and inverted by one workgroup (16-column panels in one wavefront's registers, MFMA block updates), the row panel is
+// U_kk^{-T} A_k* as a product with the inverted block,
 // and the trailing update is a GEMM on the matrix cores. X is then assembled by recursive halving,
 // X12 = -X11 U12 X22 (two GEMMs per split). X is what the congruence H' = X^T H X and the back-solve C = X C' use
 // (xTB's fact_solve path; scripts/kernels/congruence.c:1-8 of the reference).
 #include "common.hpp"
+#include "gemm_tile.hpp"
 #include "kernels.hpp"
 
 namespace xtbeig {
@@ -13,147 +15,166 @@ namespace {
 
 constexpr int NB = 64;
 
-// Factor A(k0:k0+kb, k0:k0+kb) = U^T U and invert U, by one workgroup of 256 threads (padded with the identity past
-// kb): thread (bi, bj) = (t % 16, t / 16) owns the 4 x 4 sub-block of rows 4 bi .., columns 4 bj .. in registers.
-// Right-looking on unnormalized rows: at step k the owners of row k publish it through LDS (double-buffered by
-// parity, so one barrier per step), every thread subtracts A(k, i) A(k, j) / A(k, k) from its elements with i, j > k,
-// and U(i, j) = A(i, j) / sqrt(A(i, i)) at the end (A's rows as they stand after their own step).
+// 64-bit value of lane l (l the same in every lane).
+__device__ inline double readlane_d(double v, int l) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, l);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1 / sqrt(p) to a few ulps: the hardware estimate and two Newton steps.
+__device__ inline double rsqrt_nr(double p) {
+  double y = __builtin_amdgcn_rsq(p);
+  y = y * (1.5 - 0.5 * p * y * y);
+  y = y * (1.5 - 0.5 * p * y * y);
+  return y;
+}
+
+// acc(row(l, r), l % 16) += sum over K = 4 ks of P(i, k) Q(k, j) for a 16 x 16 block: the FP64 MFMA on operands read
+// from LDS as P(i, k) = pa[i * lp + k] and Q(k, j) = qb[k * lq + j].
+__device__ inline gemm_detail::Mfma<double>::acc_t mfma_block(const double* pa, int lp, const double* qb, int lq,
+                                                               int ksteps) {
+  using M = gemm_detail::Mfma<double>;
+  const int lane = threadIdx.x & 63, c = lane & 15, q = lane >> 4;
+  M::acc_t acc = {0, 0, 0, 0};
+  for (int ks = 0; ks < ksteps; ++ks) acc = M::op(pa[c * lp + 4 * ks + q], qb[(4 * ks + q) * lq + c], acc);
+  return acc;
+}
+
+// Factor A(k0:k0+kb, k0:k0+kb) = L L^T (U = L^T) and invert L, by one workgroup of 256 threads, with the block in LDS
+// (padded with the identity past kb) and lanes along rows:
+// - four 16-column panels, right-looking: wavefront 0 factors the panel with row c0 + l in lane l's registers (the
+//   pivot and the panel rows' entries reach the other lanes by readlane, so a panel needs no barrier), then the four
+//   wavefronts subtract L21 L21^T from the trailing lower blocks with FP64 MFMA;
+// - Y = L^{-1}: each wavefront inverts one 16 x 16 diagonal block by forward substitution (lane = column), then the
+//   sibling blocks at sizes 16 and 32, Y_BA = -Y_BB (L_BA Y_AA), as MFMA block products.
+// Writes U = L^T and X = U^{-1} = Y^T at (k0, k0) (zero below the diagonal); info = k0 + k + 1 at a non-positive
+// pivot k (then nothing is written).
 __global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* __restrict__ a, int lda,
                                                          double* __restrict__ u, int ldu, double* __restrict__ x,
                                                          int ldx, int* info) {
-  __shared__ double rowb[2][NB];
-  __shared__ double piv[NB];
-  __shared__ double Us[NB][NB + 1];
-  __shared__ double Xs[NB][NB + 1];
-  __shared__ double Ts[NB][NB + 1];
+  constexpr int LD = NB + 1;
+  __shared__ double Am[NB][LD];
+  __shared__ double Ym[NB][LD];
+  __shared__ double Tm[32][33];
+  __shared__ int bad_s;
   if (*info != 0) return;
-  const int kb = min(NB, n - k0), t = threadIdx.x;
-  const int bi = t % 16, bj = t / 16, i0 = 4 * bi, j0 = 4 * bj;
-  double c[4][4];
+  const int kb = min(NB, n - k0), t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int lc = lane & 15, lq = lane >> 4;
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e % NB, j = e / NB;
+    Am[i][j] = (i < kb && j < kb) ? a[idx(k0 + i, k0 + j, lda)] : (i == j ? 1.0 : 0.0);
+    Ym[i][j] = 0.0;  // Y's upper blocks are read (as zeros) by the size-32 merge
+  }
+  if (t == 0) bad_s = 0;
+  __syncthreads();
+  for (int p = 0; p < NB / 16; ++p) {
+    const int c0 = 16 * p;
+    if (wid == 0) {
+      const int i = c0 + lane;
+      const bool live = i < NB;
+      double xr[16];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+      for (int q = 0; q < 16; ++q) xr[q] = live ? Am[i][c0 + q] : 0.0;
+      int bad = 0;
+      // no early exit (it would keep the loop rolled and index xr at run time): after a non-positive pivot the steps
+      // go on with pivot 1, and nothing of this panel is stored
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = i0 + r, j = j0 + q;
-      c[r][q] = (i < kb && j < kb) ? a[idx(k0 + i, k0 + j, lda)] : (i == j ? 1.0 : 0.0);
-    }
-  int bad = 0;
-  // right-looking on unnormalized rows: after step k, row k holds U(k, k) U(k, j); every thread owns its elements
-  for (int k = 0; k < NB; ++k) {
-    const int par = k & 1;
-    if (bi == (k >> 2)) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (i0 + r == k) {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) rowb[par][j0 + q] = c[r][q];
+      for (int kk = 0; kk < 16; ++kk) {
+        double piv = readlane_d(xr[kk], kk);
+        if (!(piv > 0.0)) {
+          if (bad == 0) bad = k0 + c0 + kk + 1;
+          piv = 1.0;
         }
-    }
-    __syncthreads();
-    const double p = rowb[par][k];
-    if (!(p > 0.0)) {
-      bad = k0 + k + 1;
-      break;
-    }
-    if (t == 0) piv[k] = p;
-    const double rp = 1.0 / p;
-    double ri[4], rj[4];
+        const double y = rsqrt_nr(piv);
+        xr[kk] = (lane == kk) ? piv * y : (lane > kk ? xr[kk] * y : xr[kk]);
+        // lanes <= kk change only entries above the diagonal, which nothing reads
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      ri[r] = rowb[par][i0 + r] * rp;
-      rj[r] = rowb[par][j0 + r];
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (i0 + r > k && j0 + q > k) c[r][q] -= ri[r] * rj[q];
-  }
-  if (bad) {
-    if (t == 0) atomicCAS(info, 0, bad);
-    return;
-  }
-  __syncthreads();
-  // U(i, j) = c(i, j) / sqrt(piv[i]) for i <= j
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const double s = 1.0 / sqrt(piv[i0 + r]);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = i0 + r, j = j0 + q;
-      Us[i][j] = (i <= j) ? c[r][q] * s : 0.0;
-    }
-  }
-  __syncthreads();
-  // X = U^{-1}: the 4 x 4 diagonal blocks by their owners (back substitution in registers), then for block sizes
-  // s = 4, 8, 16, 32 the off-diagonal block of every sibling pair A = [c, c + s), B = [c + s, c + 2s):
-  // X_AB = -X_AA (U_AB X_BB), through T = U_AB X_BB in LDS
-  if (bi == bj) {
-    double xb[4][4];
-#pragma unroll
-    for (int cc = 0; cc < 4; ++cc)
-#pragma unroll
-      for (int r = 3; r >= 0; --r) {
-        double sum = (r == cc) ? 1.0 : 0.0;
-#pragma unroll
-        for (int l = r + 1; l < 4; ++l) sum -= Us[i0 + r][i0 + l] * xb[l][cc];
-        xb[r][cc] = (r <= cc) ? sum / Us[i0 + r][i0 + r] : 0.0;
+        for (int jj = kk + 1; jj < 16; ++jj) xr[jj] -= xr[kk] * readlane_d(xr[kk], jj);
       }
+      if (bad) {
+        if (lane == 0) bad_s = bad;
+      } else if (live) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+        for (int q = 0; q < 16; ++q) Am[i][c0 + q] = xr[q];
+      }
+    }
+    __syncthreads();
+    if (*(volatile int*)&bad_s) {
+      if (t == 0) atomicCAS(info, 0, bad_s);
+      return;
+    }
+    // A22 -= L21 L21^T over the lower 16 x 16 blocks (I >= J) of rows and columns c0 + 16 ..
+    const int nb2 = NB / 16 - 1 - p;
+    for (int blk = wid; blk < nb2 * (nb2 + 1) / 2; blk += 4) {
+      int I = 0;
+      while ((I + 1) * (I + 2) / 2 <= blk) ++I;
+      const int J = blk - I * (I + 1) / 2;
+      const int rI = c0 + 16 + 16 * I, rJ = c0 + 16 + 16 * J;
+      // P(i, k) = L(rI + i, c0 + k), Q(k, j) = L(rJ + j, c0 + k)
+      using M = gemm_detail::Mfma<double>;
+      M::acc_t acc = {0, 0, 0, 0};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Xs[i0 + r][i0 + q] = xb[r][q];
+      for (int ks = 0; ks < 4; ++ks) acc = M::op(Am[rI + lc][c0 + 4 * ks + lq], Am[rJ + lc][c0 + 4 * ks + lq], acc);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Am[rI + M::row(lane, r)][rJ + lc] -= acc[r];
+    }
+    __syncthreads();
   }
-  if (bi > bj) {
+  // diagonal blocks of Y = L^{-1}: wavefront w, lane column c, Y(i, c) = (delta_ic - sum_{c <= l < i} L(i, l) Y(l, c))
+  // / L(i, i) for i >= c
+  {
+    const int d0 = 16 * wid;
+    double y[16];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int i = 0; i < 16; ++i) {
+      double s0 = (i == lc) ? 1.0 : 0.0, s1 = 0.0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Xs[i0 + r][j0 + q] = 0.0;
+      for (int l = 0; l + 1 < i; l += 2) {
+        s0 -= Am[d0 + i][d0 + l] * y[l];
+        s1 -= Am[d0 + i][d0 + l + 1] * y[l + 1];
+      }
+      if (i % 2 == 1) s0 -= Am[d0 + i][d0 + i - 1] * y[i - 1];
+      y[i] = (i >= lc) ? (s0 + s1) / Am[d0 + i][d0 + i] : 0.0;
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Ym[d0 + i][d0 + lc] = y[i];
+    }
   }
   __syncthreads();
-  for (int s = 4; s < NB; s *= 2) {
-    // this thread's sub-block lies in some pair's X_AB: rows in an A half, columns in the B half of the same pair
-    const bool mine = ((i0 / s) % 2 == 0) && (j0 / (2 * s) == i0 / (2 * s)) && ((j0 / s) % 2 == 1);
-    double tv[4][4];
-    if (mine) {
-      const int cB = (j0 / s) * s;
+  using M = gemm_detail::Mfma<double>;
+  // size 16: pairs (A, B) = (0, 1) and (2, 3) by wavefronts 0 and 1: T = L_BA Y_AA, then Y_BA = -Y_BB T
+  if (wid < 2) {
+    const int A0 = 32 * wid, B0 = A0 + 16;
+    M::acc_t acc = mfma_block(&Am[B0][A0], LD, &Ym[A0][A0], LD, 4);
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 4; ++r) Tm[16 * wid + M::row(lane, r)][lc] = acc[r];
+    __builtin_amdgcn_wave_barrier();
+    acc = mfma_block(&Ym[B0][B0], LD, &Tm[16 * wid][0], 33, 4);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          double sum = 0;
-          for (int l = cB; l < cB + s; ++l) sum += Us[i0 + r][l] * Xs[l][j0 + q];
-          tv[r][q] = sum;
-        }
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) Ts[i0 + r][j0 + q] = tv[r][q];
-    }
-    __syncthreads();
-    if (mine) {
-      const int cA = (i0 / s) * s;
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          double sum = 0;
-          for (int l = cA; l < cA + s; ++l) sum += Xs[i0 + r][l] * Ts[l][j0 + q];
-          tv[r][q] = -sum;
-        }
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) Xs[i0 + r][j0 + q] = tv[r][q];
-    }
-    __syncthreads();
+    for (int r = 0; r < 4; ++r) Ym[B0 + M::row(lane, r)][A0 + lc] = -acc[r];
   }
-  // write U and X (zero below the diagonal), lanes along rows
+  __syncthreads();
+  // size 32: A = [0, 32), B = [32, 64); wavefront w takes the output block (w / 2, w % 2) of T = L_BA Y_AA, then of
+  // Y_BA = -Y_BB T
+  {
+    const int bi = wid >> 1, bj = wid & 1;
+    M::acc_t acc = mfma_block(&Am[32 + 16 * bi][0], LD, &Ym[0][16 * bj], LD, 8);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Tm[16 * bi + M::row(lane, r)][16 * bj + lc] = acc[r];
+    __syncthreads();
+    acc = mfma_block(&Ym[32 + 16 * bi][32], LD, &Tm[0][16 * bj], 33, 8);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Ym[32 + 16 * bi + M::row(lane, r)][16 * bj + lc] = -acc[r];
+  }
+  __syncthreads();
   for (int e = t; e < NB * NB; e += 256) {
     const int i = e % NB, j = e / NB;
     if (i < kb && j < kb) {
-      u[idx(k0 + i, k0 + j, ldu)] = Us[i][j];
-      x[idx(k0 + i, k0 + j, ldx)] = Xs[i][j];
+      u[idx(k0 + i, k0 + j, ldu)] = (i <= j) ? Am[j][i] : 0.0;
+      x[idx(k0 + i, k0 + j, ldx)] = (i <= j) ? Ym[j][i] : 0.0;
     }
   }
 }
