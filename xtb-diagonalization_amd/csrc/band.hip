@@ -614,18 +614,46 @@ __global__ __launch_bounds__(256) void panel_update_factor_kernel(int m, T* __re
     return;
   }
   const int mn = m - NB;  // the next panel's rows
+  // Every load that does not wait on Y first: this thread's rows of A2(:, 0 ..< NB) (the factorization holds them
+  // in registers anyway), of V and X where they fit (FP32, 8 columns: 64 more registers at 4 rows per thread), and
+  // the panel columns' rows of V and X; their latency then overlaps the small products below.
+  constexpr bool EARLY = sizeof(T) == 4 && NB == 8 && R <= 4;
+  constexpr int RE = EARLY ? R : 1;
+  T xr[R][NB], vr[RE][NB], xq[RE][NB];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = NB + t + 256 * r;
+    const bool ok = t + 256 * r < mn;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) xr[r][c] = ok ? a2[idx(i, c, lda)] : T(0);
+  }
+  if constexpr (EARLY) {
+#pragma unroll
+    for (int r = 0; r < RE; ++r) {
+      const int i = NB + t + 256 * r;
+      const bool ok = t + 256 * r < mn;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        vr[r][c] = ok ? v[idx(i, c, ldv)] : T(0);
+        xq[r][c] = ok ? x[i + (size_t)c * m] : T(0);
+      }
+    }
+  }
+  const int cc = t % NB, lc = (t / NB) % NB;  // (c, l) of the panel columns' part for t < NB * NB
+  T vcl[NB];
+#pragma unroll
+  for (int l2 = 0; l2 < NB; ++l2) vcl[l2] = v[idx(cc, l2, ldv)];
+  const T xcl = x[cc + (size_t)lc * m];
   form_y<T>(NB, ypart, nrb, tt, ldt, L);
   // V(c, :) and W(c, :) of the panel's columns c < NB, in Ai[c][0 ..< NB] and Ai[c][16 ..]
   if (t < NB * NB) {
-    const int c = t % NB, l = t / NB;
-    T w = x[c + (size_t)l * m];
+    T w = xcl;
 #pragma unroll
-    for (int l2 = 0; l2 < NB; ++l2) w -= T(0.5) * v[idx(c, l2, ldv)] * L.Ys[l2][l];
-    L.Ai[c][l] = v[idx(c, l, ldv)];
-    L.Ai[c][16 + l] = w;
+    for (int l2 = 0; l2 < NB; ++l2) w -= T(0.5) * vcl[l2] * L.Ys[l2][lc];
+    L.Ai[cc][lc] = vcl[lc];
+    L.Ai[cc][16 + lc] = w;
   }
   __syncthreads();
-  T xr[R][NB];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = NB + t + 256 * r;  // row of A2
@@ -633,8 +661,13 @@ __global__ __launch_bounds__(256) void panel_update_factor_kernel(int m, T* __re
       T vi[NB], wi[NB];
 #pragma unroll
       for (int l = 0; l < NB; ++l) {
-        vi[l] = v[idx(i, l, ldv)];
-        wi[l] = x[i + (size_t)l * m];
+        if constexpr (EARLY) {
+          vi[l] = vr[r][l];
+          wi[l] = xq[r][l];
+        } else {
+          vi[l] = v[idx(i, l, ldv)];
+          wi[l] = x[i + (size_t)l * m];
+        }
       }
 #pragma unroll
       for (int l = 0; l < NB; ++l) {
@@ -648,11 +681,8 @@ __global__ __launch_bounds__(256) void panel_update_factor_kernel(int m, T* __re
         T u = 0;
 #pragma unroll
         for (int l = 0; l < NB; ++l) u += vi[l] * L.Ai[c][16 + l] + wi[l] * L.Ai[c][l];
-        xr[r][c] = a2[idx(i, c, lda)] - u;
+        xr[r][c] -= u;
       }
-    } else {
-#pragma unroll
-      for (int c = 0; c < NB; ++c) xr[r][c] = T(0);
     }
   }
   factor_panel<T, NB, R>(mn, xr, a2 + idx(NB, 0, lda), lda, v1n, ldv1, taun, ttn, ldtn);

@@ -1,7 +1,7 @@
 // The MFMA GEMM tile shared by every matrix-product kernel (gemm.hip, and the divide-and-conquer merge products
 // whose sizes are only known on the device). 64 x 64 tile of C per workgroup of 256 threads: 4 wavefronts in a 2 x 2
-// grid, 32 x 32 each = 2 x 2 MFMA blocks of 16 x 16; K staged through LDS 16 at a time with the next slab prefetched
-// into registers. MFMA operand layout (16x16x4): lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16); the
+// grid, 32 x 32 each = 2 x 2 MFMA blocks of 16 x 16; K staged through LDS 32 at a time, with the next two slabs
+// prefetched into registers (a slab is 2048 FP64 MFMA cycles per wavefront, less than the latency of its loads). MFMA operand layout (16x16x4): lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16); the
 // accumulator layout differs between FP32 and FP64 (Mfma<T>::row).
 #pragma once
 #include "common.hpp"
@@ -9,7 +9,8 @@
 namespace xtbeig {
 namespace gemm_detail {
 
-constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+constexpr int BM = 64, BN = 64, BK = 32, PAD = 4;
+constexpr int SL = BK / 4;  // elements of each operand a thread loads per slab
 
 template <typename T> struct Mfma;
 // Accumulator layout: register r of lane l holds C(row(l, r), l % 16). The FP32 16x16x4 instruction places rows
@@ -29,42 +30,55 @@ template <> struct Mfma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// Loads of one K slab: 4 elements of op(A) (64 x 16) and 4 of op(B) (16 x 64) per thread.
+// Loads of one K slab: SL elements of op(A) (64 x BK) and SL of op(B) (BK x 64) per thread, along the contiguous
+// dimension of each operand.
 template <typename T, bool TA, bool TB>
 struct Slab {
-  T a[4], b[4];
+  T a[SL], b[SL];
+  // One base pointer per thread and operand, then uniform strides between its SL elements (so that few addresses
+  // stay live across the two slabs in flight).
   __device__ inline void load(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, int M, int N,
                               int K, int m0, int n0, int k0, int t) {
+    if (!TA) {
+      const int i = t & 63, kq = t >> 6;  // k = kq + 4 r
+      const bool oki = m0 + i < M;
+      const T* pa = A + (m0 + i) + (int64_t)(k0 + kq) * lda;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int i, k;
-      if (!TA) { i = t & 63; k = (t >> 6) + 4 * r; } else { k = t & 15; i = (t >> 4) + 16 * r; }
-      const int gi = m0 + i, gk = k0 + k;
-      T v = T(0);
-      if (gi < M && gk < K) v = TA ? A[idx(gk, gi, lda)] : A[idx(gi, gk, lda)];
-      a[r] = v;
+      for (int r = 0; r < SL; ++r) a[r] = (oki && k0 + kq + 4 * r < K) ? pa[(int64_t)(4 * r) * lda] : T(0);
+    } else {
+      const int k = t % BK, iq = t / BK;  // i = iq + (256 / BK) r
+      const bool okk = k0 + k < K;
+      const T* pa = A + (k0 + k) + (int64_t)(m0 + iq) * lda;
+#pragma unroll
+      for (int r = 0; r < SL; ++r)
+        a[r] = (okk && m0 + iq + (256 / BK) * r < M) ? pa[(int64_t)((256 / BK) * r) * lda] : T(0);
     }
+    if (!TB) {
+      const int k = t % BK, jq = t / BK;  // j = jq + (256 / BK) r
+      const bool okk = k0 + k < K;
+      const T* pb = B + (k0 + k) + (int64_t)(n0 + jq) * ldb;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      int j, k;
-      if (!TB) { k = t & 15; j = (t >> 4) + 16 * r; } else { j = t & 63; k = (t >> 6) + 4 * r; }
-      const int gj = n0 + j, gk = k0 + k;
-      T v = T(0);
-      if (gj < N && gk < K) v = TB ? B[idx(gj, gk, ldb)] : B[idx(gk, gj, ldb)];
-      b[r] = v;
+      for (int r = 0; r < SL; ++r)
+        b[r] = (okk && n0 + jq + (256 / BK) * r < N) ? pb[(int64_t)((256 / BK) * r) * ldb] : T(0);
+    } else {
+      const int j = t & 63, kq = t >> 6;  // k = kq + 4 r
+      const bool okj = n0 + j < N;
+      const T* pb = B + (n0 + j) + (int64_t)(k0 + kq) * ldb;
+#pragma unroll
+      for (int r = 0; r < SL; ++r) b[r] = (okj && k0 + kq + 4 * r < K) ? pb[(int64_t)(4 * r) * ldb] : T(0);
     }
   }
   __device__ inline void store(T (*As)[BM + PAD], T (*Bs)[BN + PAD], int t) const {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < SL; ++r) {
       int i, k;
-      if (!TA) { i = t & 63; k = (t >> 6) + 4 * r; } else { k = t & 15; i = (t >> 4) + 16 * r; }
+      if (!TA) { i = t & 63; k = (t >> 6) + 4 * r; } else { k = t % BK; i = t / BK + (256 / BK) * r; }
       As[k][i] = a[r];
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < SL; ++r) {
       int j, k;
-      if (!TB) { k = t & 15; j = (t >> 4) + 16 * r; } else { j = t & 63; k = (t >> 6) + 4 * r; }
+      if (!TB) { k = t % BK; j = t / BK + (256 / BK) * r; } else { j = t & 63; k = (t >> 6) + 4 * r; }
       Bs[k][j] = b[r];
     }
   }
@@ -72,7 +86,7 @@ struct Slab {
 
 // One 64 x 64 tile of C = alpha op(A) op(B) + beta C at (m0, n0), by the calling workgroup of 256 threads.
 template <typename T, bool TA, bool TB>
-__device__ inline void gemm_tile(int M, int N, int K, T alpha, const T* __restrict__ A, int lda,
+__device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T* __restrict__ A, int lda,
                                  const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc, int m0,
                                  int n0, T (*As)[BM + PAD], T (*Bs)[BN + PAD]) {
   using acc_t = typename Mfma<T>::acc_t;
@@ -84,13 +98,7 @@ __device__ inline void gemm_tile(int M, int N, int K, T alpha, const T* __restri
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = acc_t{0, 0, 0, 0};
 
-  Slab<T, TA, TB> slab;
-  slab.load(A, lda, B, ldb, M, N, K, m0, n0, 0, t);
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
-    slab.store(As, Bs, t);
-    __syncthreads();
-    if (k0 + BK < K) slab.load(A, lda, B, ldb, M, N, K, m0, n0, k0 + BK, t);
+  auto compute = [&]() {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4), c = lane & 15;
@@ -104,6 +112,24 @@ __device__ inline void gemm_tile(int M, int N, int K, T alpha, const T* __restri
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = Mfma<T>::op(av[bi], bv[bj], acc[bi][bj]);
     }
+  };
+  // two slabs in flight: the loads of slab s + 2 are issued as slab s goes to LDS, so each has two slabs' compute
+  // to arrive
+  Slab<T, TA, TB> s0, s1;
+  s0.load(A, lda, B, ldb, M, N, K, m0, n0, 0, t);
+  if (BK < K) s1.load(A, lda, B, ldb, M, N, K, m0, n0, BK, t);
+  for (int k0 = 0; k0 < K; k0 += 2 * BK) {
+    __syncthreads();
+    s0.store(As, Bs, t);
+    __syncthreads();
+    if (k0 + 2 * BK < K) s0.load(A, lda, B, ldb, M, N, K, m0, n0, k0 + 2 * BK, t);
+    compute();
+    if (k0 + BK >= K) break;
+    __syncthreads();
+    s1.store(As, Bs, t);
+    __syncthreads();
+    if (k0 + 3 * BK < K) s1.load(A, lda, B, ldb, M, N, K, m0, n0, k0 + 3 * BK, t);
+    compute();
   }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
