@@ -138,6 +138,117 @@ def cpu_baseline(h, s, sample=5):
     return sum(times) / len(times), w
 
 
+def components(torch, np, workloads, xtbeig, hnd, h, s):
+    """Each entry point of the path timed alone at n = 1204, one JSON line per entry point.
+
+    Device-resident stages: HIP events on the handle's stream around each call, inputs restored (outside the timed
+    region) before every call, the median of 7 calls after 2 warm-up ones. Host-array entry points (the sytrd.c
+    symbol set and the LAPACK-style drivers): wall clock around the call, host-device transfers included.
+    """
+    n = N
+    stream = torch.cuda.ExternalStream(hnd.stream_ptr())
+
+    def dev_time(restore, call, reps=7, warm=2):
+        ts = []
+        for r in range(warm + reps):
+            restore()
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            call()
+            e1.record(stream)
+            st = hnd.synchronize()
+            if st != [0, 0, 0]:
+                raise RuntimeError(f"status {st}")
+            if r >= warm:
+                ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    def host_time(call, reps=3):
+        ts = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            call()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts[1:])) * 1e3
+
+    def emit(row, name, ms, dtype, note):
+        print(json.dumps({"component": name, "path_row": row, "n": n, "ms": round(ms, 4), "dtype": dtype,
+                          "timing": note}), flush=True)
+
+    a, _ = workloads.standard_matrix(n, 11)
+    for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+        b = xtbeig.default_band(n, fp32=dt == torch.float32)
+        A0 = torch.from_numpy(np.ascontiguousarray(a.T)).cuda().to(dt)
+        A = A0.clone()
+        V1 = torch.zeros((n, n), dtype=dt, device="cuda")
+        T1 = torch.zeros((n, b), dtype=dt, device="cuda")
+        AB = torch.zeros(xtbeig.band_elems(n, b), dtype=dt, device="cuda")
+        ms = dev_time(lambda: A.copy_(A0), lambda: hnd.reduce_to_band(A, b, V1, T1, AB))
+        emit("a7", f"reduce_to_band (dense -> band {b})", ms, tag, "HIP events, device-resident")
+        AB0 = AB.clone()
+        V2 = torch.zeros((n, n), dtype=dt, device="cuda")
+        D = torch.zeros(n, dtype=dt, device="cuda")
+        E = torch.zeros(n, dtype=dt, device="cuda")
+        ms = dev_time(lambda: AB.copy_(AB0), lambda: hnd.chase(n, b, AB, V2, D, E))
+        emit("a8", f"chase (band {b} -> tridiagonal)", ms, tag, "HIP events, device-resident")
+        Z = torch.eye(n, dtype=dt, device="cuda")
+        ms = dev_time(lambda: None, lambda: hnd.back_bulges(n, b, V2, Z, n))
+        emit("a9", "bulge back-transform Z <- Q2 Z (m = n)", ms, tag, "HIP events, device-resident")
+        ms = dev_time(lambda: None, lambda: hnd.back_band(n, b, V1, T1, Z, n))
+        emit("a10/a11", "band back-transform Z <- Q1 Z (m = n, WY blocks)", ms, tag, "HIP events, device-resident")
+        W = torch.zeros(n, dtype=torch.float64, device="cuda")
+        Zs = torch.zeros((n, n), dtype=dt, device="cuda")
+        ms = dev_time(lambda: A.copy_(A0),
+                      lambda: (hnd.dsyevd if dt == torch.float64 else hnd.ssyevd)(A, W, Zs))
+        emit("a12", "syevd (standard eigensolve, two-stage chain, all eigenpairs)", ms, tag,
+             "HIP events, device-resident")
+    d0 = torch.from_numpy(np.linspace(-1.0, 1.0, n) + 0.01 * np.sin(np.arange(n))).cuda()
+    e0 = torch.full((n,), 0.3, dtype=torch.float64, device="cuda")
+    d, e = d0.clone(), e0.clone()
+    Zt = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    ms = dev_time(lambda: (d.copy_(d0), e.copy_(e0)), lambda: hnd.dstedc(d, e, Zt))
+    emit("a5", "stedc (divide and conquer, all eigenpairs of T)", ms, "f64", "HIP events, device-resident")
+    S0 = torch.from_numpy(np.ascontiguousarray(s.T)).cuda()
+    S = S0.clone()
+    U = torch.zeros_like(S)
+    X = torch.zeros_like(S)
+    ms = dev_time(lambda: S.copy_(S0), lambda: hnd.potrf_inverse(S, U, X))
+    emit("a13", "potrf + inverse of S (U, X = U^-1)", ms, "f64", "HIP events, device-resident")
+    for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+        Ga = torch.randn(n, n, dtype=dt, device="cuda")
+        Gb = torch.randn(n, n, dtype=dt, device="cuda")
+        Gc = torch.zeros(n, n, dtype=dt, device="cuda")
+        ms = dev_time(lambda: None, lambda: hnd.gemm("N", "N", n, n, n, 1.0, Ga, n, Gb, n, 0.0, Gc, n))
+        emit("a11/a13/a15", f"gemm NN {n}^3 ({2 * n ** 3 / ms / 1e9:.1f} TF/s)", ms, tag,
+             "HIP events, device-resident")
+    # host-array entry points (the sytrd.c symbol set and LAPACK-style drivers), transfers included
+    for dtype, tag in ((np.float64, "f64"), (np.float32, "f32")):
+        res = {}
+        ms = host_time(lambda: res.update(r=xtbeig.sytrd_upper(a, dtype=dtype)))
+        pre = "d" if tag == "f64" else "s"
+        emit("a1-a4", f"xtb_{pre}sytrd_upper (one-stage reduction)", ms, tag, "wall clock, host arrays")
+        ra, rd, re_, rtau = res["r"]
+        ms = host_time(lambda: xtbeig.stedc(rd, re_, dtype=dtype))
+        emit("a5", "xtb_{}stedc (host arrays)".format("d" if tag == "f64" else "s"), ms, tag,
+             "wall clock, host arrays")
+        zi = np.eye(n, dtype=dtype)
+        ms = host_time(lambda: xtbeig.ormtr_upper(ra, rtau, zi, dtype=dtype))
+        emit("a6", "xtb_{}ormtr_upper (m = n)".format("d" if tag == "f64" else "s"), ms, tag,
+             "wall clock, host arrays")
+    ms = host_time(lambda: xtbeig.sytrd_upper(a, full=True))
+    emit("a14", "xtb_dsytrd_full (full padded copy)", ms, "f64", "wall clock, host arrays")
+    ms = host_time(lambda: xtbeig.dsymmetrize_pad(a, ldd=n + 8))
+    emit("a14", "xtb_dsymmetrize_pad", ms, "f64", "wall clock, host arrays")
+    ms = host_time(lambda: xtbeig.syevd(a))
+    emit("a12", "xtbeig_dsyevd (host arrays)", ms, "f64", "wall clock, host arrays")
+    ms = host_time(lambda: xtbeig.dsygvd(h, s))
+    emit("a13", "xtbeig_dsygvd (host arrays)", ms, "f64", "wall clock, host arrays")
+    ms = host_time(lambda: xtbeig.dsygvd(h, s, mixed=True))
+    emit("a13/a15", "xtbeig_dsygvd_mixed (host arrays, FP32 + refinement)", ms, "f32+f64", "wall clock, host arrays")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +257,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU baseline (for profiler runs)")
     ap.add_argument("--cpu-sample", type=int, default=5, help="solves in the CPU baseline's sample")
     ap.add_argument("--pmc-calibration", action="store_true", help="run a 256 MiB copy first (counter scaling)")
+    ap.add_argument("--components", action="store_true",
+                    help="time each entry point of the path alone at n = 1204 (one JSON line each) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the eigensolves are "
@@ -167,6 +280,9 @@ def main():
     w = torch.zeros(N, dtype=torch.float64, device="cuda")
     c = torch.zeros((N, N), dtype=torch.float64, device="cuda")
     hnd = xtbeig.Handle(0)
+    if args.components:
+        components(torch, np, workloads, xtbeig, hnd, h, s)
+        return
     if args.pmc_calibration:
         pmc_calibration(torch)
     band = xtbeig.default_band(N, fp32=True)

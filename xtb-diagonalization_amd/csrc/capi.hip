@@ -56,10 +56,17 @@ void read_status(Handle& h, int* st) {
   XTB_HIP(hipStreamSynchronize(h.stream));
 }
 
+// A device buffer of a host-array call: the next of the calling thread's handle slots, kept between calls (a
+// hipMalloc / hipFree pair per call cost more than the transfers). Buffers are released in reverse order.
 struct DevBuf {
   void* p = nullptr;
-  explicit DevBuf(size_t bytes) { XTB_HIP(hipMalloc(&p, bytes > 0 ? bytes : 1)); }
-  ~DevBuf() { if (p) (void)hipFree(p); }
+  explicit DevBuf(size_t bytes) {
+    Handle& h = thread_handle();
+    p = h.io_slot(h.io_top++, bytes > 0 ? bytes : 1);
+  }
+  ~DevBuf() { --thread_handle().io_top; }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
   template <typename T> T* as() { return static_cast<T*>(p); }
 };
 
@@ -69,16 +76,36 @@ void h2d(Handle& h, void* dst, const void* src, size_t bytes) {
 void d2h(Handle& h, void* dst, const void* src, size_t bytes) {
   XTB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h.stream));
 }
-// column-major sub-matrix copies with leading dimensions
+// Column-major sub-matrix copies with leading dimensions. Contiguous on both sides: one copy. A strided host matrix
+// goes through the handle's pinned staging buffer (packed or unpacked column by column on the host, one transfer),
+// since a pitched copy from pageable memory moves column by column.
 template <typename T>
 void h2d_mat(Handle& h, T* dst, int ldd, const T* src, int lds, int m, int n) {
-  XTB_HIP(hipMemcpy2DAsync(dst, sizeof(T) * ldd, src, sizeof(T) * lds, sizeof(T) * m, n, hipMemcpyHostToDevice,
+  if (m <= 0 || n <= 0) return;
+  if (lds == m && ldd == m) {
+    h2d(h, dst, src, sizeof(T) * (size_t)m * n);
+    return;
+  }
+  XTB_HIP(hipStreamSynchronize(h.stream));  // the staging buffer may still feed an earlier copy
+  T* stage = static_cast<T*>(h.host_staging(sizeof(T) * (size_t)m * n));
+  for (int j = 0; j < n; ++j) std::memcpy(stage + (size_t)j * m, src + (size_t)j * lds, sizeof(T) * (size_t)m);
+  XTB_HIP(hipMemcpy2DAsync(dst, sizeof(T) * ldd, stage, sizeof(T) * m, sizeof(T) * m, n, hipMemcpyHostToDevice,
                            h.stream));
+  XTB_HIP(hipStreamSynchronize(h.stream));
 }
 template <typename T>
 void d2h_mat(Handle& h, T* dst, int ldd, const T* src, int lds, int m, int n) {
-  XTB_HIP(hipMemcpy2DAsync(dst, sizeof(T) * ldd, src, sizeof(T) * lds, sizeof(T) * m, n, hipMemcpyDeviceToHost,
+  if (m <= 0 || n <= 0) return;
+  if (lds == m && ldd == m) {
+    d2h(h, dst, src, sizeof(T) * (size_t)m * n);
+    return;
+  }
+  XTB_HIP(hipStreamSynchronize(h.stream));
+  T* stage = static_cast<T*>(h.host_staging(sizeof(T) * (size_t)m * n));
+  XTB_HIP(hipMemcpy2DAsync(stage, sizeof(T) * m, src, sizeof(T) * lds, sizeof(T) * m, n, hipMemcpyDeviceToHost,
                            h.stream));
+  XTB_HIP(hipStreamSynchronize(h.stream));
+  for (int j = 0; j < n; ++j) std::memcpy(dst + (size_t)j * ldd, stage + (size_t)j * m, sizeof(T) * (size_t)m);
 }
 
 bool is_char(const char* c, char u) { return c && (*c == u || *c == (char)(u + 32)); }

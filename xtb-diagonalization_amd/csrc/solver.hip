@@ -48,6 +48,8 @@ Handle::~Handle() {
   if (aux) (void)hipFree(aux);
   if (dinfo) (void)hipFree(dinfo);
   if (pinned) (void)hipHostFree(pinned);
+  for (auto& sl : io_slots)
+    if (sl.first) (void)hipFree(sl.first);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -104,6 +106,19 @@ void* Handle::host_staging(size_t bytes) {
     pinned_bytes = bytes;
   }
   return pinned;
+}
+
+void* Handle::io_slot(int k, size_t bytes) {
+  if ((int)io_slots.size() <= k) io_slots.resize(k + 1, {nullptr, 0});
+  auto& sl = io_slots[k];
+  if (bytes > sl.second) {
+    // the previous contents are no longer in use: every host-array call synchronizes before it returns
+    if (sl.first) XTB_HIP(hipFree(sl.first));
+    sl.first = nullptr;
+    XTB_HIP(hipMalloc(&sl.first, bytes));
+    sl.second = bytes;
+  }
+  return sl.first;
 }
 
 const DcPlan& Handle::plan(int n) {

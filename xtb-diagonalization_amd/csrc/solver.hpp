@@ -28,6 +28,10 @@ struct Handle {
   int* dinfo = nullptr;
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // device buffers of the host-array entry points, kept between calls (grown on demand): slot k is the k-th buffer
+  // a call holds at once
+  std::vector<std::pair<void*, size_t>> io_slots;
+  int io_top = 0;
   int refine_iters = 2;
   int band_override = 0;
   // SCC precision schedule: settings (fp32_rmsq 0 = FP64 only) and the counters of the current sequence
@@ -50,6 +54,7 @@ struct Handle {
   void reserve(size_t bytes);
   void reserve_aux(size_t bytes);
   void* host_staging(size_t bytes);
+  void* io_slot(int k, size_t bytes);
   const DcPlan& plan(int n);
   const BackPlan& backplan(int n, int b);
   // band width for an n x n solve: the caller's override, else the tuned table, else the built-in one
