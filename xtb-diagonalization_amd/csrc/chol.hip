@@ -218,9 +218,11 @@ void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu,
       // U(k0 block, rest) = U_kk^{-T} A(k0 block, rest)
       gemm<double>(s, 'T', 'N', kb, r, kb, 1.0, x + idx(k0, k0, ldx), ldx, a + idx(k0, k0 + kb, lda), lda, 0.0,
                    u + idx(k0, k0 + kb, ldu), ldu);
-      // A(rest, rest) -= U(k0 block, rest)^T U(k0 block, rest)
-      gemm<double>(s, 'T', 'N', r, r, kb, -1.0, u + idx(k0, k0 + kb, ldu), ldu, u + idx(k0, k0 + kb, ldu), ldu, 1.0,
-                   a + idx(k0 + kb, k0 + kb, lda), lda);
+      // A(rest, rest) -= U(k0 block, rest)^T U(k0 block, rest), on the tiles the later steps read: the diagonal
+      // blocks and everything above them (the diagonal kernel reads its whole 64 x 64 block, the row panels the part
+      // right of it)
+      gemm_upper_tn<double>(s, r, kb, -1.0, u + idx(k0, k0 + kb, ldu), ldu, u + idx(k0, k0 + kb, ldu), ldu, 1.0,
+                            a + idx(k0 + kb, k0 + kb, lda), lda);
     }
   }
   inverse_blocks(s, 0, n, u, ldu, x, ldx, tmp);

@@ -27,6 +27,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
   gemm_tile<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.x * BM, blockIdx.y * BN, As, Bs);
 }
 
+// The tiles (I, J) with I <= J of an n x n C only, one workgroup each (tile q = J (J + 1) / 2 + I).
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_upper_kernel(int N, int K, T alpha, const T* __restrict__ A, int lda,
+                                                         const T* __restrict__ B, int ldb, T beta,
+                                                         T* __restrict__ C, int ldc) {
+  __shared__ T As[BK][BM + PAD];
+  __shared__ T Bs[BK][BN + PAD];
+  const int q = blockIdx.x;
+  int J = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+  while (J * (J + 1) / 2 > q) --J;
+  while ((J + 1) * (J + 2) / 2 <= q) ++J;
+  const int I = q - J * (J + 1) / 2;
+  gemm_tile<T, TA, TB>(N, N, K, alpha, A, lda, B, ldb, beta, C, ldc, I * BM, J * BN, As, Bs);
+}
+
 template <typename T, bool TA, bool TB>
 void launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, int lda, int64_t sA, const T* B, int ldb,
             int64_t sB, T beta, T* C, int ldc, int64_t sC, int batch) {
@@ -88,6 +103,23 @@ int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T
   XTB_LAUNCH_CHECK();
   return splits;
 }
+template <typename T>
+void gemm_upper_tn(hipStream_t s, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
+                   int ldc) {
+  static_assert(BM == BN, "square tiles");
+  if (n <= 0) return;
+  if (k <= 0) {
+    gemm<T>(s, 'T', 'N', n, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return;
+  }
+  const int nt = cdiv(n, BM);
+  hipLaunchKernelGGL((gemm_upper_kernel<T, true, false>), dim3(nt * (nt + 1) / 2), dim3(256), 0, s, n, k, alpha, A,
+                     lda, B, ldb, beta, C, ldc);
+  XTB_LAUNCH_CHECK();
+}
+template void gemm_upper_tn<double>(hipStream_t, int, int, double, const double*, int, const double*, int, double,
+                                    double*, int);
+
 template int gemm_splitk<double>(hipStream_t, int, int, int, const double*, int, const double*, int, double*, int);
 template int gemm_splitk<float>(hipStream_t, int, int, int, const float*, int, const float*, int, float*, int);
 
