@@ -30,10 +30,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha,
 // The tiles (I, J) with I <= J of an n x n C only, one workgroup each (tile q = J (J + 1) / 2 + I).
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_upper_kernel(int N, int K, T alpha, const T* __restrict__ A, int lda,
-                                                         const T* __restrict__ B, int ldb, T beta,
-                                                         T* __restrict__ C, int ldc) {
+                                                         int64_t sA, const T* __restrict__ B, int ldb, int64_t sB,
+                                                         T beta, T* __restrict__ C, int ldc, int64_t sC) {
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
+  A += blockIdx.y * sA;
+  B += blockIdx.y * sB;
+  C += blockIdx.y * sC;
   const int q = blockIdx.x;
   int J = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
   while (J * (J + 1) / 2 > q) --J;
@@ -105,20 +108,22 @@ int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T
 }
 template <typename T>
 void gemm_upper_tn(hipStream_t s, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
-                   int ldc) {
+                   int ldc, int batch, int64_t sA, int64_t sB, int64_t sC) {
   static_assert(BM == BN, "square tiles");
-  if (n <= 0) return;
+  if (n <= 0 || batch <= 0) return;
   if (k <= 0) {
-    gemm<T>(s, 'T', 'N', n, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    gemm<T>(s, 'T', 'N', n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB, sC);
     return;
   }
   const int nt = cdiv(n, BM);
-  hipLaunchKernelGGL((gemm_upper_kernel<T, true, false>), dim3(nt * (nt + 1) / 2), dim3(256), 0, s, n, k, alpha, A,
-                     lda, B, ldb, beta, C, ldc);
+  hipLaunchKernelGGL((gemm_upper_kernel<T, true, false>), dim3(nt * (nt + 1) / 2, batch), dim3(256), 0, s, n, k,
+                     alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
   XTB_LAUNCH_CHECK();
 }
 template void gemm_upper_tn<double>(hipStream_t, int, int, double, const double*, int, const double*, int, double,
-                                    double*, int);
+                                    double*, int, int, int64_t, int64_t, int64_t);
+template void gemm_upper_tn<float>(hipStream_t, int, int, float, const float*, int, const float*, int, float, float*,
+                                   int, int, int64_t, int64_t, int64_t);
 
 template int gemm_splitk<double>(hipStream_t, int, int, int, const double*, int, const double*, int, double*, int);
 template int gemm_splitk<float>(hipStream_t, int, int, int, const float*, int, const float*, int, float*, int);

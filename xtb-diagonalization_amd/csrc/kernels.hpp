@@ -15,7 +15,7 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
 // not written (a syrk-shaped update whose consumer reads the upper triangle).
 template <typename T>
 void gemm_upper_tn(hipStream_t s, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
-                   int ldc);
+                   int ldc, int batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0);
 
 template <typename T>
 int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int kc);

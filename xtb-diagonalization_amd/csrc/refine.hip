@@ -20,14 +20,21 @@ namespace xtbeig {
 
 namespace {
 
-// R <- I - R (R = C^T S C on entry), lambda_i = A_ii / (C^T S C)_ii.
-__global__ void oa_prepare_kernel(int n, double* __restrict__ r, int ldr, const double* __restrict__ a, int lda,
+// R <- I - R (R = C^T S C on entry), lambda_i = A_ii / (C^T S C)_ii. Only the upper triangles of R and A are formed
+// (the products are symmetric): the thread of (i, j), i < j, also writes the mirrored (j, i) of both.
+__global__ void oa_prepare_kernel(int n, double* __restrict__ r, int ldr, double* __restrict__ a, int lda,
                                   double* __restrict__ lam) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= n) return;
+  if (i >= n || i > j) return;
   const double v = r[idx(i, j, ldr)];
-  if (i == j) lam[i] = a[idx(i, i, lda)] / v;
-  r[idx(i, j, ldr)] = (i == j ? 1.0 : 0.0) - v;
+  if (i == j) {
+    lam[i] = a[idx(i, i, lda)] / v;
+    r[idx(i, i, ldr)] = 1.0 - v;
+  } else {
+    r[idx(i, j, ldr)] = -v;
+    r[idx(j, i, ldr)] = -v;
+    a[idx(j, i, lda)] = a[idx(i, j, lda)];
+  }
 }
 
 __global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, const double* __restrict__ a, int lda,
@@ -109,7 +116,7 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   copy_matrix<double, double>(s, n, n, h, ldh, shm + nn, n);
   for (int it = 0; it < iterations; ++it) {
     gemm<double>(s, 'N', 'N', n, n, n, 1.0, shm, n, c, ldc, 0.0, sc, n, 2, nn, 0, nn);  // S C, H C
-    gemm<double>(s, 'T', 'N', n, n, n, 1.0, c, ldc, sc, n, 0.0, r, n, 2, 0, nn, nn);    // C^T S C, C^T H C
+    gemm_upper_tn<double>(s, n, n, 1.0, c, ldc, sc, n, 0.0, r, n, 2, 0, nn, nn);  // C^T S C, C^T H C (upper tiles)
     hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam);
     XTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n);

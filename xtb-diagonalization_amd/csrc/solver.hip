@@ -300,8 +300,8 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
     double* z = c.take<double>(nn);
     StdBuffers<double> s = carve_std<double>(c, n, b);
     gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, n, 0.0, hx, n);
-    gemm<double>(st, 'T', 'N', n, n, n, 1.0, x, n, hx, n, 0.0, hp, n);
-    symmetrize<double>(st, 'L', n, hp, n);
+    gemm_upper_tn<double>(st, n, n, 1.0, x, n, hx, n, 0.0, hp, n);  // X^T H X is symmetric: upper tiles, mirrored
+    symmetrize<double>(st, 'U', n, hp, n);
     syevd_core<double>(h, s, n, b, hp, n, w, z, n);
     gemm<double>(st, 'N', 'N', n, n, n, 1.0, x, n, z, n, 0.0, cm, ldc);
   } else {
@@ -317,8 +317,8 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
     copy_matrix<double, float>(st, n, n, x, n, x32, n);
     copy_matrix<double, float>(st, n, n, hm, ldh, h32, n);
     gemm<float>(st, 'N', 'N', n, n, n, 1.0f, h32, n, x32, n, 0.0f, hx, n);
-    gemm<float>(st, 'T', 'N', n, n, n, 1.0f, x32, n, hx, n, 0.0f, hp, n);
-    symmetrize<float>(st, 'L', n, hp, n);
+    gemm_upper_tn<float>(st, n, n, 1.0f, x32, n, hx, n, 0.0f, hp, n);
+    symmetrize<float>(st, 'U', n, hp, n);
     syevd_core<float>(h, s, n, b, hp, n, w, z, n);
     gemm<float>(st, 'N', 'N', n, n, n, 1.0f, x32, n, z, n, 0.0f, c32, n);
     copy_matrix<float, double>(st, n, n, c32, n, cm, ldc);
