@@ -189,8 +189,11 @@ StdBuffers<T> carve_std(Carver& c, int n, int b) {
 // is formed on the handle's second stream while the chase and the divide and conquer run on the first: Q1 as the band
 // back-transformation of the identity (as soon as the band is there), then M = Q1 Q2 by the bulge reflectors applied
 // from the right (once the chase is done); the first stream then needs one product, z = M Zdc.
+// pre (optional, n x n, ld ldpre): z = pre Q1 Q2 Zdc instead, with P = pre M formed on the second stream into pm
+// (n x n, ld n) right after M, so that the first stream still needs one product after the divide and conquer.
 template <typename T>
-void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, double* w, T* z, int ldz) {
+void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, double* w, T* z, int ldz,
+                const T* pre = nullptr, int ldpre = 0, T* pm = nullptr) {
   hipStream_t st = h.stream, st2 = h.stream2;
   if (n == 1) {
     copy_matrix<T, double>(st, 1, 1, a, lda, w, 1);
@@ -215,6 +218,7 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   // before the divide and conquer so that it starts as soon as the chase ends
   XTB_HIP(hipStreamWaitEvent(st2, h.ev_chase, 0));
   back_transform_bulges<T>(st2, bp, s.v2, s.qt, ldzt, n, true);
+  if (pre) gemm<T>(st2, 'N', 'N', n, n, n, T(1), pre, ldpre, s.qt, ldzt, T(0), pm, n);
   XTB_HIP(hipEventRecord(h.ev_join, st2));
   copy_matrix<T, double>(st, n, 1, s.dT, n, s.d64, n);
   copy_matrix<T, double>(st, n - 1, 1, s.eT, n, s.e64, n);
@@ -229,7 +233,8 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
     copy_matrix<double, T>(st, n, n, s.z64, n, s.zt, n);
     zdc = s.zt;
   }
-  gemm<T>(st, 'N', 'N', n, n, n, T(1), s.qt, ldzt, zdc, n, T(0), z, ldz);
+  if (pre) gemm<T>(st, 'N', 'N', n, n, n, T(1), pm, n, zdc, n, T(0), z, ldz);
+  else gemm<T>(st, 'N', 'N', n, n, n, T(1), s.qt, ldzt, zdc, n, T(0), z, ldz);
 }
 
 template <typename T>
@@ -261,16 +266,14 @@ size_t sygvd_bytes(int n, int b, bool mixed) {
   c.take<double>(nn);  // X
   c.take<double>(potrf_workspace(n));
   if (!mixed) {
-    c.take<double>(nn);  // H X
+    c.take<double>(nn);  // H X, then X Q1 Q2
     c.take<double>(nn);  // H'
-    c.take<double>(nn);  // Z
     carve_std<double>(c, n, b);
   } else {
     c.take<float>(nn);  // X32
     c.take<float>(nn);  // H32
-    c.take<float>(nn);  // H X
+    c.take<float>(nn);  // H X, then X Q1 Q2
     c.take<float>(nn);  // H'
-    c.take<float>(nn);  // Z32
     c.take<float>(nn);  // C32
     carve_std<float>(c, n, b);
     c.take<double>(refine_workspace(n));
@@ -297,19 +300,17 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   if (!mixed) {
     double* hx = c.take<double>(nn);
     double* hp = c.take<double>(nn);
-    double* z = c.take<double>(nn);
     StdBuffers<double> s = carve_std<double>(c, n, b);
     gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, n, 0.0, hx, n);
     gemm_upper_tn<double>(st, n, n, 1.0, x, n, hx, n, 0.0, hp, n);  // X^T H X is symmetric: upper tiles, mirrored
     symmetrize<double>(st, 'U', n, hp, n);
-    syevd_core<double>(h, s, n, b, hp, n, w, z, n);
-    gemm<double>(st, 'N', 'N', n, n, n, 1.0, x, n, z, n, 0.0, cm, ldc);
+    // C = X Z = (X Q1 Q2) Zdc, X Q1 Q2 formed beside the divide and conquer (into hx, free by then)
+    syevd_core<double>(h, s, n, b, hp, n, w, cm, ldc, x, n, hx);
   } else {
     float* x32 = c.take<float>(nn);
     float* h32 = c.take<float>(nn);
     float* hx = c.take<float>(nn);
     float* hp = c.take<float>(nn);
-    float* z = c.take<float>(nn);
     float* c32 = c.take<float>(nn);
     StdBuffers<float> s = carve_std<float>(c, n, b);
     double* rwork = c.take<double>(refine_workspace(n));
@@ -319,8 +320,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
     gemm<float>(st, 'N', 'N', n, n, n, 1.0f, h32, n, x32, n, 0.0f, hx, n);
     gemm_upper_tn<float>(st, n, n, 1.0f, x32, n, hx, n, 0.0f, hp, n);
     symmetrize<float>(st, 'U', n, hp, n);
-    syevd_core<float>(h, s, n, b, hp, n, w, z, n);
-    gemm<float>(st, 'N', 'N', n, n, n, 1.0f, x32, n, z, n, 0.0f, c32, n);
+    syevd_core<float>(h, s, n, b, hp, n, w, c32, n, x32, n, hx);
     copy_matrix<float, double>(st, n, n, c32, n, cm, ldc);
     refine_oa(st, n, hm, ldh, sm, lds, cm, ldc, w, h.refine_iters, rwork, riwork);
   }
