@@ -15,16 +15,20 @@ namespace {
 
 using namespace gemm_detail;
 
+constexpr int GEMM_THREADS = 512;  // eight wavefronts per 64 x 64 tile, two per SIMD
+
 template <typename T, bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A, int lda,
-                                                   int64_t sA, const T* __restrict__ B, int ldb, int64_t sB, T beta,
-                                                   T* __restrict__ C, int ldc, int64_t sC) {
+__global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A,
+                                                            int lda, int64_t sA, const T* __restrict__ B, int ldb,
+                                                            int64_t sB, T beta, T* __restrict__ C, int ldc,
+                                                            int64_t sC) {
   __shared__ T As[BK][BM + PAD];
   __shared__ T Bs[BK][BN + PAD];
   A += blockIdx.z * sA;
   B += blockIdx.z * sB;
   C += blockIdx.z * sC;
-  gemm_tile<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.x * BM, blockIdx.y * BN, As, Bs);
+  gemm_tile<T, TA, TB, GEMM_THREADS>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.x * BM, blockIdx.y * BN,
+                                     As, Bs);
 }
 
 // The tiles (I, J) with I <= J of an n x n C only, one workgroup each (tile q = J (J + 1) / 2 + I).
@@ -49,7 +53,7 @@ template <typename T, bool TA, bool TB>
 void launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, int lda, int64_t sA, const T* B, int ldb,
             int64_t sB, T beta, T* C, int ldc, int64_t sC, int batch) {
   dim3 grid(cdiv(m, BM), cdiv(n, BN), batch);
-  hipLaunchKernelGGL((gemm_kernel<T, TA, TB>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta,
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB>), grid, dim3(GEMM_THREADS), 0, s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta,
                      C, ldc, sC);
   XTB_LAUNCH_CHECK();
 }

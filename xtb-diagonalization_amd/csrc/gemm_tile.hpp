@@ -10,7 +10,6 @@ namespace xtbeig {
 namespace gemm_detail {
 
 constexpr int BM = 64, BN = 64, BK = 32, PAD = 4;
-constexpr int SL = BK / 4;  // elements of each operand a thread loads per slab
 
 template <typename T> struct Mfma;
 // Accumulator layout: register r of lane l holds C(row(l, r), l % 16). The FP32 16x16x4 instruction places rows
@@ -30,92 +29,96 @@ template <> struct Mfma<float> {
   static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-// Loads of one K slab: SL elements of op(A) (64 x BK) and SL of op(B) (BK x 64) per thread, along the contiguous
-// dimension of each operand.
-template <typename T, bool TA, bool TB>
+// Loads of one K slab by NT threads: BK * 64 / NT elements of op(A) (64 x BK) and as many of op(B) (BK x 64) per
+// thread, along the contiguous dimension of each operand.
+template <typename T, bool TA, bool TB, int NT = 256>
 struct Slab {
-  T a[SL], b[SL];
-  // One base pointer per thread and operand, then uniform strides between its SL elements (so that few addresses
-  // stay live across the two slabs in flight).
+  static constexpr int E = BK * 64 / NT;  // elements of each operand per thread
+  static constexpr int RS = NT / 64;      // K step between a thread's elements along a column of 64
+  static constexpr int CS = NT / BK;      // column step between a thread's elements along K-contiguous columns
+  T a[E], b[E];
+  // One base pointer per thread and operand, then uniform strides between its elements (so that few addresses stay
+  // live across the two slabs in flight).
   __device__ inline void load(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, int M, int N,
                               int K, int m0, int n0, int k0, int t) {
     if (!TA) {
-      const int i = t & 63, kq = t >> 6;  // k = kq + 4 r
+      const int i = t & 63, kq = t >> 6;  // k = kq + RS r
       const bool oki = m0 + i < M;
       const T* pa = A + (m0 + i) + (int64_t)(k0 + kq) * lda;
 #pragma unroll
-      for (int r = 0; r < SL; ++r) a[r] = (oki && k0 + kq + 4 * r < K) ? pa[(int64_t)(4 * r) * lda] : T(0);
+      for (int r = 0; r < E; ++r) a[r] = (oki && k0 + kq + RS * r < K) ? pa[(int64_t)(RS * r) * lda] : T(0);
     } else {
-      const int k = t % BK, iq = t / BK;  // i = iq + (256 / BK) r
+      const int k = t % BK, iq = t / BK;  // i = iq + CS r
       const bool okk = k0 + k < K;
       const T* pa = A + (k0 + k) + (int64_t)(m0 + iq) * lda;
 #pragma unroll
-      for (int r = 0; r < SL; ++r)
-        a[r] = (okk && m0 + iq + (256 / BK) * r < M) ? pa[(int64_t)((256 / BK) * r) * lda] : T(0);
+      for (int r = 0; r < E; ++r) a[r] = (okk && m0 + iq + CS * r < M) ? pa[(int64_t)(CS * r) * lda] : T(0);
     }
     if (!TB) {
-      const int k = t % BK, jq = t / BK;  // j = jq + (256 / BK) r
+      const int k = t % BK, jq = t / BK;  // j = jq + CS r
       const bool okk = k0 + k < K;
       const T* pb = B + (k0 + k) + (int64_t)(n0 + jq) * ldb;
 #pragma unroll
-      for (int r = 0; r < SL; ++r)
-        b[r] = (okk && n0 + jq + (256 / BK) * r < N) ? pb[(int64_t)((256 / BK) * r) * ldb] : T(0);
+      for (int r = 0; r < E; ++r) b[r] = (okk && n0 + jq + CS * r < N) ? pb[(int64_t)(CS * r) * ldb] : T(0);
     } else {
-      const int j = t & 63, kq = t >> 6;  // k = kq + 4 r
+      const int j = t & 63, kq = t >> 6;  // k = kq + RS r
       const bool okj = n0 + j < N;
       const T* pb = B + (n0 + j) + (int64_t)(k0 + kq) * ldb;
 #pragma unroll
-      for (int r = 0; r < SL; ++r) b[r] = (okj && k0 + kq + 4 * r < K) ? pb[(int64_t)(4 * r) * ldb] : T(0);
+      for (int r = 0; r < E; ++r) b[r] = (okj && k0 + kq + RS * r < K) ? pb[(int64_t)(RS * r) * ldb] : T(0);
     }
   }
   __device__ inline void store(T (*As)[BM + PAD], T (*Bs)[BN + PAD], int t) const {
 #pragma unroll
-    for (int r = 0; r < SL; ++r) {
+    for (int r = 0; r < E; ++r) {
       int i, k;
-      if (!TA) { i = t & 63; k = (t >> 6) + 4 * r; } else { k = t % BK; i = t / BK + (256 / BK) * r; }
+      if (!TA) { i = t & 63; k = (t >> 6) + RS * r; } else { k = t % BK; i = t / BK + CS * r; }
       As[k][i] = a[r];
     }
 #pragma unroll
-    for (int r = 0; r < SL; ++r) {
+    for (int r = 0; r < E; ++r) {
       int j, k;
-      if (!TB) { k = t % BK; j = t / BK + (256 / BK) * r; } else { j = t & 63; k = (t >> 6) + 4 * r; }
+      if (!TB) { k = t % BK; j = t / BK + CS * r; } else { j = t & 63; k = (t >> 6) + RS * r; }
       Bs[k][j] = b[r];
     }
   }
 };
 
-// One 64 x 64 tile of C = alpha op(A) op(B) + beta C at (m0, n0), by the calling workgroup of 256 threads.
-template <typename T, bool TA, bool TB>
+// One 64 x 64 tile of C = alpha op(A) op(B) + beta C at (m0, n0), by the calling workgroup of NT threads (256: four
+// wavefronts of 32 x 32; 512: eight of 32 x 16, two per SIMD).
+template <typename T, bool TA, bool TB, int NT = 256>
 __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T* __restrict__ A, int lda,
-                                 const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc, int m0,
-                                 int n0, T (*As)[BM + PAD], T (*Bs)[BN + PAD]) {
+                                          const T* __restrict__ B, int ldb, T beta, T* __restrict__ C, int ldc,
+                                          int m0, int n0, T (*As)[BM + PAD], T (*Bs)[BN + PAD]) {
   using acc_t = typename Mfma<T>::acc_t;
+  constexpr int WN = NT / 128;        // wavefront columns
+  constexpr int BJ = 64 / WN / 16;    // 16-column MFMA blocks per wavefront
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int wm = wid & 1, wn = wid >> 1;
-  acc_t acc[2][2];
+  acc_t acc[2][BJ];
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = acc_t{0, 0, 0, 0};
+    for (int bj = 0; bj < BJ; ++bj) acc[bi][bj] = acc_t{0, 0, 0, 0};
 
   auto compute = [&]() {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4), c = lane & 15;
-      T av[2], bv[2];
+      T av[2], bv[BJ];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi) av[bi] = As[kr][wm * 32 + bi * 16 + c];
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) bv[bj] = Bs[kr][wn * 32 + bj * 16 + c];
+      for (int bj = 0; bj < BJ; ++bj) bv[bj] = Bs[kr][wn * (64 / WN) + bj * 16 + c];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = Mfma<T>::op(av[bi], bv[bj], acc[bi][bj]);
+        for (int bj = 0; bj < BJ; ++bj) acc[bi][bj] = Mfma<T>::op(av[bi], bv[bj], acc[bi][bj]);
     }
   };
   // two slabs in flight: the loads of slab s + 2 are issued as slab s goes to LDS, so each has two slabs' compute
   // to arrive
-  Slab<T, TA, TB> s0, s1;
+  Slab<T, TA, TB, NT> s0, s1;
   s0.load(A, lda, B, ldb, M, N, K, m0, n0, 0, t);
   if (BK < K) s1.load(A, lda, B, ldb, M, N, K, m0, n0, BK, t);
   for (int k0 = 0; k0 < K; k0 += 2 * BK) {
@@ -134,8 +137,8 @@ __device__ __forceinline__ void gemm_tile(int M, int N, int K, T alpha, const T*
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      const int col = n0 + wn * 32 + bj * 16 + (lane & 15);
+    for (int bj = 0; bj < BJ; ++bj) {
+      const int col = n0 + wn * (64 / WN) + bj * 16 + (lane & 15);
       if (col >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
