@@ -32,6 +32,25 @@ def test_sygvd_on_golden(torch_cuda, n, mixed):
     w, c, u, info = xtbeig.dsygvd(h, s, mixed=mixed)
     assert info == 0
     check(h, s, w, c, e, cref)
+    # b on exit, as LAPACK's dsygvd leaves it: the Cholesky factor U (S = U^T U) in the upper triangle, the strictly
+    # lower triangle of S untouched
+    uu = np.triu(u)
+    assert np.all(np.diag(uu) > 0)
+    assert np.max(np.abs(uu.T @ uu - s)) < 1e-14 * n * np.linalg.norm(s, 2)
+    assert np.array_equal(np.tril(u, -1), np.tril(s, -1))
+
+
+def test_sygvd_lower_factor_on_golden(torch_cuda):
+    """uplo 'L': the lower triangles are referenced, b returns L = U^T (S = L L^T) with the upper triangle of S
+    untouched."""
+    n = 64
+    h, s, e, cref = hsbin.read(os.path.join(GOLDEN, f"hs_{n}.bin"))
+    w, c, ll, info = xtbeig.dsygvd(h, s, uplo="L")
+    assert info == 0
+    check(h, s, w, c, e, cref)
+    lo = np.tril(ll)
+    assert np.max(np.abs(lo @ lo.T - s)) < 1e-14 * n * np.linalg.norm(s, 2)
+    assert np.array_equal(np.triu(ll, 1), np.triu(s, 1))
 
 
 def test_syevd_on_c0_golden(torch_cuda):

@@ -236,7 +236,7 @@ static void sygvd_host(bool mixed, const int* itype, const char* jobz, const cha
     Handle& h = thread_handle();
     const size_t nn = (size_t)n * n;
     DevBuf dh(sizeof(double) * nn), ds(sizeof(double) * nn), dc(sizeof(double) * nn), dw(sizeof(double) * n),
-        du(sizeof(double) * nn), dx(sizeof(double) * nn), ds2(sizeof(double) * nn), dt(sizeof(T) * nn);
+        dx(sizeof(double) * nn), dt(sizeof(T) * nn);
     const char ul = is_char(uplo, 'U') ? 'U' : 'L';
     auto upload = [&](double* dst, const T* src, int ld) {
       if (std::is_same<T, double>::value) {
@@ -261,22 +261,17 @@ static void sygvd_host(bool mixed, const int* itype, const char* jobz, const cha
     upload(ds.as<double>(), b, ldb);
     xtbeig::symmetrize<double>(h.stream, ul, n, dh.as<double>(), n);
     xtbeig::symmetrize<double>(h.stream, ul, n, ds.as<double>(), n);
+    const double* uf = nullptr;  // the solve's own Cholesky factor of S, in the handle's arena
     xtbeig::sygvd_device(h, n, dh.as<double>(), n, ds.as<double>(), n, dw.as<double>(), dc.as<double>(), n, mixed,
-                         h.band_override);
+                         h.band_override, &uf);
     download(w, n, dw.as<double>(), n, 1, false, false);
     if (wantz) download(a, lda, dc.as<double>(), n, n, false, false);
     read_status(h, st);
     if (st[0] == 0) {
       // b on exit: the Cholesky factor in the referenced triangle (U, or L = U^T), the other triangle untouched,
       // as LAPACK's dsygvd leaves it
-      xtbeig::copy_matrix<double, double>(h.stream, n, n, ds.as<double>(), n, ds2.as<double>(), n);
-      h.reserve(xtbeig::potrf_workspace(n) * sizeof(double) + 4096);
-      xtbeig::potrf_inverse(h.stream, n, ds2.as<double>(), n, du.as<double>(), n, dx.as<double>(), n,
-                            static_cast<double*>(h.arena), h.dinfo);
-      if (ul == 'L') xtbeig::transpose_matrix<double, double>(h.stream, n, n, du.as<double>(), n, dx.as<double>(), n);
-      download(b, ldb, ul == 'U' ? du.as<double>() : dx.as<double>(), n, n, true, ul == 'U');
-      int st2[3];
-      read_status(h, st2);
+      if (ul == 'L') xtbeig::transpose_matrix<double, double>(h.stream, n, n, uf, n, dx.as<double>(), n);
+      download(b, ldb, ul == 'U' ? uf : dx.as<double>(), n, n, true, ul == 'U');
     }
   });
   if (rc) *info = 2 * n + 1;

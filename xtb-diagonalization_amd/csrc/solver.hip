@@ -283,7 +283,7 @@ size_t sygvd_bytes(int n, int b, bool mixed) {
 }
 
 void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
-                  int ldc, bool mixed, int b) {
+                  int ldc, bool mixed, int b, const double** u_out) {
   if (n <= 0) return;
   hipStream_t st = h.stream;
   if (b <= 0) b = h.band_for(n, mixed);
@@ -297,6 +297,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   double* tmp = c.take<double>(potrf_workspace(n));
   copy_matrix<double, double>(st, n, n, sm, lds, scopy, n);
   potrf_inverse(st, n, scopy, n, u, n, x, n, tmp, h.dinfo + 0);
+  if (u_out) *u_out = u;  // no later stage carves over it
   if (!mixed) {
     double* hx = c.take<double>(nn);
     double* hp = c.take<double>(nn);

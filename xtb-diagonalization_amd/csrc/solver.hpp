@@ -81,8 +81,10 @@ int default_band(int n, bool fp32);
 
 template <typename T>
 void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b);
+// u_out (optional): where S's Cholesky factor U (n x n, ld n, zero below the diagonal) stays in the handle's arena
+// until the next call on the handle.
 void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
-                  int ldc, bool mixed, int b);
+                  int ldc, bool mixed, int b, const double** u_out = nullptr);
 void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz);
 
 // batch.hip: the generalized solve through a captured graph (one per (n, precision, band), inputs and outputs
