@@ -206,14 +206,23 @@ size_t potrf_workspace(int n) {
 }
 
 void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu, double* x, int ldx, double* tmp,
-                   int* info) {
+                   int* info, hipStream_t s2, hipEvent_t ev, hipEvent_t ev_join) {
   if (n <= 0) return;
   set_zero(s, n, n, u, ldu);
   set_zero(s, n, n, x, ldx);
+  const bool beside = s2 != nullptr && ev != nullptr && ev_join != nullptr;
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = std::min(NB, n - k0), r = n - k0 - kb;
     hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, n, k0, a, lda, u, ldu, x, ldx, info);
     XTB_LAUNCH_CHECK();
+    if (beside && k0 > 0) {
+      // X(0:k0, block) = -(X(0:k0, 0:k0) U(0:k0, block)) X_kk on s2: U's block column came from the row panels of
+      // the earlier steps, X_kk from this step's diagonal kernel, X(0:k0, 0:k0) from s2's earlier steps
+      XTB_HIP(hipEventRecord(ev, s));
+      XTB_HIP(hipStreamWaitEvent(s2, ev, 0));
+      gemm<double>(s2, 'N', 'N', k0, kb, k0, 1.0, x, ldx, u + idx(0, k0, ldu), ldu, 0.0, tmp, k0);
+      gemm<double>(s2, 'N', 'N', k0, kb, kb, -1.0, tmp, k0, x + idx(k0, k0, ldx), ldx, 0.0, x + idx(0, k0, ldx), ldx);
+    }
     if (r > 0) {
       // U(k0 block, rest) = U_kk^{-T} A(k0 block, rest)
       gemm<double>(s, 'T', 'N', kb, r, kb, 1.0, x + idx(k0, k0, ldx), ldx, a + idx(k0, k0 + kb, lda), lda, 0.0,
@@ -225,7 +234,12 @@ void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu,
                             a + idx(k0 + kb, k0 + kb, lda), lda);
     }
   }
-  inverse_blocks(s, 0, n, u, ldu, x, ldx, tmp);
+  if (beside) {
+    XTB_HIP(hipEventRecord(ev_join, s2));
+    XTB_HIP(hipStreamWaitEvent(s, ev_join, 0));
+  } else {
+    inverse_blocks(s, 0, n, u, ldu, x, ldx, tmp);
+  }
 }
 
 }  // namespace xtbeig

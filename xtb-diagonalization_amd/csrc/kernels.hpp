@@ -40,10 +40,12 @@ void symmetrize_pad(hipStream_t s, int n, const double* src, int lds, double* ds
 // chol.hip ---------------------------------------------------------------------------------------------------------
 // S = U^T U (upper Cholesky) of the symmetric positive definite a (n x n, full storage, destroyed); writes U (zero
 // below the diagonal) and X = U^{-1} (zero below). info (device int): 0, or the LAPACK potrf index j > 0 of the
-// first non-positive pivot. tmp: potrf_workspace(n) doubles.
+// first non-positive pivot. tmp: potrf_workspace(n) doubles. With a second stream s2 and two events, X's block
+// columns are formed on s2 while s factors (left-looking: X(0:k, k) = -X(0:k, 0:k) U(0:k, k) X_kk once block k's
+// diagonal is inverted), and s waits for s2 before it returns; without, X is assembled at the end by halving.
 size_t potrf_workspace(int n);
 void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu, double* x, int ldx, double* tmp,
-                   int* info);
+                   int* info, hipStream_t s2 = nullptr, hipEvent_t ev = nullptr, hipEvent_t ev_join = nullptr);
 
 // band.hip ---------------------------------------------------------------------------------------------------------
 struct BandWork;

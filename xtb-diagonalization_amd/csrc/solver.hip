@@ -296,7 +296,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   double* x = c.take<double>(nn);
   double* tmp = c.take<double>(potrf_workspace(n));
   copy_matrix<double, double>(st, n, n, sm, lds, scopy, n);
-  potrf_inverse(st, n, scopy, n, u, n, x, n, tmp, h.dinfo + 0);
+  potrf_inverse(st, n, scopy, n, u, n, x, n, tmp, h.dinfo + 0, h.stream2, h.ev_band, h.ev_join);
   if (u_out) *u_out = u;  // no later stage carves over it
   if (!mixed) {
     double* hx = c.take<double>(nn);
