@@ -48,7 +48,7 @@ __global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, c
   e[idx(i, j, lde)] = v;
 }
 
-// E_ii from column i of E and R (one workgroup per column; R symmetric).
+// 1 + E_ii from column i of E and R (one workgroup per column; R symmetric).
 __global__ __launch_bounds__(256) void oa_diag_kernel(int n, const double* __restrict__ r, int ldr,
                                                       double* __restrict__ e, int lde) {
   __shared__ double red[16];
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void oa_diag_kernel(int n, const double* __res
   }
   s1 = block_sum(s1, red);
   s2 = block_sum(s2, red);
-  if (t == 0) e[idx(i, i, lde)] = 0.5 * (rc[i] - s1 + 2.0 * s2);
+  if (t == 0) e[idx(i, i, lde)] = 1.0 + 0.5 * (rc[i] - s1 + 2.0 * s2);  // I + E: the update is one product C (I + E)
 }
 
 // rank of lambda_i in ascending order (ties by index)
@@ -114,26 +114,37 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   const dim3 eg(cdiv(n, 256), n);
   copy_matrix<double, double>(s, n, n, sm, lds, shm, n);
   copy_matrix<double, double>(s, n, n, h, ldh, shm + nn, n);
+  // the current C alternates between c (ldc) and cnew (ld n)
+  double* cur = c;
+  int ldcur = ldc;
   for (int it = 0; it < iterations; ++it) {
-    gemm<double>(s, 'N', 'N', n, n, n, 1.0, shm, n, c, ldc, 0.0, sc, n, 2, nn, 0, nn);  // S C, H C
-    gemm_upper_tn<double>(s, n, n, 1.0, c, ldc, sc, n, 0.0, r, n, 2, 0, nn, nn);  // C^T S C, C^T H C (upper tiles)
+    gemm<double>(s, 'N', 'N', n, n, n, 1.0, shm, n, cur, ldcur, 0.0, sc, n, 2, nn, 0, nn);  // S C, H C
+    gemm_upper_tn<double>(s, n, n, 1.0, cur, ldcur, sc, n, 0.0, r, n, 2, 0, nn, nn);  // C^T S C, C^T H C (upper)
     hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam);
     XTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n);
     XTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(oa_diag_kernel, dim3(n), dim3(256), 0, s, n, r, n, e, n);
     XTB_LAUNCH_CHECK();
-    copy_matrix<double, double>(s, n, n, c, ldc, cnew, n);
-    gemm<double>(s, 'N', 'N', n, n, n, 1.0, c, ldc, e, n, 1.0, cnew, n);    // C + C E
-    copy_matrix<double, double>(s, n, n, cnew, n, c, ldc);
+    double* nxt = (cur == c) ? cnew : c;
+    const int ldnxt = (cur == c) ? n : ldc;
+    gemm<double>(s, 'N', 'N', n, n, n, 1.0, cur, ldcur, e, n, 0.0, nxt, ldnxt);  // C (I + E)
+    cur = nxt;
+    ldcur = ldnxt;
   }
-  // sort
+  // sort: columns of the current C scattered into the other buffer in ascending order of lambda, ending in c
   hipLaunchKernelGGL(rank_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, lam, iwork);
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, n, iwork, c, ldc,
-                     cnew, n, lam, w);
-  XTB_LAUNCH_CHECK();
-  copy_matrix<double, double>(s, n, n, cnew, n, c, ldc);
+  if (cur == cnew) {
+    hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, n, iwork, cnew, n,
+                       c, ldc, lam, w);
+    XTB_LAUNCH_CHECK();
+  } else {
+    hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, n, iwork, c, ldc,
+                       cnew, n, lam, w);
+    XTB_LAUNCH_CHECK();
+    copy_matrix<double, double>(s, n, n, cnew, n, c, ldc);
+  }
 }
 
 }  // namespace xtbeig
