@@ -165,14 +165,6 @@ def components(torch, np, workloads, xtbeig, hnd, h, s):
                 ts.append(e0.elapsed_time(e1))
         return float(np.median(ts))
 
-    def host_time(call, reps=3):
-        ts = []
-        for _ in range(reps + 1):
-            t0 = time.perf_counter()
-            call()
-            ts.append(time.perf_counter() - t0)
-        return float(np.median(ts[1:])) * 1e3
-
     def emit(row, name, ms, dtype, note):
         print(json.dumps({"component": name, "path_row": row, "n": n, "ms": round(ms, 4), "dtype": dtype,
                           "timing": note}), flush=True)
@@ -223,30 +215,73 @@ def components(torch, np, workloads, xtbeig, hnd, h, s):
         ms = dev_time(lambda: None, lambda: hnd.gemm("N", "N", n, n, n, 1.0, Ga, n, Gb, n, 0.0, Gc, n))
         emit("a11/a13/a15", f"gemm NN {n}^3 ({2 * n ** 3 / ms / 1e9:.1f} TF/s)", ms, tag,
              "HIP events, device-resident")
-    # host-array entry points (the sytrd.c symbol set and LAPACK-style drivers), transfers included
-    for dtype, tag in ((np.float64, "f64"), (np.float32, "f32")):
-        res = {}
-        ms = host_time(lambda: res.update(r=xtbeig.sytrd_upper(a, dtype=dtype)))
-        pre = "d" if tag == "f64" else "s"
+    # host-array entry points (the sytrd.c symbol set and LAPACK-style drivers), transfers included: the C calls
+    # alone, on caller arrays allocated once (inputs restored outside the timed region)
+    import ctypes
+    L = xtbeig.lib()
+    I, P = xtbeig._i, xtbeig._ptr
+
+    def host_call(restore, call, reps=3):
+        ts = []
+        for _ in range(reps + 1):
+            restore()
+            t0 = time.perf_counter()
+            call()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts[1:])) * 1e3
+
+    for dtype, pre in ((np.float64, "d"), (np.float32, "s")):
+        tag = "f64" if pre == "d" else "f32"
+        a0 = np.asfortranarray(a, dtype=dtype)
+        A_ = a0.copy(order="F")
+        d_, e_, tau_ = (np.zeros(n, dtype=dtype) for _ in range(3))
+        work = np.zeros(32 * n, dtype=dtype)
+        ms = host_call(lambda: np.copyto(A_, a0),
+                       lambda: getattr(L, f"xtb_{pre}sytrd_upper")(I(n), P(A_), I(n), P(d_), P(e_), P(tau_), P(work)))
         emit("a1-a4", f"xtb_{pre}sytrd_upper (one-stage reduction)", ms, tag, "wall clock, host arrays")
-        ra, rd, re_, rtau = res["r"]
-        ms = host_time(lambda: xtbeig.stedc(rd, re_, dtype=dtype))
-        emit("a5", "xtb_{}stedc (host arrays)".format("d" if tag == "f64" else "s"), ms, tag,
-             "wall clock, host arrays")
-        zi = np.eye(n, dtype=dtype)
-        ms = host_time(lambda: xtbeig.ormtr_upper(ra, rtau, zi, dtype=dtype))
-        emit("a6", "xtb_{}ormtr_upper (m = n)".format("d" if tag == "f64" else "s"), ms, tag,
-             "wall clock, host arrays")
-    ms = host_time(lambda: xtbeig.sytrd_upper(a, full=True))
+        d0, e0 = d_.copy(), e_.copy()
+        Zt = np.zeros((n, n), dtype=dtype, order="F")
+        lw, liw = 1 + 4 * n + n * n, 3 + 5 * n
+        wk, iwk, info = np.zeros(lw, dtype=dtype), np.zeros(liw, dtype=np.int32), ctypes.c_int(0)
+        ms = host_call(lambda: (np.copyto(d_, d0), np.copyto(e_, e0)),
+                       lambda: getattr(L, f"xtb_{pre}stedc")(I(n), P(d_), P(e_), P(Zt), I(n), P(wk), I(lw), P(iwk),
+                                                            I(liw), ctypes.byref(info)))
+        emit("a5", f"xtb_{pre}stedc (host arrays)", ms, tag, "wall clock, host arrays")
+        Z0 = np.asfortranarray(np.eye(n, dtype=dtype))
+        Z_ = Z0.copy(order="F")
+        wk2 = np.zeros(32 * (4 * n + 32), dtype=dtype)
+        ms = host_call(lambda: np.copyto(Z_, Z0),
+                       lambda: getattr(L, f"xtb_{pre}ormtr_upper")(I(n), I(n), P(A_), I(n), P(tau_), P(Z_), I(n),
+                                                                  P(wk2)))
+        emit("a6", f"xtb_{pre}ormtr_upper (m = n)", ms, tag, "wall clock, host arrays")
+    a0 = np.asfortranarray(a, dtype=np.float64)
+    A_ = a0.copy(order="F")
+    d_, e_, tau_ = (np.zeros(n) for _ in range(3))
+    work = np.zeros(32 * n)
+    ms = host_call(lambda: np.copyto(A_, a0),
+                   lambda: L.xtb_dsytrd_full(I(n), P(A_), I(n), P(d_), P(e_), P(tau_), P(work)))
     emit("a14", "xtb_dsytrd_full (full padded copy)", ms, "f64", "wall clock, host arrays")
-    ms = host_time(lambda: xtbeig.dsymmetrize_pad(a, ldd=n + 8))
+    Dp = np.zeros((n + 8, n), order="F")
+    ms = host_call(lambda: None, lambda: L.xtb_dsymmetrize_pad(I(n), P(a0), I(n), P(Dp), I(n + 8)))
     emit("a14", "xtb_dsymmetrize_pad", ms, "f64", "wall clock, host arrays")
-    ms = host_time(lambda: xtbeig.syevd(a))
+    lw, liw = 1 + 6 * n + 2 * n * n, 3 + 5 * n
+    wk, iwk, info = np.zeros(lw), np.zeros(liw, dtype=np.int32), ctypes.c_int(0)
+    w_ = np.zeros(n)
+    ms = host_call(lambda: np.copyto(A_, a0),
+                   lambda: L.xtbeig_dsyevd(b"V", b"U", I(n), P(A_), I(n), P(w_), P(wk), I(lw), P(iwk), I(liw),
+                                           ctypes.byref(info)))
     emit("a12", "xtbeig_dsyevd (host arrays)", ms, "f64", "wall clock, host arrays")
-    ms = host_time(lambda: xtbeig.dsygvd(h, s))
-    emit("a13", "xtbeig_dsygvd (host arrays)", ms, "f64", "wall clock, host arrays")
-    ms = host_time(lambda: xtbeig.dsygvd(h, s, mixed=True))
-    emit("a13/a15", "xtbeig_dsygvd_mixed (host arrays, FP32 + refinement)", ms, "f32+f64", "wall clock, host arrays")
+    h0, s0 = np.asfortranarray(h), np.asfortranarray(s)
+    H_, S_ = h0.copy(order="F"), s0.copy(order="F")
+    for fn, row, name, dt in (("xtbeig_dsygvd", "a13", "xtbeig_dsygvd (host arrays)", "f64"),
+                              ("xtbeig_dsygvd_mixed", "a13/a15", "xtbeig_dsygvd_mixed (host arrays, FP32 + refinement)",
+                               "f32+f64")):
+        ms = host_call(lambda: (np.copyto(H_, h0), np.copyto(S_, s0)),
+                       lambda: getattr(L, fn)(I(1), b"V", b"U", I(n), P(H_), I(n), P(S_), I(n), P(w_), P(wk), I(lw),
+                                              P(iwk), I(liw), ctypes.byref(info)))
+        if info.value != 0:
+            raise RuntimeError(f"{fn} info {info.value}")
+        emit(row, name, ms, dt, "wall clock, host arrays")
 
 
 def main():
