@@ -184,10 +184,8 @@ void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, 
 }
 
 void autotune_bands(Handle& h, bool mixed, int nmin, int nmax, int* buckets_out, int* bands_out, float* ms_out) {
-  const int cand64[] = {8, 16, 32};
-  const int cand32[] = {16, 32};  // FP32 b = 8 leaves C^T S C - I above the gate after two refinement steps
-  const int* cand = mixed ? cand32 : cand64;
-  const int ncand = mixed ? 2 : 3;
+  const int cand[] = {8, 16, 32};
+  const int ncand = 3;
   const int keep_override = h.band_override;
   int idx_out = 0;
   for (int q = 0; q < Handle::NBUCKET; ++q) {
