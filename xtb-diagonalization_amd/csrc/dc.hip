@@ -471,7 +471,7 @@ __global__ void q2_gather_kernel(const Node* nodes, const NodeOut* out, const in
 }
 
 // One lane per root. S (the K x K eigenvector matrix of the rank-one problem, ld sz) is written here only for K <= 2.
-constexpr int SEC_G = 8;  // lanes per secular root: each sums every SEC_G-th pole
+constexpr int SEC_G = 16;  // lanes per secular root: each sums every SEC_G-th pole
 
 __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const NodeOut* out, const double* rho_lvl,
                                                      const double* __restrict__ dlamda_all,
@@ -532,9 +532,11 @@ __device__ inline double group_prod(double v) {
   if (G >= 2) v *= dpp_mov<DPP_XOR1>(v);
   if (G >= 4) v *= dpp_mov<DPP_XOR2>(v);
   if (G >= 8) v *= dpp_mov<DPP_HALF_MIRROR>(v);
+  if (G >= 16) v *= dpp_mov<DPP_MIRROR>(v);
+  static_assert(G <= 16, "group_prod: groups within a row of 16 lanes");
   return v;
 }
-constexpr int GU_G = 8;  // lanes per z_i: each multiplies every GU_G-th factor
+constexpr int GU_G = 16;  // lanes per z_i: each multiplies every GU_G-th factor
 
 __global__ void gu_kernel(const Node* nodes, const NodeOut* out, const double* __restrict__ dlamda_all,
                           const double* __restrict__ w_all, const int* __restrict__ orig_all,
