@@ -471,7 +471,7 @@ __global__ void q2_gather_kernel(const Node* nodes, const NodeOut* out, const in
 }
 
 // One lane per root. S (the K x K eigenvector matrix of the rank-one problem, ld sz) is written here only for K <= 2.
-constexpr int SEC_G = 16;  // lanes per secular root: each sums every SEC_G-th pole
+constexpr int SEC_G = 32;  // lanes per secular root: each sums every SEC_G-th pole
 
 __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const NodeOut* out, const double* rho_lvl,
                                                      const double* __restrict__ dlamda_all,
