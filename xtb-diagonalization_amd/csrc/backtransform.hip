@@ -29,7 +29,7 @@ namespace xtbeig {
 
 namespace {
 
-constexpr int BULGE_G = 32;     // sweeps per bulge block
+constexpr int BULGE_G = 16;     // sweeps per bulge block
 constexpr int BULGE_WAVES = 8;  // wavefronts per workgroup
 constexpr int GROUP_K = 64;     // reflectors per merged panel block
 constexpr int SPLIT_KC = 128;   // rows per chunk of the split product W = zt V
