@@ -268,24 +268,37 @@ __device__ inline T part_sum(T v) {
 // with zeros (band_elems(n, b) covers b extra columns, and the kernel clears every stored element outside the matrix
 // before the chase), so a bulge at the end of the band takes the same branch-free path as any other: the rows past n
 // give x = 0, w = 0 there and leave the zeros in place.
+// In band storage A(rs + a, rs + c) sits at rs (ld + 1) + a + ld c = rs 2b + a + ld c, so every element a bulge
+// touches is at rs 2b plus a constant of the lane: one set for the first bulge of a sweep (whose left block is the
+// single column s = rs - 1) and one for the others. The kernel forms them once.
+struct LaneOffsets {
+  int x0, x1;  // x = A[R, c]: first bulge, others
+  int l0, l1;  // the lane's left block column
+  int bl, dr, dc;
+};
+
+template <int B, int H>
+__device__ inline LaneOffsets lane_offsets(int cl, int h) {
+  constexpr int U = B / H, LD = 2 * B - 1;
+  const int u0 = h * U;
+  return {-LD, -LD * B, u0 - LD, u0 + LD * (cl - B), B + cl + LD * u0, cl + LD * u0, u0 + LD * cl};
+}
+
 template <typename T, int B, int H>
-__device__ __forceinline__ bool chase_bulge_lanes(int n, const Band<T>& A, T* __restrict__ v2, int ldv2, int s, int k,
-                                                  int cl, int h, T* __restrict__ qs) {
+__device__ __forceinline__ bool chase_bulge_lanes(int n, T* __restrict__ ab, T* __restrict__ v2s, int rs, bool first,
+                                                  int cl, int h, const LaneOffsets& o, T* __restrict__ qs) {
   constexpr int U = B / H;
   constexpr int LD = 2 * B - 1;
-  const int rs = s + 1 + k * B;
   if (rs >= n) return false;
-  const int re = rs + B;
-  const int c = (k == 0) ? s : rs - B;
   // left block: columns rs - B ..< rs; for the first bulge of a sweep only column s is nonzero (the columns before it
   // are already tridiagonal), and every lane reads that column
-  const int lcol = (k == 0) ? s : c + cl;
   const int u0 = h * U;
-  T* const xp = &A(rs, c);                // x = A[R, c] at xp[u]
-  T* const lp = &A(rs + u0, lcol);        // left block column at lp[j]
-  T* const bp = &A(re + cl, rs + u0);     // block-below row at bp[j LD]
-  T* const dr = &A(rs + cl, rs + u0);     // diagonal block row rs + cl, columns rs + u <= rs + cl, at dr[j LD]
-  T* const dc = &A(rs + u0, rs + cl);     // diagonal block column rs + cl, rows rs + u >= rs + cl, at dc[j]
+  T* const pos = ab + rs * (2 * B);        // A(rs, rs)
+  T* const xp = pos + (first ? o.x0 : o.x1);  // x = A[R, c] at xp[u]
+  T* const lp = pos + (first ? o.l0 : o.l1);  // left block column at lp[j]
+  T* const bp = pos + o.bl;               // block-below row re + cl, columns rs + u, at bp[j LD]
+  T* const dr = pos + o.dr;               // diagonal block row rs + cl, columns rs + u <= rs + cl, at dr[j LD]
+  T* const dc = pos + o.dc;               // diagonal block column rs + cl, rows rs + u >= rs + cl, at dc[j]
   T L[U], Bl[U], D[U], w[U];
 #pragma unroll
   for (int j = 0; j < U; ++j) {
@@ -307,7 +320,7 @@ __device__ __forceinline__ bool chase_bulge_lanes(int n, const Band<T>& A, T* __
 #pragma unroll
   for (int j = 0; j < U; ++j) w[j] = (j == 0 && u0 == 0) ? sq : s2 * w[j];
   const T wc = (cl == 0) ? sq : s2 * xc;
-  if (h == 0 && rs + cl < n) v2[(int64_t)(rs + cl) + (int64_t)ldv2 * s] = wc;
+  if (h == 0 && rs + cl < n) v2s[rs + cl] = wc;
   T dl = 0, y = 0, p = 0;
 #pragma unroll
   for (int j = 0; j < U; ++j) {
@@ -332,7 +345,7 @@ __device__ __forceinline__ bool chase_bulge_lanes(int n, const Band<T>& A, T* __
 #pragma unroll
   for (int j = 0; j < U; ++j) D[j] -= wc * qs[u0 + j] + qc * w[j];
   __builtin_amdgcn_wave_barrier();
-  if (k > 0 || cl == 0) {
+  if (!first || cl == 0) {
 #pragma unroll
     for (int j = 0; j < U; ++j) lp[j] = (cl == 0) ? ((j == 0 && u0 == 0) ? beta : T(0)) : L[j];
   }
@@ -389,21 +402,32 @@ __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __res
   const int sweeps = max(0, n - 2);
   const int KB = (n + B - 1) / B + 8;  // progress code: s * KB + bulges done (monotonic per group)
   const int pg = (g + NG - 1) % NG;    // the group of sweep s - 1
+  const LaneOffsets off = lane_offsets<B, H>(cl, h);
   bool failed = false;
-  int s = g, k = 0;
+  // per sweep: s, its first row rs = s + 1 + k B, the progress code s KB + k this group publishes after bulge k - 1,
+  // the code (s - 1) KB + k + 3 it waits for before bulge k, and v2's column s
+  int s = g, k = 0, rs = s + 1, code = s * KB, wait = code - KB + 3;
+  T* v2s = v2 + (int64_t)ldv2 * s;
   long spins = 0;
   while (__any(s < sweeps)) {
     bool ready = s < sweeps;
-    if (ready && s > 0) ready = ld_acquire(&prog[pg]) >= (s - 1) * KB + (k + 3);
+    if (ready && s > 0) ready = ld_acquire(&prog[pg]) >= wait;
     if (ready) {
       spins = 0;
-      if (chase_bulge_lanes<T, B, H>(n, A, v2, ldv2, s, k, cl, h, qs)) {
+      if (chase_bulge_lanes<T, B, H>(n, ab, v2s, rs, k == 0, cl, h, off, qs)) {
         ++k;
-        if (gl == 0) st_release(&prog[g], s * KB + k);
+        ++code;
+        ++wait;
+        rs += B;
+        if (gl == 0) st_release(&prog[g], code);
       } else {
-        if (gl == 0) st_release(&prog[g], s * KB + (KB - 1));
+        if (gl == 0) st_release(&prog[g], code - k + (KB - 1));
         s += NG;
         k = 0;
+        rs = s + 1;
+        code = s * KB;
+        wait = code - KB + 3;
+        v2s = v2 + (int64_t)ldv2 * s;
       }
     } else if (s < sweeps && ++spins > (1L << 24)) {
       failed = true;
