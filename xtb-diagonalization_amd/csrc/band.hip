@@ -412,17 +412,37 @@ __global__ __launch_bounds__(256) void panel_xfin_kernel(int m, int k, const T* 
   __shared__ T Vr[64][17];
   __shared__ T Ts[16][17];
   const int t = threadIdx.x, rb = blockIdx.x, i0 = rb * 64;
+  // the thread's four elements: the partials of all four are loaded a chunk of splits at a time, every load of a
+  // chunk in flight at once, and summed in split order
+  constexpr int CH = 8;
+  T sum[4];
+  size_t off[4];
+  bool ok[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int e = t + 256 * q, r = e % 64, c = e / 64;
-    const bool ok = i0 + r < m && c < k;
-    T sum = 0;
-    if (ok) {
-#pragma unroll 4
-      for (int s2 = 0; s2 < nks; ++s2) sum += xpart[(size_t)s2 * m * k + (i0 + r) + (size_t)c * m];
-    }
-    Xs[r][c] = sum;
-    Vr[r][c] = ok ? v[idx(i0 + r, c, ldv)] : T(0);
+    ok[q] = i0 + r < m && c < k;
+    off[q] = (size_t)(i0 + r) + (size_t)c * m;
+    sum[q] = 0;
+  }
+  const size_t split = (size_t)m * k;
+  for (int s0 = 0; s0 < nks; s0 += CH) {
+    T pv[4][CH];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int j = 0; j < CH; ++j) pv[q][j] = (ok[q] && s0 + j < nks) ? xpart[(size_t)(s0 + j) * split + off[q]] : T(0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int j = 0; j < CH; ++j)
+        if (s0 + j < nks) sum[q] += pv[q][j];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q, r = e % 64, c = e / 64;
+    Xs[r][c] = sum[q];
+    Vr[r][c] = ok[q] ? v[idx(i0 + r, c, ldv)] : T(0);
   }
   {
     const int r = t % 16, c = t / 16;
@@ -447,10 +467,10 @@ __global__ __launch_bounds__(256) void panel_xfin_kernel(int m, int k, const T* 
   }
   __syncthreads();
   const int a = t % 16, c = t / 16;
-  T sum = 0;
+  T vx = 0;
 #pragma unroll 8
-  for (int r = 0; r < 64; ++r) sum += Vr[r][a] * Xs[r][c];
-  ypart[(size_t)rb * 256 + t] = sum;  // (V^T X)(a, c) over this block's rows
+  for (int r = 0; r < 64; ++r) vx += Vr[r][a] * Xs[r][c];
+  ypart[(size_t)rb * 256 + t] = vx;  // (V^T X)(a, c) over this block's rows
 }
 
 // A2 -= V W^T + W V^T with W = X - V Y / 2 and Y = T^T (V^T X) (the row blocks' parts of V^T X summed in block
@@ -473,9 +493,16 @@ __device__ __forceinline__ void form_y(int k, const T* __restrict__ ypart, int n
   const int t = threadIdx.x;
   {
     const int r = t % 16, c = t / 16;
+    // the row blocks' parts summed in block order, up to 32 loads in flight at once
     T sum = 0;
-#pragma unroll 8
-    for (int q = 0; q < nrb; ++q) sum += ypart[(size_t)q * 256 + t];
+    for (int q0 = 0; q0 < nrb; q0 += 32) {
+      T pv[32];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) pv[j] = (q0 + j < nrb) ? ypart[(size_t)(q0 + j) * 256 + t] : T(0);
+#pragma unroll
+      for (int j = 0; j < 32; ++j)
+        if (q0 + j < nrb) sum += pv[j];
+    }
     L.Ss[r][c] = sum;
     L.Ts[r][c] = (r < k && c < k && r <= c) ? tt[r + (size_t)c * ldt] : T(0);
   }
