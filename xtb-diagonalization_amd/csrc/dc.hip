@@ -96,6 +96,35 @@ __global__ void unscale_kernel(int n, double* d, const double* scale) {
   if (i < n) d[i] *= *scale;
 }
 
+// Exclusive scan of one int per thread over a 256-thread workgroup in thread order: sums (MAX false, identity 0) or
+// maxima (MAX true, identity -1). Within a wavefront by lane shifts, then the four wavefront totals through part
+// (LDS, 4 ints); *total gets the whole workgroup's. Every thread calls it (two barriers).
+template <bool MAX>
+__device__ inline int block_exclusive_scan(int v, int* part, int* total = nullptr) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int ID = MAX ? -1 : 0;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x = MAX ? max(x, y) : x + y;
+  }
+  if (lane == 63) part[wid] = x;
+  __syncthreads();
+  int pre = ID, tot = ID;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int pw = part[w];
+    if (w < wid) pre = MAX ? max(pre, pw) : pre + pw;
+    tot = MAX ? max(tot, pw) : tot + pw;
+  }
+  int ex = __shfl_up(x, 1, 64);
+  if (lane == 0) ex = ID;
+  if (total != nullptr) *total = tot;
+  __syncthreads();  // part free again
+  return MAX ? max(pre, ex) : pre + ex;
+}
+
 // dlaed2 for one merge (one workgroup). LDS: d, z, dlamda, w (doubles) and indxq, indx, indxc, indxp, coltyp (ints).
 __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const int* node_pre_index,
                                                       double* __restrict__ d_all, const double* __restrict__ q_all,
@@ -124,7 +153,7 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
   __shared__ int sh_k, sh_nrot, sh_ctot[4], sh_early, sh_serial;
   __shared__ double sh_rho, sh_tol;
   __shared__ double red_v[256];
-  __shared__ int red_i[256], scan[4][257];
+  __shared__ int red_i[256], part[4];
   const int t = threadIdx.x, nt = blockDim.x;
   const double* q = q_all + idx(off, off, ldq);
   const double rho_in = rho_pre[node_pre_index[blockIdx.x]];
@@ -219,18 +248,8 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
       SMALL[j] = sm;
       if (!sm) last = j;
     }
-    scan[0][t] = last;
-    __syncthreads();
-    if (t == 0) {
-      int run = -1;
-      for (int u = 0; u < nt; ++u) {
-        const int v = scan[0][u];
-        scan[0][u] = run;  // exclusive: the last large-z position before chunk u
-        if (v >= 0) run = v;
-      }
-    }
-    __syncthreads();
-    int prev = scan[0][t];
+    // the last large-z position before this chunk (positions grow with the chunk: the maximum of the earlier ones)
+    int prev = block_exclusive_scan<true>(last, part);
     bool close = false;
     for (int j = j0; j < j1; ++j) {
       if (!SMALL[j]) {
@@ -325,19 +344,9 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
     const int per = (n + nt - 1) / nt, j0 = min(n, t * per), j1 = min(n, j0 + per);
     int nl = 0;
     for (int j = j0; j < j1; ++j) nl += !SMALL[j];
-    scan[1][t] = nl;
-    __syncthreads();
-    if (t == 0) {
-      int run = 0;
-      for (int u = 0; u < nt; ++u) {
-        const int v = scan[1][u];
-        scan[1][u] = run;
-        run += v;
-      }
-      sh_nrot = 0;
-    }
-    __syncthreads();
-    int kl = scan[1][t], ks = j0 - scan[1][t];  // large / small met before this chunk
+    const int lb = block_exclusive_scan<false>(nl, part);
+    if (t == 0) sh_nrot = 0;
+    int kl = lb, ks = j0 - lb;  // large / small met before this chunk
     for (int j = j0; j < j1; ++j) {
       const int nj = INDX[j];
       if (!SMALL[j]) {
@@ -361,22 +370,12 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
     const int per = (n + nt - 1) / nt, j0 = min(n, t * per), j1 = min(n, j0 + per);
     int cnt4[4] = {0, 0, 0, 0};
     for (int j = j0; j < j1; ++j) cnt4[COLTYP[INDXP[j] - 1] - 1]++;
-    for (int c = 0; c < 4; ++c) scan[c][t] = cnt4[c];
-    __syncthreads();
-    if (t < 4) {
-      int run = 0;
-      for (int u = 0; u < nt; ++u) {
-        const int v = scan[t][u];
-        scan[t][u] = run;
-        run += v;
-      }
-      scan[t][nt] = run;
-    }
-    __syncthreads();
-    const int c0 = scan[0][nt], c1 = scan[1][nt], c2 = scan[2][nt];
-    const int base[4] = {0, c0, c0 + c1, c0 + c1 + c2};
+    int ex4[4], tot4[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ex4[c] = block_exclusive_scan<false>(cnt4[c], part, &tot4[c]);
+    const int base[4] = {0, tot4[0], tot4[0] + tot4[1], tot4[0] + tot4[1] + tot4[2]};
     int pos[4];
-    for (int c = 0; c < 4; ++c) pos[c] = base[c] + scan[c][t];
+    for (int c = 0; c < 4; ++c) pos[c] = base[c] + ex4[c];
     for (int j = j0; j < j1; ++j) {
       const int js = INDXP[j], ct = COLTYP[js - 1] - 1;
       INDX[pos[ct]] = js;
@@ -384,8 +383,8 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
       pos[ct]++;
     }
     if (t == 0) {
-      for (int c = 0; c < 4; ++c) sh_ctot[c] = scan[c][nt];
-      sh_k = n - scan[3][nt];
+      for (int c = 0; c < 4; ++c) sh_ctot[c] = tot4[c];
+      sh_k = n - tot4[3];
     }
     __syncthreads();
   }
