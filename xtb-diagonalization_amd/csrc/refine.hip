@@ -21,19 +21,36 @@ namespace xtbeig {
 namespace {
 
 // R <- I - R (R = C^T S C on entry), lambda_i = A_ii / (C^T S C)_ii. Only the upper triangles of R and A are formed
-// (the products are symmetric): the thread of (i, j), i < j, also writes the mirrored (j, i) of both.
-__global__ void oa_prepare_kernel(int n, double* __restrict__ r, int ldr, double* __restrict__ a, int lda,
-                                  double* __restrict__ lam) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= n || i > j) return;
-  const double v = r[idx(i, j, ldr)];
-  if (i == j) {
-    lam[i] = a[idx(i, i, lda)] / v;
-    r[idx(i, i, ldr)] = 1.0 - v;
-  } else {
-    r[idx(i, j, ldr)] = -v;
-    r[idx(j, i, ldr)] = -v;
-    a[idx(j, i, lda)] = a[idx(i, j, lda)];
+// (the products are symmetric), so each lower element (j, i) is set from (i, j). One workgroup per 64 x 64 tile
+// (I, J), I <= J, of the upper triangle: the tile is read and written down its columns, and its mirror image, staged
+// transposed in LDS, is written down the columns of tile (J, I).
+__global__ __launch_bounds__(256) void oa_prepare_kernel(int n, double* __restrict__ r, int ldr,
+                                                         double* __restrict__ a, int lda, double* __restrict__ lam) {
+  const int I = blockIdx.x, J = blockIdx.y;
+  if (I > J) return;
+  __shared__ double tr[64][65], ta[64][65];
+  const int t = threadIdx.x, i0 = I * 64, j0 = J * 64;
+#pragma unroll 4
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int ii = e % 64, jj = e / 64, i = i0 + ii, j = j0 + jj;
+    if (i >= n || j >= n || i > j) continue;
+    const double v = r[idx(i, j, ldr)];
+    if (i == j) {
+      lam[i] = a[idx(i, i, lda)] / v;
+      r[idx(i, i, ldr)] = 1.0 - v;
+    } else {
+      r[idx(i, j, ldr)] = -v;
+      tr[jj][ii] = -v;
+      ta[jj][ii] = a[idx(i, j, lda)];
+    }
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int jj = e % 64, ii = e / 64, i = i0 + ii, j = j0 + jj;  // lower element (j, i) = upper (i, j)
+    if (i >= n || j >= n || i >= j) continue;
+    r[idx(j, i, ldr)] = tr[jj][ii];
+    a[idx(j, i, lda)] = ta[jj][ii];
   }
 }
 
@@ -67,17 +84,20 @@ __global__ __launch_bounds__(256) void oa_diag_kernel(int n, const double* __res
   if (t == 0) e[idx(i, i, lde)] = 1.0 + 0.5 * (rc[i] - s1 + 2.0 * s2);  // I + E: the update is one product C (I + E)
 }
 
-// rank of lambda_i in ascending order (ties by index)
-__global__ void rank_kernel(int n, const double* __restrict__ lam, int* __restrict__ rank) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// rank of lambda_i in ascending order (ties by index): RANK_G lanes per i, each counting every RANK_G-th j
+constexpr int RANK_G = 16;
+__global__ __launch_bounds__(256) void rank_kernel(int n, const double* __restrict__ lam, int* __restrict__ rank) {
+  const int g = (blockIdx.x * blockDim.x + threadIdx.x) / RANK_G, gl = threadIdx.x % RANK_G;
+  const int i = min(g, n - 1);  // lanes past n count for a real i and store nothing
   const double li = lam[i];
   int r = 0;
-  for (int j = 0; j < n; ++j) {
+  for (int j = gl; j < n; j += RANK_G) {
     const double lj = lam[j];
     r += (lj < li) || (lj == li && j < i);
   }
-  rank[i] = r;
+#pragma unroll
+  for (int m = RANK_G / 2; m > 0; m >>= 1) r += __shfl_xor(r, m, RANK_G);
+  if (g < n && gl == 0) rank[g] = r;
 }
 
 // dst(:, rank[j]) = src(:, j), w[rank[j]] = lam[j]
@@ -112,6 +132,7 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   double* shm = cnew + nn;  // [S | H], each n x n (ld n): the paired products run as one batched launch
   double* lam = shm + 2 * nn;
   const dim3 eg(cdiv(n, 256), n);
+  const dim3 tg(cdiv(n, 64), cdiv(n, 64));
   copy_matrix<double, double>(s, n, n, sm, lds, shm, n);
   copy_matrix<double, double>(s, n, n, h, ldh, shm + nn, n);
   // the current C alternates between c (ldc) and cnew (ld n)
@@ -120,7 +141,7 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   for (int it = 0; it < iterations; ++it) {
     gemm<double>(s, 'N', 'N', n, n, n, 1.0, shm, n, cur, ldcur, 0.0, sc, n, 2, nn, 0, nn);  // S C, H C
     gemm_upper_tn<double>(s, n, n, 1.0, cur, ldcur, sc, n, 0.0, r, n, 2, 0, nn, nn);  // C^T S C, C^T H C (upper)
-    hipLaunchKernelGGL(oa_prepare_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam);
+    hipLaunchKernelGGL(oa_prepare_kernel, tg, dim3(256), 0, s, n, r, n, a, n, lam);
     XTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n);
     XTB_LAUNCH_CHECK();
@@ -133,7 +154,7 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
     ldcur = ldnxt;
   }
   // sort: columns of the current C scattered into the other buffer in ascending order of lambda, ending in c
-  hipLaunchKernelGGL(rank_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, lam, iwork);
+  hipLaunchKernelGGL(rank_kernel, dim3(cdiv(n * RANK_G, 256)), dim3(256), 0, s, n, lam, iwork);
   XTB_LAUNCH_CHECK();
   if (cur == cnew) {
     hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, n, iwork, cnew, n,
