@@ -1,0 +1,39 @@
+"""The generalized solve at the workload sizes (n = 1204; n = 1788 with the band-32 configuration and with the
+built-in band) and at sizes on either side of the 64-row tiles and of n = 1204, in both precisions, against host
+LAPACK (scipy eigh) with the FP64 gates: eig err <= 1e-12 ||H||, residual <= 1e-13, ||C^T S C - I|| <= 1e-12 n."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import workloads
+import xtbeig
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(np.asfortranarray(a).T)).cuda()
+
+
+def host(t):
+    return np.asfortranarray(t.cpu().numpy().T)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("n,band", [(1204, 0), (1788, 32), (1788, 0), (63, 0), (65, 0), (127, 0), (129, 0),
+                                    (257, 0), (1203, 0), (1205, 0), (600, 16), (700, 32)])
+def test_sygvd_sizes(handle, torch_cuda, n, band, mixed):
+    torch = torch_cuda
+    h, s = workloads.generalized_pair(n, 1001 + n)
+    print(f"n={n} band={band} mixed={mixed}", flush=True)
+    handle.set_tuning(band, 2)
+    try:
+        H, S = dev(torch, h), dev(torch, s)
+        W = torch.zeros(n, dtype=torch.float64, device="cuda")
+        C = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        handle.sygvd(H, S, W, C, xtbeig.FP32_REFINE if mixed else xtbeig.FP64)
+        assert handle.synchronize() == [0, 0, 0]
+    finally:
+        handle.set_tuning(0, 2)
+    g = workloads.gates(h, s, W.cpu().numpy(), host(C), sl.eigh(h, s, eigvals_only=True))
+    assert workloads.passes(g), (n, band, mixed, g)
