@@ -37,3 +37,25 @@ def test_sygvd_sizes(handle, torch_cuda, n, band, mixed):
         handle.set_tuning(0, 2)
     g = workloads.gates(h, s, W.cpu().numpy(), host(C), sl.eigh(h, s, eigvals_only=True))
     assert workloads.passes(g), (n, band, mixed, g)
+
+
+def test_sygvd_refuses_sizes_past_the_limit(handle, torch_cuda):
+    # n - band > 2048 is refused on the host before any work is enqueued; the handle keeps working
+    torch = torch_cuda
+    n = 2100
+    H = torch.eye(n, dtype=torch.float64, device="cuda")
+    S = H.clone()
+    W = torch.zeros(n, dtype=torch.float64, device="cuda")
+    C = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError, match="exceeds the largest supported size"):
+        handle.sygvd(H, S, W, C, xtbeig.FP64)
+    assert handle.synchronize() == [0, 0, 0]
+    m = 131
+    h, s = workloads.generalized_pair(m, 5)
+    H2, S2 = dev(torch, h), dev(torch, s)
+    W2 = torch.zeros(m, dtype=torch.float64, device="cuda")
+    C2 = torch.zeros((m, m), dtype=torch.float64, device="cuda")
+    handle.sygvd(H2, S2, W2, C2, xtbeig.FP32_REFINE)
+    assert handle.synchronize() == [0, 0, 0]
+    g = workloads.gates(h, s, W2.cpu().numpy(), host(C2), sl.eigh(h, s, eigvals_only=True))
+    assert workloads.passes(g), g

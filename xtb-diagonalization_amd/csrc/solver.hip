@@ -245,11 +245,21 @@ size_t syevd_bytes(int n, int b) {
   return c.off + 4096;
 }
 
+// The dense-to-band panel factorization holds a panel column in registers, at most 2048 rows below the band (8 rows
+// per thread of one workgroup), and the divide and conquer stages each merge in LDS: larger solves are refused before
+// any work is enqueued.
+static void check_size(int n, int b) {
+  if (n - b > 2048)
+    throw std::runtime_error("n = " + std::to_string(n) + " exceeds the largest supported size, 2048 + band width (" +
+                             std::to_string(b) + ")");
+}
+
 template <typename T>
 void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b) {
   if (n <= 0) return;
   if (b <= 0) b = h.band_for(n, sizeof(T) == 4);
   b = std::max(1, std::min(b, std::max(1, n - 1)));
+  check_size(n, b);
   h.reserve(syevd_bytes<T>(n, b));
   Carver c(h.arena);
   T* work = c.take<T>((size_t)n * n);
@@ -288,6 +298,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   hipStream_t st = h.stream;
   if (b <= 0) b = h.band_for(n, mixed);
   b = std::max(1, std::min(b, std::max(1, n - 1)));
+  check_size(n, b);
   h.reserve(sygvd_bytes(n, b, mixed));
   Carver c(h.arena);
   const size_t nn = (size_t)n * n;
