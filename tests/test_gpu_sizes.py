@@ -59,3 +59,39 @@ def test_sygvd_refuses_sizes_past_the_limit(handle, torch_cuda):
     assert handle.synchronize() == [0, 0, 0]
     g = workloads.gates(h, s, W2.cpu().numpy(), host(C2), sl.eigh(h, s, eigvals_only=True))
     assert workloads.passes(g), g
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_host_sygvd_staged_and_strided(torch_cuda, uplo, mixed):
+    """The LAPACK-style host-array call on matrices large enough (2.9 MB) and strided enough (lda = n + 7) to go
+    through the handle's pinned staging buffer: results pass the gates, the padding rows past n come back untouched,
+    and b holds the Cholesky factor in the referenced triangle with the other one untouched."""
+    n, ld = 600, 607
+    h, s = workloads.generalized_pair(n, 77)
+    a = np.full((ld, n), 7.0, order="F")
+    b = np.full((ld, n), 9.0, order="F")
+    a[:n] = h
+    b[:n] = s
+    w = np.zeros(n)
+    lw, liw = 1 + 6 * n + 2 * n * n, 3 + 5 * n
+    work = np.zeros(lw)
+    iwork = np.zeros(liw, dtype=np.int32)
+    info = xtbeig.ctypes.c_int(0)
+    f = xtbeig.lib().xtbeig_dsygvd_mixed if mixed else xtbeig.lib().xtbeig_dsygvd
+    f(xtbeig._i(1), b"V", uplo.encode(), xtbeig._i(n), xtbeig._ptr(a), xtbeig._i(ld), xtbeig._ptr(b),
+      xtbeig._i(ld), xtbeig._ptr(w), xtbeig._ptr(work), xtbeig._i(lw), xtbeig._ptr(iwork), xtbeig._i(liw),
+      xtbeig.ctypes.byref(info))
+    assert info.value == 0, xtbeig.last_error()
+    assert np.all(a[n:] == 7.0) and np.all(b[n:] == 9.0)
+    g = workloads.gates(h, s, w, np.asfortranarray(a[:n]), sl.eigh(h, s, eigvals_only=True))
+    assert workloads.passes(g), g
+    f_ = b[:n]
+    if uplo == "U":
+        u = np.triu(f_)
+        assert np.max(np.abs(u.T @ u - s)) < 1e-14 * n * np.linalg.norm(s, 2)
+        assert np.array_equal(np.tril(f_, -1), np.tril(s, -1))
+    else:
+        lo = np.tril(f_)
+        assert np.max(np.abs(lo @ lo.T - s)) < 1e-14 * n * np.linalg.norm(s, 2)
+        assert np.array_equal(np.triu(f_, 1), np.triu(s, 1))
