@@ -128,3 +128,37 @@ def test_ssygvd(torch_cuda, n):
     assert np.max(np.abs(w - wref)) < 1e-6 * np.linalg.norm(h, 2) * 4
     c = c.astype(np.float64)
     assert np.max(np.abs(c.T @ s @ c - np.eye(n))) < 1e-5
+
+
+def test_one_stage_chain_strided_host_arrays(torch_cuda):
+    """The sytrd.c symbol set on strided caller arrays large enough for the pinned staging path (n = 600,
+    lda = n + 5, ldz = n + 3): xtb_dsytrd_upper, xtb_dstedc, xtb_dormtr_upper give A Z = Z diag(d) with orthonormal
+    Z; the padding rows past n and the strictly lower triangle of A come back untouched."""
+    n, lda, ldz = 600, 605, 603
+    a0 = sym(n, 4242)
+    a = np.full((lda, n), 5.0, order="F")
+    a[:n] = np.triu(a0) + np.tril(np.full((n, n), 7.0), -1)
+    a_in = a.copy()
+    d, e, tau, work = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(32 * n)
+    L = xtbeig.lib()
+    L.xtb_dsytrd_upper(xtbeig._i(n), xtbeig._ptr(a), xtbeig._i(lda), xtbeig._ptr(d), xtbeig._ptr(e),
+                       xtbeig._ptr(tau), xtbeig._ptr(work))
+    assert np.all(a[n:] == 5.0)
+    il = np.tril_indices(n, -1)
+    assert np.array_equal(a[:n][il], a_in[:n][il])
+    z = np.full((ldz, n), 3.0, order="F")
+    lw, liw = 1 + 4 * n + n * n, 3 + 5 * n
+    w2, iw = np.zeros(lw), np.zeros(liw, dtype=np.int32)
+    info = xtbeig.ctypes.c_int(0)
+    L.xtb_dstedc(xtbeig._i(n), xtbeig._ptr(d), xtbeig._ptr(e), xtbeig._ptr(z), xtbeig._i(ldz), xtbeig._ptr(w2),
+                 xtbeig._i(lw), xtbeig._ptr(iw), xtbeig._i(liw), xtbeig.ctypes.byref(info))
+    assert info.value == 0
+    assert np.all(z[n:] == 3.0)
+    L.xtb_dormtr_upper(xtbeig._i(n), xtbeig._i(n), xtbeig._ptr(a), xtbeig._i(lda), xtbeig._ptr(tau), xtbeig._ptr(z),
+                       xtbeig._i(ldz), xtbeig._ptr(work))
+    assert np.all(z[n:] == 3.0)
+    zz = z[:n]
+    nrm = np.linalg.norm(a0, 2)
+    assert np.max(np.abs(a0 @ zz - zz * d)) < 1e-13 * n * nrm
+    assert np.max(np.abs(zz.T @ zz - np.eye(n))) < 1e-13 * n
+    assert np.max(np.abs(d - np.linalg.eigvalsh(a0))) < 1e-13 * n * nrm
