@@ -59,7 +59,7 @@ void xtb_sormtr_upper(const int *n, const int *m, const float *a, const int *lda
 /* ---- 1b. LAPACK-convention entry points ------------------------------------------------------------------------ */
 
 /* dsytrd(uplo, ...): uplo = 'U' only (info = -1 otherwise), as xtb_dsytrd_upper; lwork >= 1, lwork = -1 queries
- * the optimal n * 32 into work[0]. The north_star's d/s sytrd (LAPACK dsytrd's interface). */
+ * the optimal n * 32 into work[0]. d/s sytrd with LAPACK dsytrd's interface. */
 void xtbeig_dsytrd(const char *uplo, const int *n, double *a, const int *lda, double *d, double *e, double *tau,
                    double *work, const int *lwork, int *info);
 void xtbeig_ssytrd(const char *uplo, const int *n, float *a, const int *lda, float *d, float *e, float *tau,
@@ -178,8 +178,8 @@ int xtbeig_dev_sygvd_graph(xtbeig_handle_t handle, int precision, int n, const d
 /* count independent generalized solves of uneven n (arrays of count device pointers and sizes, in host memory) on up
  * to `slots` concurrent streams of sub-handles, longest first onto the least-loaded slot; use_graph: each solve
  * through the slot's graph cache. Waits for completion; latency_ms (count, may be NULL) receives each solve's time
- * on its stream. Status words of the slots are merged into the handle's (xtbeig_synchronize). The batched
- * configuration of the north_star; no reference counterpart. */
+ * on its stream. Status words of the slots are merged into the handle's (xtbeig_synchronize). For batches of
+ * independent problems (256 of n in [124, 1204], say); no reference counterpart. */
 int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, const int *n, const double *const *h,
                              const int *ldh, const double *const *s, const int *lds, double *const *w,
                              double *const *c, const int *ldc, int slots, int use_graph, float *latency_ms);
