@@ -41,7 +41,7 @@ def timed_solves(torch, hnd, hd, sd, w, c, prec, steps, warmup):
     for _ in range(warmup):
         hnd.sygvd(hd, sd, w, c, prec)
     st = hnd.synchronize()
-    if st != [0, 0, 0]:
+    if st != [0, 0, 0, 0]:
         raise RuntimeError(f"solver status {st}")
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
@@ -54,7 +54,7 @@ def timed_solves(torch, hnd, hd, sd, w, c, prec, steps, warmup):
     st = hnd.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
-    if st != [0, 0, 0]:
+    if st != [0, 0, 0, 0]:
         raise RuntimeError(f"solver status {st}")
     return t0.elapsed_time(t1) / steps, wall / steps * 1e3
 
@@ -97,7 +97,7 @@ def kernel_roofline(torch, xtbeig, hnd, h, s, b):
         ev[2 * r + 1].record(stream)
     st = hnd.synchronize()
     torch.cuda.synchronize()
-    if st != [0, 0, 0]:
+    if st != [0, 0, 0, 0]:
         raise RuntimeError(f"chase status {st}")
     times = [ev[2 * r].elapsed_time(ev[2 * r + 1]) for r in range(1, reps + 1)]
     ms = sum(times) / len(times)
@@ -159,7 +159,7 @@ def components(torch, np, workloads, xtbeig, hnd, h, s):
             call()
             e1.record(stream)
             st = hnd.synchronize()
-            if st != [0, 0, 0]:
+            if st != [0, 0, 0, 0]:
                 raise RuntimeError(f"status {st}")
             if r >= warm:
                 ts.append(e0.elapsed_time(e1))

@@ -59,7 +59,9 @@ void xtb_sormtr_upper(const int *n, const int *m, const float *a, const int *lda
 /* ---- 1b. LAPACK-convention entry points ------------------------------------------------------------------------ */
 
 /* dsytrd(uplo, ...): uplo = 'U' only (info = -1 otherwise), as xtb_dsytrd_upper; lwork >= 1, lwork = -1 queries
- * the optimal n * 32 into work[0]. d/s sytrd with LAPACK dsytrd's interface. */
+ * the optimal n * 32 into work[0]. d/s sytrd with LAPACK dsytrd's interface. info = n + 1 when the GPU work failed
+ * (HIP error, allocation or launch failure; the reason in xtbeig_last_error()); a, d, e and tau are then not valid.
+ * The void xtb_* entry points of section 1a report such failures only through xtbeig_last_error(). */
 void xtbeig_dsytrd(const char *uplo, const int *n, double *a, const int *lda, double *d, double *e, double *tau,
                    double *work, const int *lwork, int *info);
 void xtbeig_ssytrd(const char *uplo, const int *n, float *a, const int *lda, float *d, float *e, float *tau,
@@ -67,7 +69,9 @@ void xtbeig_ssytrd(const char *uplo, const int *n, float *a, const int *lda, flo
 
 /* dsyevd(jobz, uplo, ...): all eigenvalues (w, ascending) and, for jobz = 'V', eigenvectors in a. lwork >=
  * 1 + 6n + 2n^2, liwork >= 3 + 5n ('V'); lwork = -1 or liwork = -1 is a workspace query. The reference's
- * eigensolve (tridiagonalize/sbrReduce + stedc + back-transformation, Sources/EigenBench/main.swift:291-420). */
+ * eigensolve (tridiagonalize/sbrReduce + stedc + back-transformation, Sources/EigenBench/main.swift:291-420).
+ * info > 0: a secular equation of the divide and conquer did not converge (its index), or n + 1 when the bulge
+ * chase stalled or the GPU work failed or n exceeds 2048 + band width (the reason in xtbeig_last_error()). */
 void xtbeig_dsyevd(const char *jobz, const char *uplo, const int *n, double *a, const int *lda, double *w,
                    double *work, const int *lwork, int *iwork, const int *liwork, int *info);
 void xtbeig_ssyevd(const char *jobz, const char *uplo, const int *n, float *a, const int *lda, float *w, float *work,
@@ -80,8 +84,11 @@ void xtbeig_ssyevd(const char *jobz, const char *uplo, const int *n, float *a, c
 void xtbeig_dsygvd(const int *itype, const char *jobz, const char *uplo, const int *n, double *a, const int *lda,
                    double *b, const int *ldb, double *w, double *work, const int *lwork, int *iwork,
                    const int *liwork, int *info);
-/* As xtbeig_dsygvd, but computed in FP32 on the matrix cores and refined in FP64 (Ogita-Aishima, the reference's
- * fp32_refine scheme, scripts/kernels/fp32_error.c:95-158); same arguments and results as the FP64 call. */
+/* As xtbeig_dsygvd, but computed in FP32 on the matrix cores and refined in FP64 (Ogita-Aishima with Rayleigh-Ritz
+ * on clusters, the reference's fp32_refine scheme, scripts/kernels/fp32_error.c:95-158, refine.f90:254-325); same
+ * arguments and results as the FP64 call. When the refined pairs fail the a-posteriori check (status word 3 of
+ * xtbeig_synchronize), or the FP32 chase or divide and conquer fails, the call solves the problem again in FP64 and
+ * returns that result. */
 void xtbeig_dsygvd_mixed(const int *itype, const char *jobz, const char *uplo, const int *n, double *a,
                          const int *lda, double *b, const int *ldb, double *w, double *work, const int *lwork,
                          int *iwork, const int *liwork, int *info);
@@ -104,14 +111,22 @@ int xtbeig_destroy(xtbeig_handle_t handle);
 /* Enqueue on the caller's HIP stream (hipStream_t passed as void*) instead of the handle's own. */
 int xtbeig_set_stream(xtbeig_handle_t handle, void *stream);
 void *xtbeig_get_stream(xtbeig_handle_t handle);
-/* Band width of the two-stage reduction (0: the built-in table per n) and the refinement iterations. */
+/* Band width of the two-stage reduction (0: the built-in table per n; at most 32) and the refinement iterations
+ * (0 to 8; 0 returns the FP32 eigenpairs unrefined and unchecked). */
 int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations);
 /* The built-in band width for an n x n solve (fp32 nonzero: the FP32 reduction of the mixed path). No reference
  * counterpart beyond SBROptions.recommended (SBROptions.swift:73-82), which picks b per n the same way. */
 int xtbeig_default_band(int n, int fp32);
-/* Wait for the handle's stream; status[0..2] = Cholesky info, bulge-chase stall flag, divide-and-conquer info
- * (status may be NULL). Returns 0 when all three are zero. Clears them. */
+/* Wait for the handle's stream; status[0..3] = Cholesky info, bulge-chase stall flag, divide-and-conquer info, and
+ * the refinement check of an FP32 + refinement solve (nonzero: the refined eigenpairs failed it, or a cluster's
+ * Rayleigh-Ritz step met a Gram block that is not positive definite) (status: 4 ints, may be NULL). Returns 0 when all
+ * four are zero. Clears them. */
 int xtbeig_synchronize(xtbeig_handle_t handle, int *status);
+/* The last refinement check on the handle (after xtbeig_synchronize or with the stream idle): ratios[0] = the
+ * residual estimate ||(H C - S C Lambda) V||_F / sqrt(8) over 8 random sign vectors V, divided by 1e-13 n ||H||_F /
+ * sqrt(n); ratios[1] = the estimate of ||C^T S C - I||_F divided by 1e-12 n. A solve passes with both <= 1. No
+ * reference counterpart (the reference's gates need the FP64 oracle: eigbench.f90:189-204). */
+int xtbeig_refine_check(xtbeig_handle_t handle, double *ratios);
 /* Message of the last failed call on this thread. */
 const char *xtbeig_last_error(void);
 
@@ -183,8 +198,8 @@ int xtbeig_dev_sygvd_graph(xtbeig_handle_t handle, int precision, int n, const d
 int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, const int *n, const double *const *h,
                              const int *ldh, const double *const *s, const int *lds, double *const *w,
                              double *const *c, const int *ldc, int slots, int use_graph, float *latency_ms);
-/* Band-width autotuner: for every n bucket of the handle's table that meets [nmin, nmax], times the candidate band
- * widths (FP64: 8, 16, 32; FP32 + refinement: 16, 32) on a synthetic pair of the bucket's middle size and keeps the
+/* Band-width autotuner: for every n bucket of the handle's table that meets [nmin, nmax] (2 <= nmin <= nmax <=
+ * 2048), times the candidate band widths 8, 16 and 32 on a synthetic pair of the bucket's middle size and keeps the
  * fastest in the table, which later solves on the handle use (an explicit xtbeig_set_tuning band overrides it).
  * Outputs per tuned bucket (arrays of xtbeig_band_buckets() entries, may be NULL): its first n, the band, the time
  * in ms; *count the number of buckets tuned. Reference counterpart: SBROptions.recommended's fixed table

@@ -22,7 +22,7 @@ def solve_eager(torch, hnd, h, s, prec):
     w = torch.zeros(n, dtype=torch.float64, device="cuda")
     c = torch.zeros((n, n), dtype=torch.float64, device="cuda")
     hnd.sygvd(dev(torch, h), dev(torch, s), w, c, prec)
-    assert hnd.synchronize() == [0, 0, 0]
+    assert hnd.synchronize() == [0, 0, 0, 0]
     return w.cpu().numpy(), c.cpu().numpy()
 
 
@@ -40,17 +40,17 @@ def test_graph_replays_match_eager(torch_cuda, prec):
     c = torch.zeros((n, n), dtype=torch.float64, device="cuda")
     for rep in range(4):
         hnd.sygvd_graph(hd, sd, w, c, prec)
-        assert hnd.synchronize() == [0, 0, 0]
+        assert hnd.synchronize() == [0, 0, 0, 0]
         assert np.array_equal(w.cpu().numpy(), we) and np.array_equal(c.cpu().numpy(), ce), rep
         # eager work of another size in between
         solve_eager(torch, hnd, *workloads.generalized_pair(64, 303 + rep), prec)
         hnd.sygvd_graph(h2d, sd, w, c, prec)
-        assert hnd.synchronize() == [0, 0, 0]
+        assert hnd.synchronize() == [0, 0, 0, 0]
         assert np.array_equal(w.cpu().numpy(), we2) and np.array_equal(c.cpu().numpy(), ce2), rep
     # a larger solve regrows the workspace: the graph is captured again and still matches
     solve_eager(torch, hnd, *workloads.generalized_pair(300, 310), prec)
     hnd.sygvd_graph(hd, sd, w, c, prec)
-    assert hnd.synchronize() == [0, 0, 0]
+    assert hnd.synchronize() == [0, 0, 0, 0]
     assert np.array_equal(w.cpu().numpy(), we) and np.array_equal(c.cpu().numpy(), ce)
     hnd.close()
 
@@ -68,7 +68,7 @@ def test_batched_matches_eager_and_oracle(torch_cuda, prec, use_graph):
     ws = [torch.zeros(n, dtype=torch.float64, device="cuda") for n in ns]
     cs = [torch.zeros((n, n), dtype=torch.float64, device="cuda") for n in ns]
     lat = hnd.sygvd_batched(hs, ss, ws, cs, prec, slots=4, use_graph=use_graph)
-    assert hnd.synchronize() == [0, 0, 0]
+    assert hnd.synchronize() == [0, 0, 0, 0]
     assert len(lat) == len(ns) and all(x > 0 for x in lat)
     ref = xtbeig.Handle(0)
     for q, (h, s) in enumerate(probs):

@@ -52,7 +52,7 @@ def test_potrf_inverse(handle, torch_cuda):
     U = torch.zeros_like(S)
     X = torch.zeros_like(S)
     handle.potrf_inverse(S, U, X)
-    assert handle.synchronize() == [0, 0, 0]
+    assert handle.synchronize() == [0, 0, 0, 0]
     u, x = host(U), host(X)
     ref = sl.cholesky(s, lower=False)
     assert np.max(np.abs(u - ref)) < 1e-13 * np.max(np.abs(ref))
@@ -134,7 +134,7 @@ def test_chase(handle, torch_cuda, prec, b, n):
     D = torch.zeros(n, dtype=dt, device="cuda")
     E = torch.zeros(n, dtype=dt, device="cuda")
     handle.chase(n, b, AB, V2, D, E)
-    assert handle.synchronize() == [0, 0, 0]
+    assert handle.synchronize() == [0, 0, 0, 0]
     d, e = D.double().cpu().numpy(), E.double().cpu().numpy()
     if prec == "d" and n <= 1000:
         # same reflectors as the oracle's sequential chase: same tridiagonal to rounding (in FP32, and over the
@@ -167,7 +167,7 @@ def test_stedc(handle, torch_cuda, n):
     E = torch.from_numpy(np.concatenate([e, [0.0]])).cuda()
     Z = torch.zeros((n, n), dtype=torch.float64, device="cuda")
     handle.dstedc(D, E, Z)
-    assert handle.synchronize() == [0, 0, 0]
+    assert handle.synchronize() == [0, 0, 0, 0]
     w, z = D.cpu().numpy(), host(Z)
     wref = sl.eigvalsh_tridiagonal(d, e) if n > 1 else d
     scale = max(np.max(np.abs(d)), np.max(np.abs(e)) if n > 1 else 0)
@@ -195,7 +195,7 @@ def test_stedc_degenerate(handle, torch_cuda):
         E = torch.from_numpy(np.concatenate([e, [0.0]])).cuda()
         Z = torch.zeros((n, n), dtype=torch.float64, device="cuda")
         handle.dstedc(D, E, Z)
-        assert handle.synchronize() == [0, 0, 0]
+        assert handle.synchronize() == [0, 0, 0, 0]
         w, z = D.cpu().numpy(), host(Z)
         t = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
         wref = np.linalg.eigvalsh(t)
@@ -217,7 +217,7 @@ def test_syevd(handle, torch_cuda, n, prec):
         handle.dsyevd(A, W, Z)
     else:
         handle.ssyevd(A, W, Z)
-    assert handle.synchronize() == [0, 0, 0]
+    assert handle.synchronize() == [0, 0, 0, 0]
     w, z = W.cpu().numpy(), host(Z).astype(np.float64)
     g = workloads.gates(a, None, w, z, lam)
     if prec == "d":
@@ -236,6 +236,6 @@ def test_sygvd_device(handle, torch_cuda, n, mixed):
     W = torch.zeros(n, dtype=torch.float64, device="cuda")
     C = torch.zeros((n, n), dtype=torch.float64, device="cuda")
     handle.sygvd(H, S, W, C, xtbeig.FP32_REFINE if mixed else xtbeig.FP64)
-    assert handle.synchronize() == [0, 0, 0]
+    assert handle.synchronize() == [0, 0, 0, 0]
     g = workloads.gates(h, s, W.cpu().numpy(), host(C), wref)
     assert workloads.passes(g), g

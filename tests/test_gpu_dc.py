@@ -48,7 +48,7 @@ def test_merge_matches_dlaed2(handle, torch_cuda, kind, n, n1):
     D = torch.from_numpy(d.copy()).cuda()
     Q = torch.from_numpy(np.ascontiguousarray(q.T)).cuda()
     out = xtbeig.dc_merge(handle, D, Q, n1, rho)
-    assert out["status"] == [0, 0, 0]
+    assert out["status"] == [0, 0, 0, 0]
     k = ref["k"]
     assert out["k"] == k
     assert list(out["ctot"]) == list(ref["ctot"])

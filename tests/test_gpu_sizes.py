@@ -32,7 +32,7 @@ def test_sygvd_sizes(handle, torch_cuda, n, band, mixed):
         W = torch.zeros(n, dtype=torch.float64, device="cuda")
         C = torch.zeros((n, n), dtype=torch.float64, device="cuda")
         handle.sygvd(H, S, W, C, xtbeig.FP32_REFINE if mixed else xtbeig.FP64)
-        assert handle.synchronize() == [0, 0, 0]
+        assert handle.synchronize() == [0, 0, 0, 0]
     finally:
         handle.set_tuning(0, 2)
     g = workloads.gates(h, s, W.cpu().numpy(), host(C), sl.eigh(h, s, eigvals_only=True))
@@ -49,14 +49,14 @@ def test_sygvd_refuses_sizes_past_the_limit(handle, torch_cuda):
     C = torch.zeros((n, n), dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError, match="exceeds the largest supported size"):
         handle.sygvd(H, S, W, C, xtbeig.FP64)
-    assert handle.synchronize() == [0, 0, 0]
+    assert handle.synchronize() == [0, 0, 0, 0]
     m = 131
     h, s = workloads.generalized_pair(m, 5)
     H2, S2 = dev(torch, h), dev(torch, s)
     W2 = torch.zeros(m, dtype=torch.float64, device="cuda")
     C2 = torch.zeros((m, m), dtype=torch.float64, device="cuda")
     handle.sygvd(H2, S2, W2, C2, xtbeig.FP32_REFINE)
-    assert handle.synchronize() == [0, 0, 0]
+    assert handle.synchronize() == [0, 0, 0, 0]
     g = workloads.gates(h, s, W2.cpu().numpy(), host(C2), sl.eigh(h, s, eigvals_only=True))
     assert workloads.passes(g), g
 

@@ -55,7 +55,7 @@ def main():
             lat = hnd.sygvd_batched(hs, ss, ws, cs, prec, slots=args.slots, use_graph=args.graph)
             st = hnd.synchronize()
             wall = time.perf_counter() - t1
-            if st != [0, 0, 0]:
+            if st != [0, 0, 0, 0]:
                 raise RuntimeError(f"status {st}")
             if best is None or wall < best[0]:
                 best = (wall, lat)

@@ -2,7 +2,7 @@
 // batches of uneven-n problems spread over concurrent streams, and the per-bucket band-width autotuner.
 //
 // Graphs. A solve enqueues several hundred kernels; replaying them from a graph removes the host's launch work and
-// the gaps between launches. The graph for (n, precision, band) is captured after one uncaptured warm-up solve (the
+// the gaps between launches. The graph for (n, precision, band, refinement steps) is captured after one uncaptured warm-up solve (the
 // D&C plan, the back-transformation plan and the arena exist then, so the capture records kernels only: no
 // allocation, no memset, no host synchronization). It reads H and S from its own staging buffers and writes w and
 // C there; a call copies in, launches, copies out. A graph remembers the arena generation it was captured against:
@@ -63,7 +63,7 @@ void sygvd_graph(Handle& h, int n, const double* hm, int ldh, const double* sm, 
                  int ldc, bool mixed) {
   if (n <= 0) return;
   const int b = h.band_for(n, mixed);
-  const auto key = std::make_tuple(n, mixed ? 1 : 0, b);
+  const auto key = std::make_tuple(n, mixed ? 1 : 0, b, mixed ? h.refine_iters : 0);
   auto it = h.graphs.find(key);
   Handle::GraphEntry* g = (it == h.graphs.end()) ? nullptr : it->second.get();
   const size_t nn = (size_t)n * n;
@@ -163,18 +163,18 @@ void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, 
   XTB_HIP(hipStreamSynchronize(h.stream));
   // the slots' status words: any failure is reported on the caller's handle
   for (int sl = 0; sl < slots; ++sl) {
-    int st[3];
-    XTB_HIP(hipMemcpy(st, h.subs[sl]->dinfo, 3 * sizeof(int), hipMemcpyDeviceToHost));
-    XTB_HIP(hipMemset(h.subs[sl]->dinfo, 0, 3 * sizeof(int)));
-    int tot[3];
-    XTB_HIP(hipMemcpy(tot, h.dinfo, 3 * sizeof(int), hipMemcpyDeviceToHost));
+    int st[NSTATUS];
+    XTB_HIP(hipMemcpy(st, h.subs[sl]->dinfo, NSTATUS * sizeof(int), hipMemcpyDeviceToHost));
+    XTB_HIP(hipMemset(h.subs[sl]->dinfo, 0, NSTATUS * sizeof(int)));
+    int tot[NSTATUS];
+    XTB_HIP(hipMemcpy(tot, h.dinfo, NSTATUS * sizeof(int), hipMemcpyDeviceToHost));
     bool changed = false;
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < NSTATUS; ++k)
       if (st[k] && !tot[k]) {
         tot[k] = st[k];
         changed = true;
       }
-    if (changed) XTB_HIP(hipMemcpy(h.dinfo, tot, 3 * sizeof(int), hipMemcpyHostToDevice));
+    if (changed) XTB_HIP(hipMemcpy(h.dinfo, tot, NSTATUS * sizeof(int), hipMemcpyHostToDevice));
   }
   if (latency_ms)
     for (int q = 0; q < count; ++q) XTB_HIP(hipEventElapsedTime(&latency_ms[q], ev[2 * q], ev[2 * q + 1]));
@@ -221,13 +221,13 @@ void autotune_bands(Handle& h, bool mixed, int nmin, int nmax, int* buckets_out,
           XTB_HIP(hipEventElapsedTime(&ms, e0, e1));
           t = std::min(t, ms);
         }
-        int st[3];
-        XTB_HIP(hipMemcpy(st, h.dinfo, 3 * sizeof(int), hipMemcpyDeviceToHost));
-        if ((st[0] | st[1] | st[2]) == 0 && t < best_ms) {
+        int st[NSTATUS];
+        XTB_HIP(hipMemcpy(st, h.dinfo, NSTATUS * sizeof(int), hipMemcpyDeviceToHost));
+        if ((st[0] | st[1] | st[2] | st[3]) == 0 && t < best_ms) {
           best_ms = t;
           best_b = b;
         }
-        XTB_HIP(hipMemset(h.dinfo, 0, 3 * sizeof(int)));
+        XTB_HIP(hipMemset(h.dinfo, 0, NSTATUS * sizeof(int)));
       }
     } catch (...) {
       h.band_override = keep_override;

@@ -54,8 +54,8 @@ int guarded(F&& f) {
 // Device status words -> host, cleared.
 void read_status(Handle& h, int* st) {
   XTB_HIP(hipStreamSynchronize(h.stream));
-  XTB_HIP(hipMemcpy(st, h.dinfo, 3 * sizeof(int), hipMemcpyDeviceToHost));
-  XTB_HIP(hipMemsetAsync(h.dinfo, 0, 3 * sizeof(int), h.stream));
+  XTB_HIP(hipMemcpy(st, h.dinfo, xtbeig::NSTATUS * sizeof(int), hipMemcpyDeviceToHost));
+  XTB_HIP(hipMemsetAsync(h.dinfo, 0, xtbeig::NSTATUS * sizeof(int), h.stream));
   XTB_HIP(hipStreamSynchronize(h.stream));
 }
 
@@ -222,7 +222,7 @@ void syevd_host(const char* jobz, const char* uplo, const int* n_, T* a, const i
   if (*lwork < lmin) { *info = -8; return; }
   if (*liwork < limin) { *info = -10; return; }
   if (n == 0) return;
-  int st[3] = {0, 0, 0};
+  int st[xtbeig::NSTATUS] = {0, 0, 0, 0};
   const int rc = guarded([&] {
     Handle& h = thread_handle();
     DevBuf da(sizeof(T) * (size_t)n * n), dz(sizeof(T) * (size_t)n * n), dw(sizeof(double) * n),
@@ -265,7 +265,7 @@ static void sygvd_host(bool mixed, const int* itype, const char* jobz, const cha
   if (*lwork < lmin) { *info = -11; return; }
   if (*liwork < limin) { *info = -13; return; }
   if (n == 0) return;
-  int st[3] = {0, 0, 0};
+  int st[xtbeig::NSTATUS] = {0, 0, 0, 0};
   const int rc = guarded([&] {
     Handle& h = thread_handle();
     const size_t nn = (size_t)n * n;
@@ -298,9 +298,20 @@ static void sygvd_host(bool mixed, const int* itype, const char* jobz, const cha
     const double* uf = nullptr;  // the solve's own Cholesky factor of S, in the handle's arena
     xtbeig::sygvd_device(h, n, dh.as<double>(), n, ds.as<double>(), n, dw.as<double>(), dc.as<double>(), n, mixed,
                          h.band_override, &uf);
+    if (mixed) {
+      // an FP32 solve whose refined pairs fail the check, or whose chase or D&C failed, is redone in FP64
+      read_status(h, st);
+      if (st[0] == 0 && (st[1] || st[2] || st[3])) {
+        xtbeig::sygvd_device(h, n, dh.as<double>(), n, ds.as<double>(), n, dw.as<double>(), dc.as<double>(), n,
+                             false, 0, &uf);
+        for (int k = 0; k < xtbeig::NSTATUS; ++k) st[k] = 0;
+      }
+    }
     download(w, n, dw.as<double>(), n, 1, false, false);
     if (wantz) download(a, lda, dc.as<double>(), n, n, false, false);
-    read_status(h, st);
+    int st2[xtbeig::NSTATUS] = {0, 0, 0, 0};
+    read_status(h, st2);
+    for (int k = 0; k < xtbeig::NSTATUS; ++k) st[k] = st[k] ? st[k] : st2[k];
     if (st[0] == 0) {
       // b on exit: the Cholesky factor in the referenced triangle (U, or L = U^T), the other triangle untouched,
       // as LAPACK's dsygvd leaves it
@@ -314,10 +325,12 @@ static void sygvd_host(bool mixed, const int* itype, const char* jobz, const cha
 }
 
 // One-stage reduction from host memory (upper triangle referenced; full: both triangles stored and returned).
+// Returns guarded()'s code: nonzero when a HIP error, an allocation or a launch failure stopped the reduction (the
+// message in xtbeig_last_error); the void xtb_* entry points can report it only there.
 template <typename T>
-static void sytrd_host(int n, T* a, int lda, T* d, T* e, T* tau, bool full) {
-  if (n <= 0) return;
-  guarded([&] {
+static int sytrd_host(int n, T* a, int lda, T* d, T* e, T* tau, bool full) {
+  if (n <= 0) return 0;
+  return guarded([&] {
     Handle& h = thread_handle();
     const size_t nn = (size_t)n * n;
     h.reserve(xtbeig::sytrd_workspace<T>(n) * sizeof(T) + 4096);
@@ -338,9 +351,9 @@ static void sytrd_host(int n, T* a, int lda, T* d, T* e, T* tau, bool full) {
 
 // z = Q z for m columns from host memory, Q from sytrd_host (a, tau).
 template <typename T>
-static void ormtr_host(int n, int m, const T* a, int lda, const T* tau, T* z, int ldz) {
-  if (n <= 1 || m <= 0) return;
-  guarded([&] {
+static int ormtr_host(int n, int m, const T* a, int lda, const T* tau, T* z, int ldz) {
+  if (n <= 1 || m <= 0) return 0;
+  return guarded([&] {
     Handle& h = thread_handle();
     h.reserve(xtbeig::ormtr_workspace<T>(n, m) * sizeof(T) + 4096);
     DevBuf da(sizeof(T) * (size_t)n * n), dt(sizeof(T) * n), dz(sizeof(T) * (size_t)n * m);
@@ -368,7 +381,7 @@ static void sytrd_lapack(const char* uplo, const int* n_, T* a, const int* lda_,
     work[0] = (T)((n > 1 ? n : 1) * 32);
     return;
   }
-  sytrd_host<T>(n, a, lda, d, e, tau, false);
+  if (sytrd_host<T>(n, a, lda, d, e, tau, false) != 0) *info = n + 1;
 }
 
 extern "C" {
@@ -396,8 +409,10 @@ int xtbeig_set_stream(xtbeig_handle_t handle, void* stream) {
 void* xtbeig_get_stream(xtbeig_handle_t handle) { return handle ? (void*)handle->h.stream : nullptr; }
 
 int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations) {
-  if (band < 0 || band > 64 || refine_iterations < 0 || refine_iterations > 8)
-    return set_error("xtbeig_set_tuning: band must be in [0, 64] and refine_iterations in [0, 8]");
+  if (!handle) return set_error("xtbeig_set_tuning: null handle");
+  if (band < 0 || band > xtbeig::MAX_BAND || refine_iterations < 0 || refine_iterations > 8)
+    return set_error(("xtbeig_set_tuning: band must be in [0, " + std::to_string(xtbeig::MAX_BAND) +
+                      "] and refine_iterations in [0, 8]").c_str());
   handle->h.band_override = band;
   handle->h.refine_iters = refine_iterations;
   return 0;
@@ -406,11 +421,21 @@ int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations) {
 int xtbeig_default_band(int n, int fp32) { return xtbeig::default_band(n, fp32 != 0); }
 
 int xtbeig_synchronize(xtbeig_handle_t handle, int* status) {
-  int st[3] = {0, 0, 0};
+  int st[xtbeig::NSTATUS] = {0, 0, 0, 0};
   int rc = guarded([&] { read_status(handle->h, st); });
   if (rc) return rc;
-  if (status) { status[0] = st[0]; status[1] = st[1]; status[2] = st[2]; }
-  return (st[0] || st[1] || st[2]) ? 2 : 0;
+  if (status)
+    for (int k = 0; k < xtbeig::NSTATUS; ++k) status[k] = st[k];
+  return (st[0] || st[1] || st[2] || st[3]) ? 2 : 0;
+}
+
+int xtbeig_refine_check(xtbeig_handle_t handle, double* ratios) {
+  if (!handle || !ratios) return set_error("xtbeig_refine_check: null argument");
+  return guarded([&] {
+    Handle& h = handle->h;
+    XTB_HIP(hipStreamSynchronize(h.stream));
+    XTB_HIP(hipMemcpy(ratios, h.check_ratios(), 2 * sizeof(double), hipMemcpyDeviceToHost));
+  });
 }
 
 int xtbeig_dev_sygvd(xtbeig_handle_t handle, int precision, int n, const double* hm, int ldh, const double* sm,
@@ -526,7 +551,7 @@ void xtb_dstedc(const int* n_, double* d, double* e, double* z, const int* ldz_,
   if (*liwork < limin) { *info = -9; return; }
   *info = 0;
   if (n == 0) return;
-  int st[3] = {0, 0, 0};
+  int st[xtbeig::NSTATUS] = {0, 0, 0, 0};
   const int rc = guarded([&] {
     Handle& h = thread_handle();
     DevBuf dd(sizeof(double) * n), de(sizeof(double) * n), dz(sizeof(double) * (size_t)n * n);
@@ -557,7 +582,7 @@ void xtb_sstedc(const int* n_, float* d, float* e, float* z, const int* ldz_, fl
   if (*liwork < limin) { *info = -9; return; }
   *info = 0;
   if (n == 0) return;
-  int st[3] = {0, 0, 0};
+  int st[xtbeig::NSTATUS] = {0, 0, 0, 0};
   const int rc = guarded([&] {
     Handle& h = thread_handle();
     DevBuf df(sizeof(float) * n * 2), dd(sizeof(double) * n), de(sizeof(double) * n),
@@ -771,7 +796,7 @@ int xtbeig_dev_scc_sygvd(xtbeig_handle_t handle, int n, const double* hm, int ld
   bool fp32 = h.scc_fp32_rmsq > 0 && !h.scc_fp64_latched && rmsq >= h.scc_fp32_rmsq &&
               h.scc_fp32_cycles < h.scc_fp32_maxiter;
   if (h.scc_fp32_rmsq > 0 && !fp32) h.scc_fp64_latched = true;  // FP64 for the rest of the sequence
-  int st[3] = {0, 0, 0};
+  int st[xtbeig::NSTATUS] = {0, 0, 0, 0};
   int rc = guarded([&] {
     if (fp32) {
       const int keep = h.refine_iters;
@@ -784,8 +809,9 @@ int xtbeig_dev_scc_sygvd(xtbeig_handle_t handle, int n, const double* hm, int ld
       }
       h.refine_iters = keep;
       read_status(h, st);
-      if (st[1] || st[2]) {
-        // the FP32 solve failed (chase stall or D&C): redo the cycle in FP64, FP64 from here on
+      if (st[1] || st[2] || st[3]) {
+        // the FP32 solve failed (chase stall, D&C, or the refined pairs failed the check): redo the cycle in FP64,
+        // FP64 from here on
         h.scc_fp64_restart = 1;
         h.scc_fp64_latched = true;
         fp32 = false;
@@ -854,7 +880,8 @@ int xtbeig_autotune(xtbeig_handle_t handle, int precision, int nmin, int nmax, i
   if (!handle) return set_error("xtbeig_autotune: null handle");
   if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
     return set_error("xtbeig_autotune: unknown precision");
-  if (nmin < 2 || nmax < nmin || nmax > 4096) return set_error("xtbeig_autotune: need 2 <= nmin <= nmax <= 4096");
+  if (nmin < 2 || nmax < nmin || nmax > xtbeig::MAX_N)
+    return set_error(("xtbeig_autotune: need 2 <= nmin <= nmax <= " + std::to_string(xtbeig::MAX_N)).c_str());
   int cnt = 0;
   for (int q = 0; q < Handle::NBUCKET; ++q)
     if (!(xtbeig::bucket_edge(q + 1) - 1 < nmin || xtbeig::bucket_edge(q) > nmax)) ++cnt;
@@ -880,7 +907,9 @@ int xtbeig_set_band_table(xtbeig_handle_t handle, int precision, const int* band
   if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
     return set_error("xtbeig_set_band_table: unknown precision");
   for (int q = 0; q < Handle::NBUCKET; ++q)
-    if (band[q] < 0 || band[q] > 32) return set_error("xtbeig_set_band_table: band widths must be in [0, 32]");
+    if (band[q] < 0 || band[q] > xtbeig::MAX_BAND)
+      return set_error(("xtbeig_set_band_table: band widths must be in [0, " + std::to_string(xtbeig::MAX_BAND) +
+                        "]").c_str());
   for (int q = 0; q < Handle::NBUCKET; ++q) handle->h.tuned_band[precision == XTBEIG_FP32_REFINE ? 1 : 0][q] = band[q];
   return 0;
 }

@@ -2,6 +2,7 @@
 // Matrices are column-major (Fortran ABI), element (i, j) of a matrix with leading dimension ld at i + j * ld.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -28,12 +29,21 @@ struct HipError : std::runtime_error {
 
 #define XTB_LAUNCH_CHECK() XTB_HIP(hipGetLastError())
 
-// Dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per workgroup): raise the kernel's limit before launching.
+// Dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per workgroup): raise the kernel's limit before launching. The
+// attribute is process-wide, so it is always raised to the whole 160 KiB, never to one call's size: concurrent calls
+// from several threads (different n, different LDS sizes) then all write the same value, and none can lower the
+// limit under another's launch.
+constexpr size_t LDS_MAX_BYTES = 160 * 1024;
 template <typename K>
 inline void allow_lds(K kernel, size_t bytes) {
-  if (bytes > 64 * 1024)
-    XTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)bytes));
+  if (bytes <= 64 * 1024) return;
+  // the limit is on dynamic LDS: what the kernel's static LDS leaves of the 160 KiB
+  hipFuncAttributes attr;
+  XTB_HIP(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kernel)));
+  const size_t room = LDS_MAX_BYTES - std::min(LDS_MAX_BYTES, (size_t)attr.sharedSizeBytes);
+  if (bytes > room) throw std::runtime_error("allow_lds: more LDS than a workgroup has");
+  XTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)room));
 }
 
 __host__ __device__ inline int64_t idx(int64_t i, int64_t j, int64_t ld) { return i + j * ld; }

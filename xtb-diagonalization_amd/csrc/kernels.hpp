@@ -167,7 +167,11 @@ void density_backward(hipStream_t s, int batch, int n, const double* w, int64_t 
                       double* work);
 
 // refine.hip -------------------------------------------------------------------------------------------------------
-size_t refine_workspace(int n);  // doubles; iwork n ints
+size_t refine_workspace(int n);  // doubles
+size_t refine_iwork(int n);      // ints
+// iterations Ogita-Aishima steps (with Rayleigh-Ritz on clusters) on the FP32 solve's (w, C), the eigenpairs sorted,
+// then the a-posteriori check: status (device int) |= 1 when the result fails it; ratios (device, 2 doubles, may be
+// null) receive the check's residual and orthogonality ratios (<= 1 passes). iterations = 0: nothing is done.
 void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm, int lds, double* c, int ldc,
-               double* w, int iterations, double* work, int* iwork);
+               double* w, int iterations, double* work, int* iwork, int* status, double* ratios);
 }  // namespace xtbeig

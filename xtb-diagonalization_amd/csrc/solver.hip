@@ -4,11 +4,12 @@
 // Generalized solve H C = S C Lambda (itype 1), as xTB's fact_solve path (the reference: scripts/kernels/
 // eigbench.f90:101-168, scripts/kernels/congruence.c:1-52):
 //   S = U^T U and X = U^{-1}                          FP64 always (chol.hip)
-//   H' = X^T H X                                      two GEMMs, FP64 or FP32
+//   H' = X^T H X                                      two GEMMs, FP64 always (rounded to FP32 for the FP32 solve)
 //   H' Z = Z Lambda                                    syevd_core: dense -> band (MFMA), bulge chase, FP64 D&C,
 //                                                      bulge and band back-transforms, FP64 or FP32
 //   C = X Z                                           GEMM, FP64 or FP32
-//   FP32 only: FP64 Ogita-Aishima refinement of (Lambda, C) against (H, S), then the eigenpair sort.
+//   FP32 only: FP64 Ogita-Aishima refinement of (Lambda, C) against (H, S), the eigenpair sort and the a-posteriori
+//   check (status word 3).
 #include "solver.hpp"
 
 #include <algorithm>
@@ -249,7 +250,10 @@ size_t syevd_bytes(int n, int b) {
 // per thread of one workgroup), and the divide and conquer stages each merge in LDS: larger solves are refused before
 // any work is enqueued.
 static void check_size(int n, int b) {
-  if (n - b > 2048)
+  if (b > MAX_BAND)
+    throw std::runtime_error("band width " + std::to_string(b) + " exceeds the widest supported band, " +
+                             std::to_string(MAX_BAND));
+  if (n - b > MAX_N)
     throw std::runtime_error("n = " + std::to_string(n) + " exceeds the largest supported size, 2048 + band width (" +
                              std::to_string(b) + ")");
 }
@@ -281,13 +285,12 @@ size_t sygvd_bytes(int n, int b, bool mixed) {
     carve_std<double>(c, n, b);
   } else {
     c.take<float>(nn);  // X32
-    c.take<float>(nn);  // H32
-    c.take<float>(nn);  // H X, then X Q1 Q2
-    c.take<float>(nn);  // H'
+    c.take<float>(nn);  // X Q1 Q2
+    c.take<float>(nn);  // H' rounded
     c.take<float>(nn);  // C32
     carve_std<float>(c, n, b);
-    c.take<double>(refine_workspace(n));
-    c.take<int>(n);
+    c.take<double>(refine_workspace(n));  // also H X and H' in FP64 before the refinement
+    c.take<int>(refine_iwork(n));
   }
   return c.off + 4096;
 }
@@ -320,21 +323,26 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
     syevd_core<double>(h, s, n, b, hp, n, w, cm, ldc, x, n, hx);
   } else {
     float* x32 = c.take<float>(nn);
-    float* h32 = c.take<float>(nn);
-    float* hx = c.take<float>(nn);
+    float* pm = c.take<float>(nn);
     float* hp = c.take<float>(nn);
     float* c32 = c.take<float>(nn);
     StdBuffers<float> s = carve_std<float>(c, n, b);
     double* rwork = c.take<double>(refine_workspace(n));
-    int* riwork = c.take<int>(n);
+    int* riwork = c.take<int>(refine_iwork(n));
+    // H' = X^T H X in FP64, then rounded: formed in FP32, its rounding error grows with cond(S) (about
+    // eps32 cond(S) ||H||), which at cond(S) = 1e4 leaves the FP32 eigenvectors too far off for the refinement to
+    // converge; in FP64 the FP32 solve starts from eps32 ||H'|| whatever S. The two products are a small part of
+    // the solve. H X and H' sit in the refinement's workspace, which is free until the refinement starts.
+    double* hx64 = rwork;
+    double* hp64 = rwork + nn;
     copy_matrix<double, float>(st, n, n, x, n, x32, n);
-    copy_matrix<double, float>(st, n, n, hm, ldh, h32, n);
-    gemm<float>(st, 'N', 'N', n, n, n, 1.0f, h32, n, x32, n, 0.0f, hx, n);
-    gemm_upper_tn<float>(st, n, n, 1.0f, x32, n, hx, n, 0.0f, hp, n);
-    symmetrize<float>(st, 'U', n, hp, n);
-    syevd_core<float>(h, s, n, b, hp, n, w, c32, n, x32, n, hx);
+    gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, n, 0.0, hx64, n);
+    gemm_upper_tn<double>(st, n, n, 1.0, x, n, hx64, n, 0.0, hp64, n);
+    symmetrize<double>(st, 'U', n, hp64, n);
+    copy_matrix<double, float>(st, n, n, hp64, n, hp, n);
+    syevd_core<float>(h, s, n, b, hp, n, w, c32, n, x32, n, pm);
     copy_matrix<float, double>(st, n, n, c32, n, cm, ldc);
-    refine_oa(st, n, hm, ldh, sm, lds, cm, ldc, w, h.refine_iters, rwork, riwork);
+    refine_oa(st, n, hm, ldh, sm, lds, cm, ldc, w, h.refine_iters, rwork, riwork, h.dinfo + 3, h.check_ratios());
   }
 }
 

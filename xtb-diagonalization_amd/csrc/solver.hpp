@@ -11,6 +11,13 @@
 
 namespace xtbeig {
 
+// status words per handle (xtbeig_synchronize)
+constexpr int NSTATUS = 4;
+// the widest band the dense-to-band panels, the chase and the back-transforms take, and the largest n of a solve
+// (the panel factorization holds a column of up to 2048 rows below the band in one workgroup's registers)
+constexpr int MAX_BAND = 32;
+constexpr int MAX_N = 2048;
+
 struct Handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -24,8 +31,10 @@ struct Handle {
   // second device buffer, for work that wraps a solve (the solve itself carves the arena)
   void* aux = nullptr;
   size_t aux_bytes = 0;
-  // device status words: [0] potrf info, [1] chase stall, [2] stedc info
+  // device status words: [0] potrf info, [1] chase stall, [2] stedc info, [3] refinement check failed; doubles at
+  // dinfo + 32: the last refinement check's two ratios (residual, orthogonality; <= 1 passes)
   int* dinfo = nullptr;
+  double* check_ratios() const { return reinterpret_cast<double*>(dinfo + 32); }
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   // device buffers of the host-array entry points, kept between calls (grown on demand): slot k is the k-th buffer
@@ -69,7 +78,8 @@ struct Handle {
     GraphEntry& operator=(const GraphEntry&) = delete;
     ~GraphEntry();
   };
-  std::map<std::tuple<int, int, int>, std::unique_ptr<GraphEntry>> graphs;
+  // keyed by (n, precision, band, refinement steps): the step count is a host loop, baked into the capture
+  std::map<std::tuple<int, int, int, int>, std::unique_ptr<GraphEntry>> graphs;
   std::vector<std::unique_ptr<Handle>> subs;
 };
 

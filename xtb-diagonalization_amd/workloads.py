@@ -7,6 +7,9 @@ Inputs (all column-major float64, symmetric):
   * overlap_matrix(n): S = I/2 + G/2, G_ij = exp(-|r_i - r_j|^2 / (2 * 1.2^2)) for n points uniform in a cube of
     side (n / 0.8)^(1/3). (The construction gives cond(S) of about 20 at these n: the I/2 shift bounds the smallest
     eigenvalue by 1/2.)
+  * overlap_matrix_cond(n, cond): the same G shifted and scaled to cond_2(S) = cond (1e3 - 1e4 in the
+    configurations); generalized_pair_cond pairs it with H either overlap-weighted (the generalized spectrum is the
+    spectrum above) or independent of S as in generalized_pair.
 Seeds are fixed per configuration: 1000 + configuration index.
 
 Gates (the reference's accuracy quantities, Sources/EigenBench/main.swift:233-251 and scripts/kernels/
@@ -77,6 +80,46 @@ def overlap_matrix(n, rng):
     return np.asfortranarray(0.5 * (s + s.T))
 
 
+def gaussian_overlap(n, rng):
+    """G_ij = exp(-|r_i - r_j|^2 / (2 * 1.2^2)) for n points uniform in a cube of side (n / 0.8)^(1/3)."""
+    side = (n / 0.8) ** (1.0 / 3.0)
+    r = rng.uniform(0.0, side, (n, 3))
+    sq = np.einsum("ij,ij->i", r, r)
+    d2 = np.maximum(sq[:, None] + sq[None, :] - 2.0 * (r @ r.T), 0.0)
+    g = np.exp(-d2 / (2.0 * 1.2 ** 2))
+    return 0.5 * (g + g.T)
+
+
+def overlap_matrix_cond(n, rng, cond):
+    """S = (G + delta I) / (1 + delta), G the Gaussian overlap, delta chosen so that cond_2(S) = cond exactly
+    (delta = (g_max - cond g_min) / (cond - 1) from G's extreme eigenvalues); unit diagonal like an overlap matrix.
+    The configurations ask for cond(S) of 1e3 - 1e4, which S = I/2 + G/2 cannot reach."""
+    g = gaussian_overlap(n, rng)
+    ev = np.linalg.eigvalsh(g)
+    delta = (ev[-1] - cond * ev[0]) / (cond - 1.0)
+    s = (g + delta * np.eye(n)) / (1.0 + delta)
+    return np.asfortranarray(0.5 * (s + s.T))
+
+
+def generalized_pair_cond(n, seed, cond, physical=True):
+    """(H, S, lambda) with cond_2(S) = cond. physical: H = U^T (Q diag(lambda) Q^T) U with S = U^T U, so the
+    generalized eigenvalues are exactly the spectrum lambda (as for an overlap-weighted Hamiltonian, and what the
+    degenerate and near-degenerate pairs of the spectrum test); otherwise H = Q diag(lambda) Q^T as in
+    generalized_pair, whose generalized eigenvalues grow to about ||H|| / lambda_min(S) (lambda is then not the
+    generalized spectrum and None is returned for it)."""
+    rng = np.random.default_rng(seed)
+    lam = spectrum(n, rng)
+    q = householder_product(n, rng)
+    m = (q * lam) @ q.T
+    m = 0.5 * (m + m.T)
+    s = overlap_matrix_cond(n, rng, cond)
+    if physical:
+        u = np.linalg.cholesky(s).T
+        h = u.T @ m @ u
+        return np.asfortranarray(0.5 * (h + h.T)), s, np.sort(lam)
+    return np.asfortranarray(m), s, None
+
+
 def generalized_pair(n, seed):
     """(H, S) of the generalized configurations: H as standard_matrix, S = overlap_matrix, one seed."""
     rng = np.random.default_rng(seed)
@@ -107,11 +150,19 @@ def density_problem(n, seed):
     return np.asfortranarray(0.5 * (h + h.T)), s, np.sort(lam)
 
 
-def gates(h, s, w, c, w_ref):
-    """The three contract quantities, normalized so that each passes at <= 1."""
+def gates(h, s, w, c, w_ref, scaled=False):
+    """The three contract quantities, normalized so that each passes at <= 1.
+
+    scaled: the normalizations of a problem whose generalized eigenvalues are much larger than ||H||_2 (H independent
+    of an ill-conditioned S): the eigenvalue gate relative to max(||H||_2, max |w_ref|) and the residual gate relative
+    to ||H||_2 + max |w_ref| ||S||_2, the scales of the rounding errors of any FP64 method on such a problem (the FP64
+    host LAPACK itself misses the unscaled eigenvalue gate there)."""
     n = h.shape[0]
     nh = np.linalg.norm(h, 2)
-    eig = np.max(np.abs(np.asarray(w) - np.asarray(w_ref))) / nh
+    wmax = float(np.max(np.abs(w_ref)))
+    eig = np.max(np.abs(np.asarray(w) - np.asarray(w_ref))) / (max(nh, wmax) if scaled else nh)
+    if scaled and s is not None:
+        nh = nh + wmax * np.linalg.norm(s, 2)
     if s is None:
         r = h @ c - c * w
         o = np.max(np.abs(c.T @ c - np.eye(n)))

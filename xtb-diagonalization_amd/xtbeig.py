@@ -79,6 +79,7 @@ SIGNATURES = {
     "xtbeig_set_tuning": (ci, [vp, ci, ci]),
     "xtbeig_default_band": (ci, [ci, ci]),
     "xtbeig_synchronize": (ci, [vp, c_int_p]),
+    "xtbeig_refine_check": (ci, [vp, c_dbl_p]),
     "xtbeig_last_error": (ctypes.c_char_p, []),
     "xtbeig_dev_sygvd": (ci, [vp, ci, ci, vp, ci, vp, ci, vp, vp, ci]),
     "xtbeig_dev_dsyevd": (ci, [vp, ci, vp, ci, vp, vp, ci]),
@@ -296,11 +297,18 @@ class Handle:
         self._check(lib().xtbeig_set_tuning(self._h, band, refine_iterations))
 
     def synchronize(self):
-        st = (ctypes.c_int * 3)()
+        """Wait; the four status words [Cholesky info, chase stall, D&C info, refinement check], cleared."""
+        st = (ctypes.c_int * 4)()
         rc = lib().xtbeig_synchronize(self._h, st)
         if rc == 1:
             raise RuntimeError(last_error())
         return list(st)
+
+    def refine_check(self):
+        """The last refinement check's (residual, orthogonality) ratios; a solve passes with both <= 1."""
+        r = (ctypes.c_double * 2)()
+        self._check(lib().xtbeig_refine_check(self._h, r))
+        return list(r)
 
     def sygvd(self, h, s, w, c, precision=FP64):
         n = h.shape[0]
