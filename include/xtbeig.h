@@ -150,6 +150,14 @@ int xtbeig_dev_sgemm(xtbeig_handle_t handle, char transa, char transb, int m, in
 /* S = U^T U and X = U^{-1} (upper; zero below the diagonal). s (full storage) is destroyed. */
 int xtbeig_dev_potrf_inverse(xtbeig_handle_t handle, int n, double *s, int lds, double *u, int ldu, double *x,
                              int ldx);
+/* The same as xtbeig_dev_potrf_inverse with no device allocation: no hipMalloc or hipFree, no allocating library
+ * call; its scratch buffer (10 MiB) is a static device array that each device holds until the process ends. No
+ * counterpart in the reference: for callers that cannot allocate (code under HIP graph capture, tight loops,
+ * harnesses that forbid hipMalloc). n <= 2048 (a larger n returns nonzero with the reason in xtbeig_last_error()).
+ * All calls on one device share its scratch array: calls that overlap on one device race. Cholesky info in status
+ * word 0 (xtbeig_synchronize), as for xtbeig_dev_potrf_inverse. */
+int xtbeig_dev_potrf_inverse_noalloc(xtbeig_handle_t handle, int n, double *s, int lds, double *u, int ldu,
+                                     double *x, int ldx);
 /* Dense -> band (reference: Stages/BandForm.swift:28-142): a (full symmetric, destroyed) -> lower band of width b
  * in band storage ab (element (i, j), 0 <= i - j < 2b, at i + (2b - 1) j; xtbeig_band_elems(n, b) elements, which
  * include b padding columns past n that the chase clears and uses), the panels' Householder vectors v1 (n x n) and

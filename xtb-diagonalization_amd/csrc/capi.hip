@@ -487,6 +487,14 @@ int xtbeig_dev_potrf_inverse(xtbeig_handle_t handle, int n, double* s, int lds, 
   });
 }
 
+int xtbeig_dev_potrf_inverse_noalloc(xtbeig_handle_t handle, int n, double* s, int lds, double* u, int ldu,
+                                     double* x, int ldx) {
+  if (!handle) return set_error("xtbeig_dev_potrf_inverse_noalloc: null handle");
+  if (n < 0 || lds < (n > 1 ? n : 1) || ldu < (n > 1 ? n : 1) || ldx < (n > 1 ? n : 1))
+    return set_error("xtbeig_dev_potrf_inverse_noalloc: bad dimensions");
+  return guarded([&] { xtbeig::potrf_inverse_noalloc(handle->h, n, s, lds, u, ldu, x, ldx); });
+}
+
 long long xtbeig_band_elems(int n, int b) { return (long long)xtbeig::band_elems(n, b); }
 
 int xtbeig_dev_dreduce_to_band(xtbeig_handle_t handle, int n, int b, double* a, int lda, double* v1, double* t1,

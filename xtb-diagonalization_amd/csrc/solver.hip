@@ -36,6 +36,7 @@ Handle::Handle(int dev) : device(dev) {
   XTB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   XTB_HIP(hipMalloc(&dinfo, 64 * sizeof(int)));
   XTB_HIP(hipMemset(dinfo, 0, 64 * sizeof(int)));
+  scratch_dev = noalloc_scratch(device);
 }
 
 Handle::~Handle() {

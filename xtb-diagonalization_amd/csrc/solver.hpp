@@ -17,6 +17,8 @@ constexpr int NSTATUS = 4;
 // (the panel factorization holds a column of up to 2048 rows below the band in one workgroup's registers)
 constexpr int MAX_BAND = 32;
 constexpr int MAX_N = 2048;
+// the largest n of the allocation-free Cholesky entry point (its scratch array is sized for it)
+constexpr int NOALLOC_MAX_N = 2048;
 
 struct Handle {
   int device = 0;
@@ -35,6 +37,8 @@ struct Handle {
   // dinfo + 32: the last refinement check's two ratios (residual, orthogonality; <= 1 passes)
   int* dinfo = nullptr;
   double* check_ratios() const { return reinterpret_cast<double*>(dinfo + 32); }
+  // device address of the allocation-free entry point's scratch array on this handle's device (noalloc.hip)
+  void* scratch_dev = nullptr;
   void* pinned = nullptr;
   size_t pinned_bytes = 0;
   // device buffers of the host-array entry points, kept between calls (grown on demand): slot k is the k-th buffer
@@ -82,6 +86,12 @@ struct Handle {
   std::map<std::tuple<int, int, int, int>, std::unique_ptr<GraphEntry>> graphs;
   std::vector<std::unique_ptr<Handle>> subs;
 };
+
+// noalloc.hip: the scratch array's device address on `device` (current), looked up once per device; the
+// allocation-free Cholesky + inverse on it.
+void* noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
+void potrf_inverse_noalloc(Handle& h, int n, double* s, int lds, double* u, int ldu, double* x, int ldx);
 
 // n bucket edges of the tuned table: bucket q holds bucket_edge(q) <= n < bucket_edge(q + 1)
 int bucket_of(int n);

@@ -90,6 +90,7 @@ SIGNATURES = {
     "xtbeig_dev_sgemm": (ci, [vp, ctypes.c_char, ctypes.c_char, ci, ci, ci, ctypes.c_float, vp, ci, vp, ci,
                               ctypes.c_float, vp, ci]),
     "xtbeig_dev_potrf_inverse": (ci, [vp, ci, vp, ci, vp, ci, vp, ci]),
+    "xtbeig_dev_potrf_inverse_noalloc": (ci, [vp, ci, vp, ci, vp, ci, vp, ci]),
     "xtbeig_band_elems": (ctypes.c_longlong, [ci, ci]),
     "xtbeig_dev_dreduce_to_band": (ci, [vp, ci, ci, vp, ci, vp, vp, vp]),
     "xtbeig_dev_sreduce_to_band": (ci, [vp, ci, ci, vp, ci, vp, vp, vp]),
