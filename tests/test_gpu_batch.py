@@ -21,7 +21,8 @@ def solve_eager(torch, hnd, h, s, prec):
     n = h.shape[0]
     w = torch.zeros(n, dtype=torch.float64, device="cuda")
     c = torch.zeros((n, n), dtype=torch.float64, device="cuda")
-    hnd.sygvd(dev(torch, h), dev(torch, s), w, c, prec)
+    hd, sd = dev(torch, h), dev(torch, s)  # referenced until the solve is done (it runs on the library's stream)
+    hnd.sygvd(hd, sd, w, c, prec)
     assert hnd.synchronize() == [0, 0, 0, 0]
     return w.cpu().numpy(), c.cpu().numpy()
 

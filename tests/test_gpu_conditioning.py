@@ -25,7 +25,8 @@ def solve(torch, handle, h, s, prec):
     n = h.shape[0]
     w = torch.zeros(n, dtype=torch.float64, device="cuda")
     c = torch.zeros((n, n), dtype=torch.float64, device="cuda")
-    handle.sygvd(dev(torch, h), dev(torch, s), w, c, prec)
+    hd, sd = dev(torch, h), dev(torch, s)  # referenced until the solve is done (it runs on the library's stream)
+    handle.sygvd(hd, sd, w, c, prec)
     st = handle.synchronize()
     return w.cpu().numpy(), c.cpu().numpy().T, st
 
