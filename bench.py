@@ -9,7 +9,8 @@ FP32+refine rate; the FP64 rate and their ratio are reported beside it.
 roofline: the dominant kernel of the FP32+refine solve (largest share of its time in the kernel trace) timed alone
 with HIP events on the handle's stream, against its bound (HBM bytes or MFMA flops per launch).
 cpu_baseline: the oracle (C restatement of the reference's 1-stage reduction and gemm back-transformation, with the
-host LAPACK's dpotrf / dsygst / dstedc / dtrsm) on the host, one core, a bounded sample of the same workload.
+host LAPACK's dpotrf / dsygst / dstedc / dtrsm) on the host, one core, a bounded sample of the same workload;
+cpu_baselines adds plain host dsygvd at one BLAS thread and at every host core this process may use.
 """
 import argparse
 import json
@@ -136,6 +137,93 @@ def cpu_baseline(h, s, sample=5):
         c = hl.dtrsm_left_upper(u, z)
         times.append(time.perf_counter() - t0)
     return sum(times) / len(times), w
+
+
+def cpu_lapack_baseline(h, s, threads, sample=5):
+    """Plain host LAPACK dsygvd(1, 'V', 'U') on `threads` BLAS threads: the median of `sample` solves after one
+    untimed warm-up."""
+    import numpy as np
+    import host_lapack as hl
+    hl.set_threads(threads)
+    try:
+        w, _, info = hl.dsygvd(h, s)
+        if info != 0:
+            raise RuntimeError(f"host dsygvd info {info}")
+        times = []
+        for _ in range(sample):
+            t0 = time.perf_counter()
+            w, _, info = hl.dsygvd(h, s)
+            times.append(time.perf_counter() - t0)
+    finally:
+        hl.set_threads(1)
+    return float(np.median(times)), w
+
+
+def host_cores():
+    """CPU threads this process may use: the affinity mask, capped by OMP_NUM_THREADS when that is set."""
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))
+    return cores
+
+
+def kernels(torch, np, workloads, xtbeig, hnd, h, s, which, reps=5):
+    """The hot kernels at n = 1204, each run `reps` times with nothing else in flight, for counter collection
+    (rocprofv3 --pmc): FP64 and FP32 GEMM n^3 (gemm_kernel) and the symmetric congruence product (gemm_upper_kernel),
+    the dense-to-band reduction at b = 8 in both precisions (panel kernels), the FP32 bulge chase at b = 8, and the
+    one-stage reduction (trd_symv_kernel and the dlatrd column kernels). `which`: the groups to run (gemm, band,
+    chase, sytrd). Prints one line per stage."""
+    n = N
+    for dt in ((torch.float64, torch.float32) if "gemm" in which else ()):
+        a = torch.randn(n, n, dtype=dt, device="cuda")
+        b = torch.randn(n, n, dtype=dt, device="cuda")
+        c = torch.zeros(n, n, dtype=dt, device="cuda")
+        for _ in range(reps):
+            hnd.gemm("N", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n)
+        hnd.synchronize()
+        log(f"gemm NN {dt} x{reps}")
+    a, _ = workloads.standard_matrix(n, 11)
+    for dt in ((torch.float64, torch.float32) if ("band" in which or "chase" in which) else ()):
+        bw = xtbeig.default_band(n, fp32=dt == torch.float32)
+        A0 = torch.from_numpy(np.ascontiguousarray(a.T)).cuda().to(dt)
+        V1 = torch.zeros((n, n), dtype=dt, device="cuda")
+        T1 = torch.zeros((n, bw), dtype=dt, device="cuda")
+        AB = torch.zeros(xtbeig.band_elems(n, bw), dtype=dt, device="cuda")
+        for _ in range(reps):
+            A = A0.clone()
+            torch.cuda.synchronize()
+            hnd.reduce_to_band(A, bw, V1, T1, AB)
+            hnd.synchronize()
+        log(f"reduce_to_band b={bw} {dt} x{reps}")
+        if dt == torch.float32 and "chase" in which:
+            AB0 = AB.clone()
+            V2 = torch.zeros((n, n), dtype=dt, device="cuda")
+            D = torch.zeros(n, dtype=dt, device="cuda")
+            E = torch.zeros(n, dtype=dt, device="cuda")
+            for _ in range(reps):
+                ABc = AB0.clone()
+                torch.cuda.synchronize()
+                hnd.chase(n, bw, ABc, V2, D, E)
+                hnd.synchronize()
+            log(f"chase b={bw} {dt} x{reps}")
+    if "sytrd" not in which:
+        return
+    import ctypes
+    L = xtbeig.lib()
+    A0 = torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    e = torch.zeros(n, dtype=torch.float64, device="cuda")
+    tau = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        A = A0.clone()
+        torch.cuda.synchronize()
+        rc = L.xtbeig_dev_dsytrd_upper(hnd._h, n, ctypes.c_void_p(A.data_ptr()), n, ctypes.c_void_p(d.data_ptr()),
+                                       ctypes.c_void_p(e.data_ptr()), ctypes.c_void_p(tau.data_ptr()), 0)
+        if rc != 0:
+            raise RuntimeError(xtbeig.last_error())
+        hnd.synchronize()
+    log("one-stage dsytrd x2")
 
 
 def components(torch, np, workloads, xtbeig, hnd, h, s):
@@ -294,6 +382,9 @@ def main():
     ap.add_argument("--pmc-calibration", action="store_true", help="run a 256 MiB copy first (counter scaling)")
     ap.add_argument("--components", action="store_true",
                     help="time each entry point of the path alone at n = 1204 (one JSON line each) and exit")
+    ap.add_argument("--kernels", default=None, metavar="GROUPS",
+                    help="run the hot kernels alone a few times each (for rocprofv3 counter runs) and exit; GROUPS "
+                         "is a comma-separated subset of gemm,band,chase,sytrd")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the eigensolves are "
@@ -315,11 +406,14 @@ def main():
     w = torch.zeros(N, dtype=torch.float64, device="cuda")
     c = torch.zeros((N, N), dtype=torch.float64, device="cuda")
     hnd = xtbeig.Handle(0)
+    if args.pmc_calibration:
+        pmc_calibration(torch)
     if args.components:
         components(torch, np, workloads, xtbeig, hnd, h, s)
         return
-    if args.pmc_calibration:
-        pmc_calibration(torch)
+    if args.kernels:
+        kernels(torch, np, workloads, xtbeig, hnd, h, s, set(args.kernels.split(",")))
+        return
     band = xtbeig.default_band(N, fp32=True)
 
     res = {}
@@ -344,7 +438,9 @@ def main():
     log(roof)
 
     cpu = None
+    cpu_all = None
     if not args.no_cpu:
+        import host_lapack as hl
         log("cpu baseline (oracle, 1 core) ...")
         t, wc = cpu_baseline(h, s, sample=args.cpu_sample)
         cpu = {"value": 1.0 / t, "unit": "gen. eigensolves/s", "cores": 1, "kind": "port",
@@ -354,6 +450,18 @@ def main():
                "seconds_per_solve": t,
                "eig_err_vs_lapack": float(np.max(np.abs(wc - wref)) / nh)}
         log(cpu)
+        cores = host_cores()
+        cpu_all = [cpu]
+        for th in sorted({1, cores}):
+            log(f"cpu baseline (host dsygvd, {th} threads) ...")
+            tl, wl = cpu_lapack_baseline(h, s, th, sample=args.cpu_sample)
+            cpu_all.append({"value": 1.0 / tl, "unit": "gen. eigensolves/s", "cores": th, "kind": "host_lapack",
+                            "sample": f"{args.cpu_sample} plain host dsygvd(1, 'V', 'U') solves at n=1204 of the same "
+                                      "workload after one warm-up, rate of the median",
+                            "seconds_per_solve": tl, "blas_threads": th, "library": os.path.basename(hl.LIBRARY),
+                            "host_cpus_affinity": len(os.sched_getaffinity(0)), "nproc": os.cpu_count(),
+                            "eig_err_vs_lapack": float(np.max(np.abs(wl - wref)) / nh)})
+            log(cpu_all[-1])
 
     mixed, fp64 = res["fp32_refine"], res["fp64"]
     flops = 7.0 * N ** 3
@@ -385,6 +493,7 @@ def main():
         "roofline_kernel": roof["kernel"],
         "roofline_ms_per_launch": roof["ms_per_launch"],
         "cpu_baseline": cpu,
+        "cpu_baselines": cpu_all,
     }
     print(json.dumps(out))
 

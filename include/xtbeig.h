@@ -114,6 +114,12 @@ void *xtbeig_get_stream(xtbeig_handle_t handle);
 /* Band width of the two-stage reduction (0: the built-in table per n; at most 32) and the refinement iterations
  * (0 to 8; 0 returns the FP32 eigenpairs unrefined and unchecked). */
 int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations);
+/* Test setting: cap the divide and conquer's secular-equation iterations at secular_maxit (0 gives up after the
+ * starting guess; -1 restores the solver's own limit), so that the failure path (info > 0 from stedc / syevd /
+ * sygvd, status word 2) can be exercised without faulting anything. handle NULL: the calling thread's handle of the
+ * host-array entry points. No reference counterpart (the reference propagates a failed dlaed4 as info,
+ * Stages/ParallelDivideAndConquer.swift:70-71). */
+int xtbeig_set_test_limits(xtbeig_handle_t handle, int secular_maxit);
 /* The built-in band width for an n x n solve (fp32 nonzero: the FP32 reduction of the mixed path). No reference
  * counterpart beyond SBROptions.recommended (SBROptions.swift:73-82), which picks b per n the same way. */
 int xtbeig_default_band(int n, int fp32);
@@ -256,6 +262,19 @@ int xtbeig_dev_density_backward(xtbeig_handle_t handle, int batch, int n, const 
                                 const double *c, int ldc, long long sc, double kT, const double *mu, int mu_fixed,
                                 const double *occ, const double *gbar, int ldg, long long sg, double *hbar, int ldhb,
                                 long long shb, double *sbar, int ldsb, long long ssb);
+/* Open-shell occupations, as the reference forms them (scripts/kernels/refine.f90:85-93): alpha and beta electrons
+ * Fermi-smeared separately, f = f_alpha + f_beta, with chemical potentials mu[2 p] (sum f_alpha = nel_alpha) and
+ * mu[2 p + 1] (sum f_beta = nel_beta), each bisected when bisect != 0; P = C diag(f) C^T. occ: 6 n per problem (the
+ * alpha channel's f | f' | log cosh, then the beta channel's), for the backward below. */
+int xtbeig_dev_density_spin(xtbeig_handle_t handle, int batch, int n, const double *w, long long sw, const double *c,
+                            int ldc, long long sc, double kT, double nel_alpha, double nel_beta, double *mu, int bisect,
+                            double *p, int ldp, long long sp, double *occ);
+/* Backward of the open-shell density matrix: the sums over the two channels of the closed-shell terms (one chemical
+ * potential term per channel unless mu_fixed); mu and occ as xtbeig_dev_density_spin left them. */
+int xtbeig_dev_density_backward_spin(xtbeig_handle_t handle, int batch, int n, const double *w, long long sw,
+                                     const double *c, int ldc, long long sc, double kT, const double *mu,
+                                     int mu_fixed, const double *occ, const double *gbar, int ldg, long long sg,
+                                     double *hbar, int ldhb, long long shb, double *sbar, int ldsb, long long ssb);
 /* The same backward without stored eigenvectors: each problem's eigenpairs are recomputed from (h, s) with the
  * given precision (XTBEIG_FP64 or XTBEIG_FP32_REFINE) and mu re-bisected to nel, one problem at a time in O(n^2)
  * device memory (for batches whose eigenvectors do not fit). */

@@ -253,3 +253,78 @@ def test_autograd_function(handle, torch_cuda, recompute):
     (P * torch.tensor(gbar, device="cuda")).sum().backward()
     assert np.max(np.abs(H.grad.cpu().numpy() - rh)) < 1e-8 * max(1.0, np.max(np.abs(rh)))
     assert np.max(np.abs(S.grad.cpu().numpy() - rs)) < 1e-8 * max(1.0, np.max(np.abs(rs)))
+
+
+def np_density_spin(h, s, kT, na, nb):
+    w, c = sl.eigh(h, s)
+    ma, mb = np_mu(w, kT, na), np_mu(w, kT, nb)
+    fa = 1.0 / (1.0 + np.exp((w - ma) / kT))
+    fb = 1.0 / (1.0 + np.exp((w - mb) / kT))
+    return (c * (fa + fb)) @ c.T, (ma, mb), w, c, fa, fb
+
+
+@pytest.mark.parametrize("kT", KTS)
+def test_density_spin_forward_and_closed_shell_limit(handle, torch_cuda, kT):
+    """Open-shell occupations (the reference's alpha / beta smearing, refine.f90:85-93) against numpy; with equal
+    counts the open-shell P and gradient are exactly twice the closed-shell ones of the same count."""
+    torch = torch_cuda
+    n = 96
+    h, s, _ = workloads.density_problem(n, workloads.SEED_BASE + 4)
+    na, nb = n / 2 + 2, n / 2 - 2
+    pn, (ma, mb), w, c, fa, fb = np_density_spin(h, s, kT, na, nb)
+    wd = torch.from_numpy(w[None, :].copy()).cuda()
+    cd = to_dev(torch, [c])
+    mu = torch.zeros((1, 2), dtype=torch.float64, device="cuda")
+    p, occ = handle.density_spin(wd, cd, kT, na, nb, mu)
+    handle.synchronize()
+    # mu itself is not pinned where a count falls in a gap wider than kT (the count is flat there to rounding): the
+    # counts it gives and P are
+    for q, nq in ((0, na), (1, nb)):
+        cnt = np.sum(1.0 / (1.0 + np.exp((w - mu[0, q].item()) / kT)))
+        assert abs(cnt - nq) < 1e-10, (q, cnt, nq)
+    pg = from_dev(p)[0]
+    assert np.max(np.abs(pg - pn)) < 1e-12
+    assert abs(np.trace(pg @ s) - (na + nb)) < 1e-10
+    o = occ.cpu().numpy()[0]
+    assert abs(o[:n].sum() - na) < 1e-10 and abs(o[3 * n:4 * n].sum() - nb) < 1e-10
+    # equal counts: exactly twice the closed-shell result
+    m = n / 2
+    mu2 = torch.zeros((1, 2), dtype=torch.float64, device="cuda")
+    p2, occ2 = handle.density_spin(wd, cd, kT, m, m, mu2)
+    mu1 = torch.zeros(1, dtype=torch.float64, device="cuda")
+    p1, occ1 = handle.density(wd, cd, kT, m, mu1)
+    gb = to_dev(torch, [random_sym(n, np.random.default_rng(4))])
+    hb2, sb2 = handle.density_backward_spin(wd, cd, kT, mu2, occ2, gb)
+    hb1, sb1 = handle.density_backward(wd, cd, kT, mu1, occ1, gb)
+    handle.synchronize()
+    assert torch.equal(p2, 2 * p1)
+    assert torch.equal(hb2, 2 * hb1) and torch.equal(sb2, 2 * sb1)
+
+
+@pytest.mark.parametrize("kT", KTS)
+def test_density_spin_backward_finite_differences(handle, torch_cuda, kT):
+    """Open-shell backward (both chemical potentials re-bisected) against five-point finite differences at h = 1e-6,
+    1e-7 relative, on the spectrum with near-degenerate pairs."""
+    torch = torch_cuda
+    n = 96
+    h, s, _ = workloads.density_problem(n, workloads.SEED_BASE + 4)
+    na, nb = n / 2 + 1, n / 2 - 1
+    rng = np.random.default_rng(21)
+    gbar = random_sym(n, rng)
+    _, _, w, c, _, _ = np_density_spin(h, s, kT, na, nb)
+    wd = torch.from_numpy(w[None, :].copy()).cuda()
+    cd = to_dev(torch, [c])
+    mu = torch.zeros((1, 2), dtype=torch.float64, device="cuda")
+    _, occ = handle.density_spin(wd, cd, kT, na, nb, mu)
+    hb, sb = handle.density_backward_spin(wd, cd, kT, mu, occ, to_dev(torch, [gbar]))
+    handle.synchronize()
+    hg, sg = from_dev(hb)[0], from_dev(sb)[0]
+
+    def loss(hh, ss):
+        return np.sum(gbar * np_density_spin(hh, ss, kT, na, nb)[0])
+
+    uh, us = hg / np.linalg.norm(hg), sg / np.linalg.norm(sg)
+    fh = fd5(lambda t: loss(h + t * uh, s), 1e-6)
+    fs = fd5(lambda t: loss(h, s + t * us), 1e-6)
+    assert abs(np.linalg.norm(hg) - fh) <= 1e-7 * abs(fh)
+    assert abs(np.linalg.norm(sg) - fs) <= 1e-7 * abs(fs)

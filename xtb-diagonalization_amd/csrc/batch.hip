@@ -122,6 +122,7 @@ void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, 
   for (auto& sub : h.subs) {
     sub->band_override = h.band_override;
     sub->refine_iters = h.refine_iters;
+    sub->test_secular_maxit = h.test_secular_maxit;
     std::copy(&h.tuned_band[0][0], &h.tuned_band[0][0] + 2 * Handle::NBUCKET, &sub->tuned_band[0][0]);
   }
   // longest first onto the least-loaded slot

@@ -420,6 +420,15 @@ int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations) {
 
 int xtbeig_default_band(int n, int fp32) { return xtbeig::default_band(n, fp32 != 0); }
 
+int xtbeig_set_test_limits(xtbeig_handle_t handle, int secular_maxit) {
+  if (secular_maxit < -1) return set_error("xtbeig_set_test_limits: secular_maxit must be >= -1");
+  return guarded([&] {
+    Handle& h = handle ? handle->h : thread_handle();
+    h.test_secular_maxit = secular_maxit;
+    for (auto& sub : h.subs) sub->test_secular_maxit = secular_maxit;
+  });
+}
+
 int xtbeig_synchronize(xtbeig_handle_t handle, int* status) {
   int st[xtbeig::NSTATUS] = {0, 0, 0, 0};
   int rc = guarded([&] { read_status(handle->h, st); });
@@ -615,6 +624,10 @@ void xtb_sstedc(const int* n_, float* d, float* e, float* z, const int* ldz_, fl
 void xtb_dsymmetrize_pad(const int* n_, const double* src, const int* lds_, double* dst, const int* ldd_) {
   const int n = *n_, lds = *lds_, ldd = *ldd_;
   if (n <= 0) return;
+  if (lds < n || ldd < n) {  // nothing is written (the reference's call has no error path: the message is all)
+    set_error("xtb_dsymmetrize_pad: leading dimensions must be >= n");
+    return;
+  }
   guarded([&] {
     Handle& h = thread_handle();
     DevBuf ds(sizeof(double) * (size_t)n * n), dt(sizeof(double) * (size_t)n * n);
@@ -653,27 +666,38 @@ void xtbeig_ssygvd(const int* itype, const char* jobz, const char* uplo, const i
   sygvd_host<float>(true, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
 }
 
+// The void entry points of the sytrd.c symbol set refuse leading dimensions below n before touching the caller's
+// arrays (the message in xtbeig_last_error; the reference's calls have no error path).
+static bool lds_ok(const char* who, int n, int lda) {
+  if (n > 0 && lda < n) {
+    set_error((std::string(who) + ": leading dimension must be >= n").c_str());
+    return false;
+  }
+  return true;
+}
 void xtb_dsytrd_upper(const int* n, double* a, const int* lda, double* d, double* e, double* tau, double* work) {
   (void)work;
-  sytrd_host<double>(*n, a, *lda, d, e, tau, false);
+  if (lds_ok("xtb_dsytrd_upper", *n, *lda)) sytrd_host<double>(*n, a, *lda, d, e, tau, false);
 }
 void xtb_ssytrd_upper(const int* n, float* a, const int* lda, float* d, float* e, float* tau, float* work) {
   (void)work;
-  sytrd_host<float>(*n, a, *lda, d, e, tau, false);
+  if (lds_ok("xtb_ssytrd_upper", *n, *lda)) sytrd_host<float>(*n, a, *lda, d, e, tau, false);
 }
 void xtb_dsytrd_full(const int* n, double* a, const int* lda, double* d, double* e, double* tau, double* work) {
   (void)work;
-  sytrd_host<double>(*n, a, *lda, d, e, tau, true);
+  if (lds_ok("xtb_dsytrd_full", *n, *lda)) sytrd_host<double>(*n, a, *lda, d, e, tau, true);
 }
 void xtb_dormtr_upper(const int* n, const int* m, const double* a, const int* lda, const double* tau, double* z,
                       const int* ldz, double* work) {
   (void)work;
-  ormtr_host<double>(*n, *m, a, *lda, tau, z, *ldz);
+  if (lds_ok("xtb_dormtr_upper", *n, *lda) && lds_ok("xtb_dormtr_upper", *n, *ldz))
+    ormtr_host<double>(*n, *m, a, *lda, tau, z, *ldz);
 }
 void xtb_sormtr_upper(const int* n, const int* m, const float* a, const int* lda, const float* tau, float* z,
                       const int* ldz, float* work) {
   (void)work;
-  ormtr_host<float>(*n, *m, a, *lda, tau, z, *ldz);
+  if (lds_ok("xtb_sormtr_upper", *n, *lda) && lds_ok("xtb_sormtr_upper", *n, *ldz))
+    ormtr_host<float>(*n, *m, a, *lda, tau, z, *ldz);
 }
 void xtbeig_dsytrd(const char* uplo, const int* n, double* a, const int* lda, double* d, double* e, double* tau,
                    double* work, const int* lwork, int* info) {
@@ -723,7 +747,7 @@ int xtbeig_dev_density(xtbeig_handle_t handle, int batch, int n, const double* w
   return guarded([&] {
     Handle& h = handle->h;
     h.reserve_aux(xtbeig::density_workspace(batch, n) * sizeof(double));
-    xtbeig::density_forward(h.stream, batch, n, w, sw, c, ldc, sc, kT, nel, bisect, mu, p, ldp, sp, occ,
+    xtbeig::density_forward(h.stream, batch, n, 1, w, sw, c, ldc, sc, kT, nel, 0.0, bisect, mu, p, ldp, sp, occ,
                             static_cast<double*>(h.aux));
   });
 }
@@ -735,8 +759,35 @@ int xtbeig_dev_density_backward(xtbeig_handle_t handle, int batch, int n, const 
   return guarded([&] {
     Handle& h = handle->h;
     h.reserve_aux(xtbeig::density_workspace(batch, n) * sizeof(double));
-    xtbeig::density_backward(h.stream, batch, n, w, sw, c, ldc, sc, kT, mu, mu_fixed, occ, gbar, ldg, sg, hbar, ldhb,
+    xtbeig::density_backward(h.stream, batch, n, 1, w, sw, c, ldc, sc, kT, mu, mu_fixed, occ, gbar, ldg, sg, hbar, ldhb,
                              shb, sbar, ldsb, ssb, static_cast<double*>(h.aux));
+  });
+}
+
+int xtbeig_dev_density_spin(xtbeig_handle_t handle, int batch, int n, const double* w, long long sw, const double* c,
+                            int ldc, long long sc, double kT, double nel_alpha, double nel_beta, double* mu, int bisect,
+                            double* p, int ldp, long long sp, double* occ) {
+  if (!handle) return set_error("xtbeig_dev_density_spin: null handle");
+  if (!(nel_alpha >= 0) || !(nel_beta >= 0) || nel_alpha > n || nel_beta > n)
+    return set_error("xtbeig_dev_density_spin: electron counts must be in [0, n]");
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve_aux(xtbeig::density_workspace(batch, n) * sizeof(double));
+    xtbeig::density_forward(h.stream, batch, n, 2, w, sw, c, ldc, sc, kT, nel_alpha, nel_beta, bisect, mu, p, ldp, sp,
+                            occ, static_cast<double*>(h.aux));
+  });
+}
+
+int xtbeig_dev_density_backward_spin(xtbeig_handle_t handle, int batch, int n, const double* w, long long sw,
+                                     const double* c, int ldc, long long sc, double kT, const double* mu,
+                                     int mu_fixed, const double* occ, const double* gbar, int ldg, long long sg,
+                                     double* hbar, int ldhb, long long shb, double* sbar, int ldsb, long long ssb) {
+  if (!handle) return set_error("xtbeig_dev_density_backward_spin: null handle");
+  return guarded([&] {
+    Handle& h = handle->h;
+    h.reserve_aux(xtbeig::density_workspace(batch, n) * sizeof(double));
+    xtbeig::density_backward(h.stream, batch, n, 2, w, sw, c, ldc, sc, kT, mu, mu_fixed, occ, gbar, ldg, sg, hbar,
+                             ldhb, shb, sbar, ldsb, ssb, static_cast<double*>(h.aux));
   });
 }
 
@@ -763,9 +814,9 @@ int xtbeig_dev_density_backward_recompute(xtbeig_handle_t handle, int batch, int
                            precision == XTBEIG_FP32_REFINE, h.band_override);
       // the occupations need mu: a forward pass without P (p = nullptr is not allowed; P goes to the backward's
       // first scratch matrix and is overwritten there)
-      xtbeig::density_forward(h.stream, 1, n, wbuf, n, cbuf, n, (int64_t)nn, kT, nel, 1, mu, work + nn, n,
+      xtbeig::density_forward(h.stream, 1, n, 1, wbuf, n, cbuf, n, (int64_t)nn, kT, nel, 0.0, 1, mu, work + nn, n,
                               (int64_t)nn, occ, work);
-      xtbeig::density_backward(h.stream, 1, n, wbuf, n, cbuf, n, (int64_t)nn, kT, mu, mu_fixed, occ,
+      xtbeig::density_backward(h.stream, 1, n, 1, wbuf, n, cbuf, n, (int64_t)nn, kT, mu, mu_fixed, occ,
                                gbar + (int64_t)q * sg, ldg, sg, hbar + (int64_t)q * shb, ldhb, shb,
                                sbar + (int64_t)q * ssb, ldsb, ssb, work);
     }

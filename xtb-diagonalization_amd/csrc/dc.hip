@@ -14,6 +14,30 @@
 //   order_kernel     dlamrg of the K new and n - K deflated eigenvalues, then gather / copy-back of the columns
 // Deflation bookkeeping (K, the index arrays, column types) follows dlaed2 step for step and is compared with the host
 // LAPACK's dlaed2 on the same inputs by the tests.
+//
+// The deflation (deflate_kernel) and the secular solver follow LAPACK's dlaed2 / dlaed4 / dlaed5 / dlaed6 closely (their variable
+// names and branch structure), so LAPACK's license applies to those parts:
+//   Copyright (c) 1992-2017 The University of Tennessee and The University of Tennessee Research Foundation.
+//   Copyright (c) 2000-2017 The University of California Berkeley. Copyright (c) 2006-2017 The University of
+//   Colorado Denver. All rights reserved.
+//   Redistribution and use in source and binary forms, with or without modification, are permitted provided that the
+//   following conditions are met: (1) Redistributions of source code must retain the above copyright notice, this
+//   list of conditions and the following disclaimer. (2) Redistributions in binary form must reproduce the above
+//   copyright notice, this list of conditions and the following disclaimer listed in this license in the
+//   documentation and/or other materials provided with the distribution. (3) Neither the name of the copyright
+//   holders nor the names of its contributors may be used to endorse or promote products derived from this software
+//   without specific prior written permission.
+//   The copyright holders provide no reassurances that the source code provided does not infringe any patent,
+//   copyright, or any other intellectual property rights of third parties. The copyright holders disclaim any
+//   liability to any recipient for claims brought against recipient by any third party for infringement of that
+//   party's intellectual property rights.
+//   THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS IS" AND ANY EXPRESS OR IMPLIED
+//   WARRANTIES, INCLUDING, BUT NOT LIMITED TO, THE IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A PARTICULAR
+//   PURPOSE ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT OWNER OR CONTRIBUTORS BE LIABLE FOR ANY DIRECT, INDIRECT,
+//   INCIDENTAL, SPECIAL, EXEMPLARY, OR CONSEQUENTIAL DAMAGES (INCLUDING, BUT NOT LIMITED TO, PROCUREMENT OF SUBSTITUTE
+//   GOODS OR SERVICES; LOSS OF USE, DATA, OR PROFITS; OR BUSINESS INTERRUPTION) HOWEVER CAUSED AND ON ANY THEORY OF
+//   LIABILITY, WHETHER IN CONTRACT, STRICT LIABILITY, OR TORT (INCLUDING NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY
+//   OUT OF THE USE OF THIS SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF SUCH DAMAGE.
 #include "common.hpp"
 #include "gemm_tile.hpp"
 #include "kernels.hpp"
@@ -477,7 +501,7 @@ __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const No
                                                      const double* __restrict__ w_all, const int* __restrict__ indxc_all,
                                                      double* __restrict__ d_all, int* __restrict__ orig_all,
                                                      double* __restrict__ tau_all, double* __restrict__ s_all,
-                                                     int* info) {
+                                                     int* info, int maxit) {
   extern __shared__ double poles[];  // dlamda (k), then z (k): every root reads all of them each iteration
   constexpr int RPB = 64 / SEC_G;    // roots per workgroup
   const Node nd = nodes[blockIdx.y];
@@ -514,7 +538,7 @@ __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const No
   }
   int orig = 0;
   double tau = 0;
-  const int bad = secular::laed4<SEC_G>(k, j, dl, w, rho, orig, tau, gl);
+  const int bad = secular::laed4<SEC_G>(k, j, dl, w, rho, orig, tau, gl, maxit);
   if (gl != 0) return;
   if (bad) atomicCAS(info, 0, nd.off + j + 1);
   orig_all[nd.off + j] = orig;
@@ -790,7 +814,7 @@ struct Probe {
 };
 
 void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cnt, int maxsz, double* d, double* z,
-               int ldz, const DcWork& W, int* info, const Probe* probe) {
+               int ldz, const DcWork& W, int* info, const Probe* probe, int maxit) {
   const size_t lds = (size_t)maxsz * (6 * sizeof(double) + 7 * sizeof(int)) + 64;
   if (lds > 160 * 1024) throw std::runtime_error("stedc: n too large for the LDS-staged deflation");
   allow_lds(deflate_kernel, lds);
@@ -814,7 +838,7 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
   allow_lds(secular_kernel, 2 * sizeof(double) * maxsz);
   hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64 / SEC_G), cnt), dim3(64), 2 * sizeof(double) * maxsz, s, lvl,
                      W.outs,
-                     W.rho_lvl, W.dlamda, W.w, W.indxc, d, W.orig, W.tau, W.sbuf, info);
+                     W.rho_lvl, W.dlamda, W.w, W.indxc, d, W.orig, W.tau, W.sbuf, info, maxit);
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz * GU_G, 64), cnt), dim3(64), 0, s, lvl, W.outs, W.dlamda, W.w, W.orig,
                      W.tau, W.wnew);
@@ -838,7 +862,8 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
 }  // namespace
 
 void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, int ldz, double* work, int* iwork,
-           int* info) {
+           int* info, int secular_maxit) {
+  const int maxit = secular_maxit >= 0 ? std::min(secular_maxit, secular::MAXIT) : secular::MAXIT;
   const int n = plan.n;
   if (n <= 0) return;
   if (n == 1) {
@@ -853,7 +878,7 @@ void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, i
   for (int L = (int)plan.lvls.size() - 1; L >= 0; --L) {
     const DcPlan::Level& lv = plan.lvls[L];
     run_level(s, n, nodes + lv.start, plan.pre_index_dev + lv.pre_start, lv.count, lv.maxsz, d, z, ldz, W, info,
-              nullptr);
+              nullptr, maxit);
   }
   hipLaunchKernelGGL(unscale_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, W.scale);
   XTB_LAUNCH_CHECK();
@@ -875,7 +900,7 @@ void dc_merge_probe(hipStream_t s, int n, int n1, double rho, double* d, double*
   XTB_HIP(hipMemcpy(pre_index, &zero, sizeof(int), hipMemcpyHostToDevice));
   XTB_HIP(hipMemcpy(W.rho_pre, &rho, sizeof(double), hipMemcpyHostToDevice));
   const Probe pr{d_defl_out, q2_out, dlamda_out, w_out, rho_out, kctot_out, indxc_out};
-  run_level(s, n, node, pre_index, 1, n, d, q, ldq, W, info, &pr);
+  run_level(s, n, node, pre_index, 1, n, d, q, ldq, W, info, &pr, secular::MAXIT);
 }
 
 }  // namespace xtbeig

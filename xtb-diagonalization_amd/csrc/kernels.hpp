@@ -90,9 +90,10 @@ struct DcPlan {
 size_t dc_workspace_doubles(int n);
 size_t dc_workspace_ints(int n);
 // Eigenvalues (ascending, into d) and eigenvectors (z, n x n, ldz) of the tridiagonal (d, e), FP64; e is destroyed.
-// info: device int, set to a positive value when a secular equation fails to converge.
+// info: device int, set to a positive value when a secular equation fails to converge. secular_maxit >= 0 caps the
+// secular solver's iterations below its own limit (a test setting that makes it give up; -1: the solver's limit).
 void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, int ldz, double* work, int* iwork,
-           int* info);
+           int* info, int secular_maxit = -1);
 void dc_merge_probe(hipStream_t s, int n, int n1, double rho, double* d, double* q, int ldq, double* work,
                     int* iwork, int* info, double* d_defl_out, double* q2_out, int* kctot_out, double* dlamda_out,
                     double* w_out, int* indxc_out, double* rho_out);
@@ -156,15 +157,17 @@ void ormtr_upper(hipStream_t s, int n, int m, const T* a, int lda, const T* tau,
 // density.hip ------------------------------------------------------------------------------------------------------
 // Batches of equal-n problems, problem p at base + p * stride. occ: 3 n per problem (f | f' | log cosh(x / 2)).
 size_t density_workspace(int batch, int n);  // doubles
-// mu (bisect: sum f = nel; else the given mu), occ, P = C diag(f) C^T.
-void density_forward(hipStream_t s, int batch, int n, const double* w, int64_t sw, const double* c, int ldc,
-                     int64_t sc, double kT, double nel, int bisect, double* mu, double* p, int ldp, int64_t sp,
-                     double* occ, double* work);
+// nspin spin channels (1, or 2: alpha and beta smeared separately, f = f_alpha + f_beta); mu and occ hold nspin
+// entries / 3 n blocks per problem. mu (bisect: sum f_channel = nel_a, nel_b; else the given mu), occ,
+// P = C diag(f) C^T.
+void density_forward(hipStream_t s, int batch, int n, int nspin, const double* w, int64_t sw, const double* c,
+                     int ldc, int64_t sc, double kT, double nel_a, double nel_b, int bisect, double* mu, double* p,
+                     int ldp, int64_t sp, double* occ, double* work);
 // Hbar, Sbar from Gbar = dL/dP (mu_fixed: no re-bisection term).
-void density_backward(hipStream_t s, int batch, int n, const double* w, int64_t sw, const double* c, int ldc,
-                      int64_t sc, double kT, const double* mu, int mu_fixed, const double* occ, const double* gbar,
-                      int ldg, int64_t sg, double* hbar, int ldhb, int64_t shb, double* sbar, int ldsb, int64_t ssb,
-                      double* work);
+void density_backward(hipStream_t s, int batch, int n, int nspin, const double* w, int64_t sw, const double* c,
+                      int ldc, int64_t sc, double kT, const double* mu, int mu_fixed, const double* occ,
+                      const double* gbar, int ldg, int64_t sg, double* hbar, int ldhb, int64_t shb, double* sbar,
+                      int ldsb, int64_t ssb, double* work);
 
 // refine.hip -------------------------------------------------------------------------------------------------------
 size_t refine_workspace(int n);  // doubles

@@ -9,6 +9,30 @@
 // Differences from the Fortran routine: the differences delta_j = d_j - lambda are recomputed from the origin pole and
 // the current shift each iteration, (d_j - d_origin) - tau, instead of being updated by repeated subtraction, and the
 // caller receives (origin index, tau) so that every later kernel can form delta_j the same way.
+//
+// The secular solver and the deflation follow LAPACK's dlaed2 / dlaed4 / dlaed5 / dlaed6 closely (their variable
+// names and branch structure), so LAPACK's license applies to those parts:
+//   Copyright (c) 1992-2017 The University of Tennessee and The University of Tennessee Research Foundation.
+//   Copyright (c) 2000-2017 The University of California Berkeley. Copyright (c) 2006-2017 The University of
+//   Colorado Denver. All rights reserved.
+//   Redistribution and use in source and binary forms, with or without modification, are permitted provided that the
+//   following conditions are met: (1) Redistributions of source code must retain the above copyright notice, this
+//   list of conditions and the following disclaimer. (2) Redistributions in binary form must reproduce the above
+//   copyright notice, this list of conditions and the following disclaimer listed in this license in the
+//   documentation and/or other materials provided with the distribution. (3) Neither the name of the copyright
+//   holders nor the names of its contributors may be used to endorse or promote products derived from this software
+//   without specific prior written permission.
+//   The copyright holders provide no reassurances that the source code provided does not infringe any patent,
+//   copyright, or any other intellectual property rights of third parties. The copyright holders disclaim any
+//   liability to any recipient for claims brought against recipient by any third party for infringement of that
+//   party's intellectual property rights.
+//   THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS IS" AND ANY EXPRESS OR IMPLIED
+//   WARRANTIES, INCLUDING, BUT NOT LIMITED TO, THE IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A PARTICULAR
+//   PURPOSE ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT OWNER OR CONTRIBUTORS BE LIABLE FOR ANY DIRECT, INDIRECT,
+//   INCIDENTAL, SPECIAL, EXEMPLARY, OR CONSEQUENTIAL DAMAGES (INCLUDING, BUT NOT LIMITED TO, PROCUREMENT OF SUBSTITUTE
+//   GOODS OR SERVICES; LOSS OF USE, DATA, OR PROFITS; OR BUSINESS INTERRUPTION) HOWEVER CAUSED AND ON ANY THEORY OF
+//   LIABILITY, WHETHER IN CONTRACT, STRICT LIABILITY, OR TORT (INCLUDING NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY
+//   OUT OF THE USE OF THIS SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF SUCH DAMAGE.
 #pragma once
 #include "common.hpp"
 
@@ -202,7 +226,7 @@ __device__ inline Sums pole_sums(int k, int ilo, int ihi, const double* d, const
 // (0: converged).
 template <int G>
 __device__ inline int laed4(int k, int i, const double* d, const double* z, double rho, int& orig, double& tau,
-                            int gl) {
+                            int gl, int maxit = MAXIT) {
   const double rhoinv = 1 / rho;
   if (i == k - 1) {
     // the largest root, origin d[k - 1]
@@ -238,7 +262,7 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
       dltlb = 0;
       dltub = midpt;
     }
-    for (int niter = 1; niter <= MAXIT + 1; ++niter) {
+    for (int niter = 1; niter <= maxit + 1; ++niter) {
       // psi over poles 0 .. n - 2, phi from pole n - 1 alone
       Sums sm = pole_sums<G>(n, ii + 1, n, d, z, dorg, tau, gl);
       const double dn = (d[n - 1] - dorg) - tau, dnm1 = (d[n - 2] - dorg) - tau;
@@ -247,7 +271,7 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
       const double erretm = 8 * (-phi - sm.psi) + sm.erretm - phi + rhoinv + fabs(tau) * (sm.dpsi + dphi);
       w = rhoinv + phi + sm.psi;
       if (fabs(w) <= EPS * erretm) return 0;
-      if (niter > MAXIT) break;
+      if (niter > maxit) break;
       if (w <= 0) dltlb = fmax(dltlb, tau); else dltub = fmin(dltub, tau);
       double cc = w - dnm1 * sm.dpsi - dn * dphi;
       const double a = (dnm1 + dn) * w - dnm1 * dn * (sm.dpsi + dphi);
@@ -310,7 +334,7 @@ __device__ inline int laed4(int k, int i, const double* d, const double* z, doub
   if (fabs(w) <= EPS * erretm) return 0;
   if (w <= 0) dltlb = fmax(dltlb, tau); else dltub = fmin(dltub, tau);
   bool swtch = false;
-  for (int niter = 2; niter <= MAXIT; ++niter) {
+  for (int niter = 2; niter <= maxit; ++niter) {
     double dpsi = sm.dpsi, dphi = sm.dphi;
     const double di = delta(i, tau), dip1 = delta(ip1, tau);
     double eta;

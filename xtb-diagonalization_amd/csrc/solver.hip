@@ -224,7 +224,7 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   XTB_HIP(hipEventRecord(h.ev_join, st2));
   copy_matrix<T, double>(st, n, 1, s.dT, n, s.d64, n);
   copy_matrix<T, double>(st, n - 1, 1, s.eT, n, s.e64, n);
-  stedc(st, h.plan(n), s.d64, s.e64, s.z64, n, s.dcwork, s.dciwork, h.dinfo + 2);
+  stedc(st, h.plan(n), s.d64, s.e64, s.z64, n, s.dcwork, s.dciwork, h.dinfo + 2, h.test_secular_maxit);
   copy_matrix<double, double>(st, n, 1, s.d64, n, w, n);
   XTB_HIP(hipStreamWaitEvent(st, h.ev_join, 0));
   // z = M Zdc (Zdc rounded to T in zt, which the second stream no longer needs)
@@ -356,7 +356,7 @@ void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz) {
   Carver c(h.arena);
   double* work = c.take<double>(dc_workspace_doubles(n));
   int* iwork = c.take<int>(dc_workspace_ints(n));
-  stedc(h.stream, h.plan(n), d, e, z, ldz, work, iwork, h.dinfo + 2);
+  stedc(h.stream, h.plan(n), d, e, z, ldz, work, iwork, h.dinfo + 2, h.test_secular_maxit);
 }
 
 template void syevd_device<double>(Handle&, int, const double*, int, double*, double*, int, int);

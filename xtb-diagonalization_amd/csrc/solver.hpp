@@ -46,6 +46,8 @@ struct Handle {
   std::vector<std::pair<void*, size_t>> io_slots;
   int io_top = 0;
   int refine_iters = 2;
+  // test setting: a cap on the divide and conquer's secular-equation iterations (-1: the solver's own limit)
+  int test_secular_maxit = -1;
   int band_override = 0;
   // SCC precision schedule: settings (fp32_rmsq 0 = FP64 only) and the counters of the current sequence
   double scc_fp32_rmsq = 0.0;

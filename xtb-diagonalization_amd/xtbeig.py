@@ -57,6 +57,10 @@ SIGNATURES = {
                                 vp]),
     "xtbeig_dev_density_backward": (ci, [vp, ci, ci, vp, ll, vp, ci, ll, ctypes.c_double, vp, ci, vp, vp, ci, ll, vp,
                                          ci, ll, vp, ci, ll]),
+    "xtbeig_dev_density_spin": (ci, [vp, ci, ci, vp, ll, vp, ci, ll, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, vp, ci, vp, ci, ll, vp]),
+    "xtbeig_dev_density_backward_spin": (ci, [vp, ci, ci, vp, ll, vp, ci, ll, ctypes.c_double, vp, ci, vp, vp, ci, ll,
+                                              vp, ci, ll, vp, ci, ll]),
     "xtbeig_dev_density_backward_recompute": (ci, [vp, ci, ci, vp, ci, ll, vp, ci, ll, ci, ctypes.c_double,
                                                    ctypes.c_double, ci, vp, ci, ll, vp, ci, ll, vp, ci, ll]),
     "xtbeig_set_scc_precision": (ci, [vp, ctypes.c_double, ci, ci]),
@@ -78,6 +82,7 @@ SIGNATURES = {
     "xtbeig_get_stream": (vp, [vp]),
     "xtbeig_set_tuning": (ci, [vp, ci, ci]),
     "xtbeig_default_band": (ci, [ci, ci]),
+    "xtbeig_set_test_limits": (ci, [vp, ci]),
     "xtbeig_synchronize": (ci, [vp, c_int_p]),
     "xtbeig_refine_check": (ci, [vp, c_dbl_p]),
     "xtbeig_last_error": (ctypes.c_char_p, []),
@@ -251,9 +256,10 @@ def stedc(d_in, e_in, dtype=np.float64):
     return d, z, info.value
 
 
-def dsymmetrize_pad(src, ldd=None):
+def dsymmetrize_pad(src, ldd=None, n=None):
+    """xtb_dsymmetrize_pad on the leading n x n block of src (lds = src.shape[0]); dst has ldd rows."""
     src = np.asfortranarray(src, dtype=np.float64)
-    n = src.shape[0]
+    n = src.shape[1] if n is None else n
     ldd = ldd or n
     dst = np.zeros((ldd, n), order="F")
     lib().xtb_dsymmetrize_pad(_i(n), _ptr(src), _i(src.shape[0]), _ptr(dst), _i(ldd))
@@ -296,6 +302,9 @@ class Handle:
 
     def set_tuning(self, band=0, refine_iterations=2):
         self._check(lib().xtbeig_set_tuning(self._h, band, refine_iterations))
+
+    def set_test_limits(self, secular_maxit=-1):
+        self._check(lib().xtbeig_set_test_limits(self._h, secular_maxit))
 
     def synchronize(self):
         """Wait; the four status words [Cholesky info, chase stall, D&C info, refinement check], cleared."""
@@ -461,6 +470,28 @@ class Handle:
                                                       n * n, float(kT), vp(mu.data_ptr()), 1 if mu_fixed else 0,
                                                       vp(occ.data_ptr()), vp(gbar.data_ptr()), n, n * n,
                                                       vp(hbar.data_ptr()), n, n * n, vp(sbar.data_ptr()), n, n * n))
+        return hbar, sbar
+
+    def density_spin(self, w, c, kT, nel_alpha, nel_beta, mu, bisect=True):
+        """Open-shell density: mu (batch, 2) alpha / beta chemical potentials; returns (P, occ (batch, 6 n))."""
+        import torch
+        batch, n = w.shape
+        p = torch.empty((batch, n, n), dtype=torch.float64, device=w.device)
+        occ = torch.empty((batch, 6 * n), dtype=torch.float64, device=w.device)
+        self._check(lib().xtbeig_dev_density_spin(self._h, batch, n, vp(w.data_ptr()), n, vp(c.data_ptr()), n, n * n,
+                                                  float(kT), float(nel_alpha), float(nel_beta), vp(mu.data_ptr()),
+                                                  1 if bisect else 0, vp(p.data_ptr()), n, n * n, vp(occ.data_ptr())))
+        return p, occ
+
+    def density_backward_spin(self, w, c, kT, mu, occ, gbar, mu_fixed=False):
+        import torch
+        batch, n = w.shape
+        hbar = torch.empty((batch, n, n), dtype=torch.float64, device=w.device)
+        sbar = torch.empty_like(hbar)
+        self._check(lib().xtbeig_dev_density_backward_spin(
+            self._h, batch, n, vp(w.data_ptr()), n, vp(c.data_ptr()), n, n * n, float(kT), vp(mu.data_ptr()),
+            1 if mu_fixed else 0, vp(occ.data_ptr()), vp(gbar.data_ptr()), n, n * n, vp(hbar.data_ptr()), n, n * n,
+            vp(sbar.data_ptr()), n, n * n))
         return hbar, sbar
 
     def density_backward_recompute(self, h, s, kT, nel, gbar, precision=0, mu_fixed=False):
