@@ -56,6 +56,16 @@ void xtb_dormtr_upper(const int *n, const int *m, const double *a, const int *ld
 void xtb_sormtr_upper(const int *n, const int *m, const float *a, const int *lda, const float *tau, float *z,
                       const int *ldz, float *work);
 
+/* The two steps either side of the reduction in xTB's per-cycle eigensolve (eigensolve.F90's congruence and
+ * back-transformation, restated in the reference's scripts/kernels/congruence.c:1-8 and eigbench.f90:101-168): the
+ * upper triangle of Z = X^T H X with X = U^-1 upper triangular (S = U^T U), Z's strictly lower triangle untouched
+ * (trmm 'R','U','N','N' + gemm + trmm 'L','U','T','N' in blocks of 96 in the reference; here two GEMMs on the GPU). */
+void xtb_dcongruence_upper(const int *n, const double *h, const int *ldh, const double *x, const int *ldx, double *z,
+                           const int *ldz);
+/* C(:, 1:m) = X C(:, 1:m), X upper triangular (n x n): the back-solve trmm('L','U','N','N', n, m) of the occupied
+ * columns (congruence.c:6, eigbench.f90:166; xTB passes m = occupied orbitals, the reference's LOG.md:310-312). */
+void xtb_dtrmm_back(const int *n, const int *m, const double *x, const int *ldx, double *c, const int *ldc);
+
 /* ---- 1b. LAPACK-convention entry points ------------------------------------------------------------------------ */
 
 /* dsytrd(uplo, ...): uplo = 'U' only (info = -1 otherwise), as xtb_dsytrd_upper; lwork >= 1, lwork = -1 queries
@@ -143,6 +153,24 @@ int xtbeig_dev_sygvd(xtbeig_handle_t handle, int precision, int n, const double 
 /* Standard solve of a device-resident symmetric A (full storage, not modified). */
 int xtbeig_dev_dsyevd(xtbeig_handle_t handle, int n, const double *a, int lda, double *w, double *z, int ldz);
 int xtbeig_dev_ssyevd(xtbeig_handle_t handle, int n, const float *a, int lda, double *w, float *z, int ldz);
+/* All n eigenvalues and the first m eigenvectors (1 <= m <= n) of the symmetric a: the last product of the
+ * back-transformation costs 2 n^2 m instead of 2 n^3 (xTB needs the occupied columns only, LOG.md:310-312). */
+int xtbeig_dev_dsyevd_partial(xtbeig_handle_t handle, int n, int m, const double *a, int lda, double *w, double *z,
+                              int ldz);
+
+/* The geometry of an SCC sequence, factored once and kept on the handle (xTB's X = U^-1 is fixed across the cycles of
+ * one geometry: congruence.c:7): S (n x n, full storage) is copied, S = U^T U and X = U^-1 are formed on the GPU and
+ * kept until the next call (Cholesky info in status word 0). */
+int xtbeig_dev_set_geometry(xtbeig_handle_t handle, int n, const double *s, int lds);
+/* Z's upper triangle = X^T H X with the handle's X (Z's strictly lower triangle untouched); congruence.c:1-5. */
+int xtbeig_dev_congruence(xtbeig_handle_t handle, int n, const double *h, int ldh, double *z, int ldz);
+/* C(:, 0:m) = X C(:, 0:m) with the handle's X (0 <= m <= n); congruence.c:6. */
+int xtbeig_dev_back_solve(xtbeig_handle_t handle, int n, int m, double *c, int ldc);
+/* One cycle's generalized solve on the handle's geometry: as xtbeig_dev_sygvd with the cached S, U, X (no
+ * factorization per cycle), all n eigenvalues and the first m eigenvectors C (n x m, ldc >= n), 1 <= m <= n. The FP32
+ * + refinement path refines all n pairs (the refinement needs the whole basis) and returns m. */
+int xtbeig_dev_sygvd_geometry(xtbeig_handle_t handle, int precision, int n, int m, const double *h, int ldh,
+                              double *w, double *c, int ldc);
 /* Tridiagonal (d, e) -> eigenvalues ascending in d, eigenvectors in z; e destroyed (FP64). */
 int xtbeig_dev_dstedc(xtbeig_handle_t handle, int n, double *d, double *e, double *z, int ldz);
 

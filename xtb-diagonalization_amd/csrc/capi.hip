@@ -473,6 +473,55 @@ int xtbeig_dev_ssyevd(xtbeig_handle_t handle, int n, const float* a, int lda, do
   });
 }
 
+int xtbeig_dev_dsyevd_partial(xtbeig_handle_t handle, int n, int m, const double* a, int lda, double* w, double* z,
+                              int ldz) {
+  if (!handle) return set_error("xtbeig_dev_dsyevd_partial: null handle");
+  if (n < 0 || m < (n > 0 ? 1 : 0) || m > n || lda < (n > 1 ? n : 1) || ldz < (n > 1 ? n : 1))
+    return set_error("xtbeig_dev_dsyevd_partial: bad dimensions (need 1 <= m <= n)");
+  return guarded([&] {
+    Handle& h = handle->h;
+    xtbeig::syevd_device<double>(h, n, a, lda, w, z, ldz, h.band_override, m);
+  });
+}
+
+int xtbeig_dev_set_geometry(xtbeig_handle_t handle, int n, const double* s, int lds) {
+  if (!handle) return set_error("xtbeig_dev_set_geometry: null handle");
+  if (n < 1 || lds < n) return set_error("xtbeig_dev_set_geometry: bad dimensions");
+  return guarded([&] { handle->h.set_geometry(n, s, lds); });
+}
+
+int xtbeig_dev_congruence(xtbeig_handle_t handle, int n, const double* hm, int ldh, double* z, int ldz) {
+  if (!handle) return set_error("xtbeig_dev_congruence: null handle");
+  if (n < 1 || ldh < n || ldz < n) return set_error("xtbeig_dev_congruence: bad dimensions");
+  return guarded([&] {
+    Handle& h = handle->h;
+    if (h.geo_n != n) throw std::runtime_error("xtbeig_dev_congruence: no geometry of this n on the handle");
+    xtbeig::congruence_device(h, n, hm, ldh, h.geo_x, n, z, ldz);
+  });
+}
+
+int xtbeig_dev_back_solve(xtbeig_handle_t handle, int n, int m, double* c, int ldc) {
+  if (!handle) return set_error("xtbeig_dev_back_solve: null handle");
+  if (n < 1 || m < 0 || m > n || ldc < n) return set_error("xtbeig_dev_back_solve: bad dimensions");
+  return guarded([&] {
+    Handle& h = handle->h;
+    if (h.geo_n != n) throw std::runtime_error("xtbeig_dev_back_solve: no geometry of this n on the handle");
+    xtbeig::back_solve_device(h, n, m, h.geo_x, n, c, ldc);
+  });
+}
+
+int xtbeig_dev_sygvd_geometry(xtbeig_handle_t handle, int precision, int n, int m, const double* hm, int ldh,
+                              double* w, double* c, int ldc) {
+  if (!handle) return set_error("xtbeig_dev_sygvd_geometry: null handle");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_dev_sygvd_geometry: unknown precision");
+  if (n < 1 || m < 1 || m > n || ldh < n || ldc < n) return set_error("xtbeig_dev_sygvd_geometry: bad dimensions");
+  return guarded([&] {
+    Handle& h = handle->h;
+    xtbeig::sygvd_geometry(h, n, m, hm, ldh, w, c, ldc, precision == XTBEIG_FP32_REFINE, h.band_override);
+  });
+}
+
 int xtbeig_dev_dstedc(xtbeig_handle_t handle, int n, double* d, double* e, double* z, int ldz) {
   return guarded([&] { xtbeig::stedc_device(handle->h, n, d, e, z, ldz); });
 }
@@ -551,6 +600,16 @@ int xtbeig_dev_dc_merge(xtbeig_handle_t handle, int n, int n1, double rho, doubl
 }
 
 // ---- host-memory entry points -----------------------------------------------------------------------------------
+
+// The void entry points of the sytrd.c symbol set refuse leading dimensions below n before touching the caller's
+// arrays (the message in xtbeig_last_error; the reference's calls have no error path).
+static bool lds_ok(const char* who, int n, int lda) {
+  if (n > 0 && lda < n) {
+    set_error((std::string(who) + ": leading dimension must be >= n").c_str());
+    return false;
+  }
+  return true;
+}
 
 void xtb_dstedc(const int* n_, double* d, double* e, double* z, const int* ldz_, double* work, const int* lwork,
                 int* iwork, const int* liwork, int* info) {
@@ -639,6 +698,43 @@ void xtb_dsymmetrize_pad(const int* n_, const double* src, const int* lds_, doub
 }
 
 
+void xtb_dcongruence_upper(const int* n_, const double* hm, const int* ldh_, const double* x, const int* ldx_,
+                           double* z, const int* ldz_) {
+  const int n = *n_, ldh = *ldh_, ldx = *ldx_, ldz = *ldz_;
+  if (n <= 0) return;
+  if (!lds_ok("xtb_dcongruence_upper", n, ldh) || !lds_ok("xtb_dcongruence_upper", n, ldx) ||
+      !lds_ok("xtb_dcongruence_upper", n, ldz))
+    return;
+  guarded([&] {
+    Handle& h = thread_handle();
+    const size_t nn = (size_t)n * n;
+    DevBuf dh(sizeof(double) * nn), dx(sizeof(double) * nn), dz(sizeof(double) * nn);
+    h2d_mat<double>(h, dh.as<double>(), n, hm, ldh, n, n);
+    h2d_mat<double>(h, dx.as<double>(), n, x, ldx, n, n);
+    xtbeig::congruence_device(h, n, dh.as<double>(), n, dx.as<double>(), n, dz.as<double>(), n);
+    d2h_triangle<double>(h, true, z, ldz, dz.as<double>(), n);
+  });
+}
+
+void xtb_dtrmm_back(const int* n_, const int* m_, const double* x, const int* ldx_, double* c, const int* ldc_) {
+  const int n = *n_, m = *m_, ldx = *ldx_, ldc = *ldc_;
+  if (n <= 0 || m <= 0) return;
+  if (m > n) {
+    set_error("xtb_dtrmm_back: need m <= n");
+    return;
+  }
+  if (!lds_ok("xtb_dtrmm_back", n, ldx) || !lds_ok("xtb_dtrmm_back", n, ldc)) return;
+  guarded([&] {
+    Handle& h = thread_handle();
+    DevBuf dx(sizeof(double) * (size_t)n * n), dc(sizeof(double) * (size_t)n * m);
+    h2d_mat<double>(h, dx.as<double>(), n, x, ldx, n, n);
+    h2d_mat<double>(h, dc.as<double>(), n, c, ldc, n, m);
+    xtbeig::back_solve_device(h, n, m, dx.as<double>(), n, dc.as<double>(), n);
+    d2h_mat<double>(h, c, ldc, dc.as<double>(), n, n, m);
+    XTB_HIP(hipStreamSynchronize(h.stream));
+  });
+}
+
 void xtbeig_dsyevd(const char* jobz, const char* uplo, const int* n, double* a, const int* lda, double* w,
                    double* work, const int* lwork, int* iwork, const int* liwork, int* info) {
   syevd_host<double>(jobz, uplo, n, a, lda, w, work, lwork, iwork, liwork, info);
@@ -666,15 +762,6 @@ void xtbeig_ssygvd(const int* itype, const char* jobz, const char* uplo, const i
   sygvd_host<float>(true, itype, jobz, uplo, n, a, lda, b, ldb, w, work, lwork, iwork, liwork, info);
 }
 
-// The void entry points of the sytrd.c symbol set refuse leading dimensions below n before touching the caller's
-// arrays (the message in xtbeig_last_error; the reference's calls have no error path).
-static bool lds_ok(const char* who, int n, int lda) {
-  if (n > 0 && lda < n) {
-    set_error((std::string(who) + ": leading dimension must be >= n").c_str());
-    return false;
-  }
-  return true;
-}
 void xtb_dsytrd_upper(const int* n, double* a, const int* lda, double* d, double* e, double* tau, double* work) {
   (void)work;
   if (lds_ok("xtb_dsytrd_upper", *n, *lda)) sytrd_host<double>(*n, a, *lda, d, e, tau, false);

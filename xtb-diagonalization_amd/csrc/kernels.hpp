@@ -27,6 +27,9 @@ void copy_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int
 // dst (n x m) = src (m x n)^T (with conversion).
 template <typename S, typename D>
 void transpose_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd);
+// dst's upper triangle (i <= j) = src's; dst's strictly lower triangle untouched.
+template <typename T>
+void copy_upper(hipStream_t s, int n, const T* src, int lds, T* dst, int ldd);
 // Mirror the referenced triangle of a (uplo 'U': upper is read) into the other one, in place.
 template <typename T>
 void symmetrize(hipStream_t s, char uplo, int n, T* a, int lda);

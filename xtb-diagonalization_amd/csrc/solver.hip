@@ -48,6 +48,7 @@ Handle::~Handle() {
   if (stream2) (void)hipStreamDestroy(stream2);
   if (arena) (void)hipFree(arena);
   if (aux) (void)hipFree(aux);
+  if (geo) (void)hipFree(geo);
   if (dinfo) (void)hipFree(dinfo);
   if (pinned) (void)hipHostFree(pinned);
   for (auto& sl : io_slots)
@@ -193,10 +194,13 @@ StdBuffers<T> carve_std(Carver& c, int n, int b) {
 // from the right (once the chase is done); the first stream then needs one product, z = M Zdc.
 // pre (optional, n x n, ld ldpre): z = pre Q1 Q2 Zdc instead, with P = pre M formed on the second stream into pm
 // (n x n, ld n) right after M, so that the first stream still needs one product after the divide and conquer.
+// m (1 <= m <= n, default n): only the first m eigenvectors are formed (all n eigenvalues are): the last product
+// costs 2 n^2 m instead of 2 n^3, as xTB's occupied-only back-transformation.
 template <typename T>
 void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, double* w, T* z, int ldz,
-                const T* pre = nullptr, int ldpre = 0, T* pm = nullptr) {
+                const T* pre = nullptr, int ldpre = 0, T* pm = nullptr, int m = -1) {
   hipStream_t st = h.stream, st2 = h.stream2;
+  if (m < 0 || m > n) m = n;
   if (n == 1) {
     copy_matrix<T, double>(st, 1, 1, a, lda, w, 1);
     set_identity(st, 1, z, ldz);
@@ -232,11 +236,11 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   if (std::is_same<T, double>::value) {
     zdc = reinterpret_cast<const T*>(s.z64);
   } else {
-    copy_matrix<double, T>(st, n, n, s.z64, n, s.zt, n);
+    copy_matrix<double, T>(st, n, m, s.z64, n, s.zt, n);
     zdc = s.zt;
   }
-  if (pre) gemm<T>(st, 'N', 'N', n, n, n, T(1), pm, n, zdc, n, T(0), z, ldz);
-  else gemm<T>(st, 'N', 'N', n, n, n, T(1), s.qt, ldzt, zdc, n, T(0), z, ldz);
+  if (pre) gemm<T>(st, 'N', 'N', n, m, n, T(1), pm, n, zdc, n, T(0), z, ldz);
+  else gemm<T>(st, 'N', 'N', n, m, n, T(1), s.qt, ldzt, zdc, n, T(0), z, ldz);
 }
 
 template <typename T>
@@ -260,7 +264,7 @@ static void check_size(int n, int b) {
 }
 
 template <typename T>
-void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b) {
+void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b, int m) {
   if (n <= 0) return;
   if (b <= 0) b = h.band_for(n, sizeof(T) == 4);
   b = std::max(1, std::min(b, std::max(1, n - 1)));
@@ -270,16 +274,13 @@ void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ld
   T* work = c.take<T>((size_t)n * n);
   StdBuffers<T> s = carve_std<T>(c, n, b);
   copy_matrix<T, T>(h.stream, n, n, a, lda, work, n);
-  syevd_core<T>(h, s, n, b, work, n, w, z, ldz);
+  syevd_core<T>(h, s, n, b, work, n, w, z, ldz, nullptr, 0, nullptr, m);
 }
 
-size_t sygvd_bytes(int n, int b, bool mixed) {
-  Carver c(nullptr);
+// The arena of the solve's steps after the Cholesky factorization (the congruence, the standard solve, the
+// back-solve and, FP32 + refinement, the refinement), for m eigenvectors.
+static void carve_core_bytes(Carver& c, int n, int b, bool mixed, int m) {
   const size_t nn = (size_t)n * n;
-  c.take<double>(nn);  // S copy
-  c.take<double>(nn);  // U
-  c.take<double>(nn);  // X
-  c.take<double>(potrf_workspace(n));
   if (!mixed) {
     c.take<double>(nn);  // H X, then X Q1 Q2
     c.take<double>(nn);  // H'
@@ -292,8 +293,65 @@ size_t sygvd_bytes(int n, int b, bool mixed) {
     carve_std<float>(c, n, b);
     c.take<double>(refine_workspace(n));  // also H X and H' in FP64 before the refinement
     c.take<int>(refine_iwork(n));
+    if (m < n) c.take<double>(nn);  // all n refined eigenvectors, of which m are returned
   }
+}
+
+size_t sygvd_bytes(int n, int b, bool mixed, int m) {
+  Carver c(nullptr);
+  const size_t nn = (size_t)n * n;
+  c.take<double>(nn);  // S copy
+  c.take<double>(nn);  // U
+  c.take<double>(nn);  // X
+  c.take<double>(potrf_workspace(n));
+  carve_core_bytes(c, n, b, mixed, m);
   return c.off + 4096;
+}
+
+// Everything after S = U^T U: H' = X^T H X (FP64), the standard solve of H' (FP64, or FP32 and rounded), C = X Z for
+// the first m columns, and for the mixed path the FP64 refinement against (H, S) and the check. x: X = U^-1 (n x n,
+// ld ldx, zero below the diagonal); sm is read by the refinement only.
+static void sygvd_core(Handle& h, Carver& c, int n, const double* hm, int ldh, const double* sm, int lds,
+                       const double* x, int ldx, double* w, double* cm, int ldc, bool mixed, int b, int m) {
+  hipStream_t st = h.stream;
+  const size_t nn = (size_t)n * n;
+  if (!mixed) {
+    double* hx = c.take<double>(nn);
+    double* hp = c.take<double>(nn);
+    StdBuffers<double> s = carve_std<double>(c, n, b);
+    gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, ldx, 0.0, hx, n);
+    gemm_upper_tn<double>(st, n, n, 1.0, x, ldx, hx, n, 0.0, hp, n);  // X^T H X is symmetric: upper tiles, mirrored
+    symmetrize<double>(st, 'U', n, hp, n);
+    // C = X Z = (X Q1 Q2) Zdc, X Q1 Q2 formed beside the divide and conquer (into hx, free by then)
+    syevd_core<double>(h, s, n, b, hp, n, w, cm, ldc, x, ldx, hx, m);
+  } else {
+    float* x32 = c.take<float>(nn);
+    float* pm = c.take<float>(nn);
+    float* hp = c.take<float>(nn);
+    float* c32 = c.take<float>(nn);
+    StdBuffers<float> s = carve_std<float>(c, n, b);
+    double* rwork = c.take<double>(refine_workspace(n));
+    int* riwork = c.take<int>(refine_iwork(n));
+    double* cfull = m < n ? c.take<double>(nn) : cm;
+    const int ldfull = m < n ? n : ldc;
+    // H' = X^T H X in FP64, then rounded: formed in FP32, its rounding error grows with cond(S) (about
+    // eps32 cond(S) ||H||), which at cond(S) = 1e4 leaves the FP32 eigenvectors too far off for the refinement to
+    // converge; in FP64 the FP32 solve starts from eps32 ||H'|| whatever S. The two products are a small part of
+    // the solve. H X and H' sit in the refinement's workspace, which is free until the refinement starts.
+    double* hx64 = rwork;
+    double* hp64 = rwork + nn;
+    copy_matrix<double, float>(st, n, n, x, ldx, x32, n);
+    gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, ldx, 0.0, hx64, n);
+    gemm_upper_tn<double>(st, n, n, 1.0, x, ldx, hx64, n, 0.0, hp64, n);
+    symmetrize<double>(st, 'U', n, hp64, n);
+    copy_matrix<double, float>(st, n, n, hp64, n, hp, n);
+    // the refinement needs the whole basis: all n vectors are formed and refined, the first m returned
+    syevd_core<float>(h, s, n, b, hp, n, w, c32, n, x32, n, pm);
+    copy_matrix<float, double>(st, n, n, c32, n, cfull, ldfull);
+    refine_oa(st, n, hm, ldh, sm, lds, cfull, ldfull, w, h.refine_iters, rwork, riwork, h.dinfo + 3,
+              h.check_ratios());
+    if (m < n) copy_matrix<double, double>(st, n, m, cfull, ldfull, cm, ldc);
+  }
 }
 
 void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
@@ -303,7 +361,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   if (b <= 0) b = h.band_for(n, mixed);
   b = std::max(1, std::min(b, std::max(1, n - 1)));
   check_size(n, b);
-  h.reserve(sygvd_bytes(n, b, mixed));
+  h.reserve(sygvd_bytes(n, b, mixed, n));
   Carver c(h.arena);
   const size_t nn = (size_t)n * n;
   double* scopy = c.take<double>(nn);
@@ -313,38 +371,74 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   copy_matrix<double, double>(st, n, n, sm, lds, scopy, n);
   potrf_inverse(st, n, scopy, n, u, n, x, n, tmp, h.dinfo + 0, h.stream2, h.ev_band, h.ev_join);
   if (u_out) *u_out = u;  // no later stage carves over it
-  if (!mixed) {
-    double* hx = c.take<double>(nn);
-    double* hp = c.take<double>(nn);
-    StdBuffers<double> s = carve_std<double>(c, n, b);
-    gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, n, 0.0, hx, n);
-    gemm_upper_tn<double>(st, n, n, 1.0, x, n, hx, n, 0.0, hp, n);  // X^T H X is symmetric: upper tiles, mirrored
-    symmetrize<double>(st, 'U', n, hp, n);
-    // C = X Z = (X Q1 Q2) Zdc, X Q1 Q2 formed beside the divide and conquer (into hx, free by then)
-    syevd_core<double>(h, s, n, b, hp, n, w, cm, ldc, x, n, hx);
-  } else {
-    float* x32 = c.take<float>(nn);
-    float* pm = c.take<float>(nn);
-    float* hp = c.take<float>(nn);
-    float* c32 = c.take<float>(nn);
-    StdBuffers<float> s = carve_std<float>(c, n, b);
-    double* rwork = c.take<double>(refine_workspace(n));
-    int* riwork = c.take<int>(refine_iwork(n));
-    // H' = X^T H X in FP64, then rounded: formed in FP32, its rounding error grows with cond(S) (about
-    // eps32 cond(S) ||H||), which at cond(S) = 1e4 leaves the FP32 eigenvectors too far off for the refinement to
-    // converge; in FP64 the FP32 solve starts from eps32 ||H'|| whatever S. The two products are a small part of
-    // the solve. H X and H' sit in the refinement's workspace, which is free until the refinement starts.
-    double* hx64 = rwork;
-    double* hp64 = rwork + nn;
-    copy_matrix<double, float>(st, n, n, x, n, x32, n);
-    gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, n, 0.0, hx64, n);
-    gemm_upper_tn<double>(st, n, n, 1.0, x, n, hx64, n, 0.0, hp64, n);
-    symmetrize<double>(st, 'U', n, hp64, n);
-    copy_matrix<double, float>(st, n, n, hp64, n, hp, n);
-    syevd_core<float>(h, s, n, b, hp, n, w, c32, n, x32, n, pm);
-    copy_matrix<float, double>(st, n, n, c32, n, cm, ldc);
-    refine_oa(st, n, hm, ldh, sm, lds, cm, ldc, w, h.refine_iters, rwork, riwork, h.dinfo + 3, h.check_ratios());
+  sygvd_core(h, c, n, hm, ldh, sm, lds, x, n, w, cm, ldc, mixed, b, n);
+}
+
+// ---- the geometry (S, U, X = U^-1) kept on the handle across the cycles of one geometry -----------------------------
+
+void Handle::set_geometry(int n, const double* sm, int lds) {
+  if (n <= 0) throw std::runtime_error("set_geometry: n must be positive");
+  const size_t nn = (size_t)n * n;
+  const size_t bytes = 3 * nn * sizeof(double);
+  if (bytes > geo_bytes) {
+    XTB_HIP(hipSetDevice(device));
+    if (geo) {
+      XTB_HIP(hipStreamSynchronize(stream));
+      XTB_HIP(hipFree(geo));
+      geo = nullptr;
+      geo_bytes = 0;
+    }
+    XTB_HIP(hipMalloc(&geo, bytes));
+    geo_bytes = bytes;
   }
+  geo_n = 0;
+  geo_s = geo;
+  geo_u = geo + nn;
+  geo_x = geo + 2 * nn;
+  reserve(nn * sizeof(double) + potrf_workspace(n) * sizeof(double) + 4096);
+  Carver c(arena);
+  double* scopy = c.take<double>(nn);
+  double* tmp = c.take<double>(potrf_workspace(n));
+  copy_matrix<double, double>(stream, n, n, sm, lds, geo_s, n);
+  copy_matrix<double, double>(stream, n, n, sm, lds, scopy, n);
+  potrf_inverse(stream, n, scopy, n, geo_u, n, geo_x, n, tmp, dinfo + 0, stream2, ev_band, ev_join);
+  geo_n = n;
+}
+
+void sygvd_geometry(Handle& h, int n, int m, const double* hm, int ldh, double* w, double* cm, int ldc, bool mixed,
+                    int b) {
+  if (n <= 0) return;
+  if (h.geo_n != n) throw std::runtime_error("sygvd_geometry: no geometry of this n on the handle (set it first)");
+  if (m < 1 || m > n) throw std::runtime_error("sygvd_geometry: need 1 <= m <= n");
+  if (b <= 0) b = h.band_for(n, mixed);
+  b = std::max(1, std::min(b, std::max(1, n - 1)));
+  check_size(n, b);
+  Carver c0(nullptr);
+  carve_core_bytes(c0, n, b, mixed, m);
+  h.reserve(c0.off + 4096);
+  Carver c(h.arena);
+  sygvd_core(h, c, n, hm, ldh, h.geo_s, n, h.geo_x, n, w, cm, ldc, mixed, b, m);
+}
+
+void congruence_device(Handle& h, int n, const double* hm, int ldh, const double* x, int ldx, double* z, int ldz) {
+  if (n <= 0) return;
+  const size_t nn = (size_t)n * n;
+  h.reserve(2 * nn * sizeof(double) + 4096);
+  Carver c(h.arena);
+  double* hx = c.take<double>(nn);
+  double* zz = c.take<double>(nn);
+  gemm<double>(h.stream, 'N', 'N', n, n, n, 1.0, hm, ldh, x, ldx, 0.0, hx, n);
+  gemm_upper_tn<double>(h.stream, n, n, 1.0, x, ldx, hx, n, 0.0, zz, n);
+  copy_upper<double>(h.stream, n, zz, n, z, ldz);
+}
+
+void back_solve_device(Handle& h, int n, int m, const double* x, int ldx, double* cm, int ldc) {
+  if (n <= 0 || m <= 0) return;
+  if (m > n) throw std::runtime_error("back_solve: need m <= n");
+  h.reserve((size_t)n * m * sizeof(double) + 4096);
+  double* t = static_cast<double*>(h.arena);
+  gemm<double>(h.stream, 'N', 'N', n, m, n, 1.0, x, ldx, cm, ldc, 0.0, t, n);
+  copy_matrix<double, double>(h.stream, n, m, t, n, cm, ldc);
 }
 
 void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz) {
@@ -359,7 +453,7 @@ void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz) {
   stedc(h.stream, h.plan(n), d, e, z, ldz, work, iwork, h.dinfo + 2, h.test_secular_maxit);
 }
 
-template void syevd_device<double>(Handle&, int, const double*, int, double*, double*, int, int);
-template void syevd_device<float>(Handle&, int, const float*, int, double*, float*, int, int);
+template void syevd_device<double>(Handle&, int, const double*, int, double*, double*, int, int, int);
+template void syevd_device<float>(Handle&, int, const float*, int, double*, float*, int, int, int);
 
 }  // namespace xtbeig

@@ -59,6 +59,13 @@ struct Handle {
   int tuned_band[2][NBUCKET] = {};
   // arena reallocations so far: captured graphs remember the generation they point into
   int arena_generation = 0;
+  // the geometry of an SCC sequence: S, U and X = U^-1 (n x n each, ld n), factored once by set_geometry and read by
+  // every cycle's solve until the next set_geometry
+  double* geo = nullptr;
+  size_t geo_bytes = 0;
+  int geo_n = 0;  // 0: no geometry set
+  double *geo_s = nullptr, *geo_u = nullptr, *geo_x = nullptr;
+  void set_geometry(int n, const double* sm, int lds);
   std::map<int, std::unique_ptr<DcPlan>> plans;
   std::map<std::pair<int, int>, std::unique_ptr<BackPlan>> backplans;
 
@@ -102,12 +109,19 @@ int bucket_edge(int q);
 int default_band(int n, bool fp32);
 
 template <typename T>
-void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b);
+void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ldz, int b, int m = -1);
 // u_out (optional): where S's Cholesky factor U (n x n, ld n, zero below the diagonal) stays in the handle's arena
 // until the next call on the handle.
 void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
                   int ldc, bool mixed, int b, const double** u_out = nullptr);
 void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz);
+// The cycle's generalized solve on the handle's geometry (set_geometry): all n eigenvalues, the first m eigenvectors.
+void sygvd_geometry(Handle& h, int n, int m, const double* hm, int ldh, double* w, double* cm, int ldc, bool mixed,
+                    int b);
+// Z's upper triangle = X^T H X (X upper triangular, n x n); Z's strictly lower triangle untouched.
+void congruence_device(Handle& h, int n, const double* hm, int ldh, const double* x, int ldx, double* z, int ldz);
+// C(:, 0:m) = X C(:, 0:m).
+void back_solve_device(Handle& h, int n, int m, const double* x, int ldx, double* cm, int ldc);
 
 // batch.hip: the generalized solve through a captured graph (one per (n, precision, band), inputs and outputs
 // staged through the graph's own buffers), the batched solve over concurrent sub-handles, and the band autotuner.

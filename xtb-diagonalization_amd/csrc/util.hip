@@ -14,6 +14,13 @@ __global__ void copy_kernel(int m, int n, const S* __restrict__ src, int lds, D*
     if (i < m) dst[idx(i, j, ldd)] = D(src[idx(i, j, lds)]);
 }
 
+// dst(i, j) = src(i, j) for i <= j (the upper triangle); dst's strictly lower triangle is not touched.
+template <typename T>
+__global__ void copy_upper_kernel(int n, const T* __restrict__ src, int lds, T* __restrict__ dst, int ldd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i <= j && i < n) dst[idx(i, j, ldd)] = src[idx(i, j, lds)];
+}
+
 // 32 x 32 tiles through LDS: tile (bi, bj) with bi <= bj reads the referenced triangle and writes both.
 template <typename T>
 __global__ void symmetrize_kernel(int n, T* a, int lda, bool upper) {
@@ -103,6 +110,13 @@ void copy_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int
 }
 
 template <typename T>
+void copy_upper(hipStream_t s, int n, const T* src, int lds, T* dst, int ldd) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(copy_upper_kernel<T>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, src, lds, dst, ldd);
+  XTB_LAUNCH_CHECK();
+}
+
+template <typename T>
 void symmetrize(hipStream_t s, char uplo, int n, T* a, int lda) {
   if (n <= 1) return;
   const int nt = cdiv(n, 32);
@@ -139,6 +153,7 @@ template void copy_matrix<double, float>(hipStream_t, int, int, const double*, i
 template void copy_matrix<float, double>(hipStream_t, int, int, const float*, int, double*, int);
 template void copy_matrix<float, float>(hipStream_t, int, int, const float*, int, float*, int);
 template void symmetrize<double>(hipStream_t, char, int, double*, int);
+template void copy_upper<double>(hipStream_t, int, const double*, int, double*, int);
 template void symmetrize<float>(hipStream_t, char, int, float*, int);
 template void set_zero<double>(hipStream_t, int, int, double*, int);
 template void set_zero<float>(hipStream_t, int, int, float*, int);

@@ -32,6 +32,13 @@ SIGNATURES = {
     "xtb_dstedc": (None, [c_int_p, vp, vp, vp, c_int_p, vp, c_int_p, vp, c_int_p, c_int_p]),
     "xtb_sstedc": (None, [c_int_p, vp, vp, vp, c_int_p, vp, c_int_p, vp, c_int_p, c_int_p]),
     "xtb_dsymmetrize_pad": (None, [c_int_p, vp, c_int_p, vp, c_int_p]),
+    "xtb_dcongruence_upper": (None, [c_int_p, vp, c_int_p, vp, c_int_p, vp, c_int_p]),
+    "xtb_dtrmm_back": (None, [c_int_p, c_int_p, vp, c_int_p, vp, c_int_p]),
+    "xtbeig_dev_dsyevd_partial": (ci, [vp, ci, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_dev_set_geometry": (ci, [vp, ci, vp, ci]),
+    "xtbeig_dev_congruence": (ci, [vp, ci, vp, ci, vp, ci]),
+    "xtbeig_dev_back_solve": (ci, [vp, ci, ci, vp, ci]),
+    "xtbeig_dev_sygvd_geometry": (ci, [vp, ci, ci, ci, vp, ci, vp, vp, ci]),
     "xtbeig_dsyevd": (None, [ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, vp, c_int_p, vp, c_int_p,
                              c_int_p]),
     "xtbeig_ssyevd": (None, [ctypes.c_char_p, ctypes.c_char_p, c_int_p, vp, c_int_p, vp, vp, c_int_p, vp, c_int_p,
@@ -256,6 +263,25 @@ def stedc(d_in, e_in, dtype=np.float64):
     return d, z, info.value
 
 
+def dcongruence_upper(h, x, z0=None):
+    """xtb_dcongruence_upper: Z's upper triangle = X^T H X; Z starts as z0 (its lower triangle is left as it is)."""
+    h = np.asfortranarray(h, dtype=np.float64)
+    x = np.asfortranarray(x, dtype=np.float64)
+    n = h.shape[0]
+    z = np.array(z0 if z0 is not None else np.zeros((n, n)), dtype=np.float64, order="F", copy=True)
+    lib().xtb_dcongruence_upper(_i(n), _ptr(h), _i(h.shape[0]), _ptr(x), _i(x.shape[0]), _ptr(z), _i(z.shape[0]))
+    return z
+
+
+def dtrmm_back(x, c, m):
+    """xtb_dtrmm_back: C(:, :m) = X C(:, :m) on a copy of c."""
+    x = np.asfortranarray(x, dtype=np.float64)
+    c = np.array(c, dtype=np.float64, order="F", copy=True)
+    n = x.shape[0]
+    lib().xtb_dtrmm_back(_i(n), _i(m), _ptr(x), _i(x.shape[0]), _ptr(c), _i(c.shape[0]))
+    return c
+
+
 def dsymmetrize_pad(src, ldd=None, n=None):
     """xtb_dsymmetrize_pad on the leading n x n block of src (lds = src.shape[0]); dst has ldd rows."""
     src = np.asfortranarray(src, dtype=np.float64)
@@ -325,6 +351,30 @@ class Handle:
         self._check(lib().xtbeig_dev_sygvd(self._h, precision, n, vp(h.data_ptr()), h.stride(0),
                                            vp(s.data_ptr()), s.stride(0), vp(w.data_ptr()), vp(c.data_ptr()),
                                            c.stride(0)))
+
+    def dsyevd_partial(self, a, w, z, m):
+        n = a.shape[0]
+        self._check(lib().xtbeig_dev_dsyevd_partial(self._h, n, m, vp(a.data_ptr()), n, vp(w.data_ptr()),
+                                                    vp(z.data_ptr()), z.stride(0) if z.dim() == 2 else n))
+
+    def set_geometry(self, s):
+        n = s.shape[0]
+        self._check(lib().xtbeig_dev_set_geometry(self._h, n, vp(s.data_ptr()), s.stride(0)))
+
+    def congruence(self, h, z):
+        n = h.shape[0]
+        self._check(lib().xtbeig_dev_congruence(self._h, n, vp(h.data_ptr()), h.stride(0), vp(z.data_ptr()),
+                                                z.stride(0)))
+
+    def back_solve(self, c, m):
+        """c: (cols, n) tensor holding the column-major n x cols matrix; the first m columns are transformed."""
+        n = c.shape[1]
+        self._check(lib().xtbeig_dev_back_solve(self._h, n, m, vp(c.data_ptr()), c.stride(0)))
+
+    def sygvd_geometry(self, h, w, c, m, precision=FP64):
+        n = h.shape[0]
+        self._check(lib().xtbeig_dev_sygvd_geometry(self._h, precision, n, m, vp(h.data_ptr()), h.stride(0),
+                                                    vp(w.data_ptr()), vp(c.data_ptr()), c.stride(0)))
 
     def dsyevd(self, a, w, z):
         n = a.shape[0]
