@@ -67,6 +67,10 @@ def kernel_roofline(torch, xtbeig, hnd, h, s, b):
     window of every bulge), the count charged to it when nothing stays in LDS or cache."""
     import numpy as np
     n = N
+    stream = torch.cuda.ExternalStream(hnd.stream_ptr())
+    # torch's own operations below run on the library's stream, ordered with its kernels
+    ctx = torch.cuda.stream(stream)
+    ctx.__enter__()
     hd = torch.from_numpy(np.ascontiguousarray(h.T)).cuda()
     sd = torch.from_numpy(np.ascontiguousarray(s.T)).cuda()
     u = torch.zeros_like(sd)
@@ -86,11 +90,11 @@ def kernel_roofline(torch, xtbeig, hnd, h, s, b):
     v2 = torch.zeros((n, n), dtype=torch.float32, device="cuda")
     d = torch.zeros(n, dtype=torch.float32, device="cuda")
     e = torch.zeros(n, dtype=torch.float32, device="cuda")
-    stream = torch.cuda.ExternalStream(hnd.stream_ptr())
     reps = 5
     abw = [ab.clone() for _ in range(reps + 1)]
     hnd.synchronize()
     torch.cuda.synchronize()
+    ctx.__exit__(None, None, None)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * (reps + 1))]
     for r in range(reps + 1):
         ev[2 * r].record(stream)

@@ -115,7 +115,8 @@ typedef struct xtbeig_handle *xtbeig_handle_t;
 /* Precision of a generalized solve. */
 enum { XTBEIG_FP64 = 0, XTBEIG_FP32_REFINE = 1 };
 
-/* Create a handle on a HIP device (its own non-blocking stream). Returns 0 on success. */
+/* Create a handle on a HIP device with its own stream (an ordinary blocking stream: ordered with the legacy default
+ * stream, so that a caller's default-stream work on the inputs precedes the solve). Returns 0 on success. */
 int xtbeig_create(xtbeig_handle_t *handle, int device);
 int xtbeig_destroy(xtbeig_handle_t handle);
 /* Enqueue on the caller's HIP stream (hipStream_t passed as void*) instead of the handle's own. */
@@ -320,6 +321,24 @@ int xtbeig_dev_density_backward_recompute(xtbeig_handle_t handle, int batch, int
 int xtbeig_dev_dc_merge(xtbeig_handle_t handle, int n, int n1, double rho, double *d, double *q, int ldq,
                         double *d_deflated, double *q2, int *k_ctot, double *dlamda, double *w, int *indxc,
                         double *rho_out);
+
+/* ---- 3. Dumps and hashes (host code; no device) ------------------------------------------------------------- */
+
+/* HS.bin, the reference's dump of one generalized eigenproblem (Sources/EigenBench/main.swift:151-175,
+ * scripts/kernels/refine.f90:64-83): int32 n, H, S (n x n float64, column-major), optionally e (n) and C (n x n).
+ * info: n and whether the eigenpairs follow (the file size must match one of the two layouts). Returns nonzero on
+ * error, with the message in xtbeig_hsbin_error(). */
+int xtbeig_hsbin_info(const char *path, int *n, int *has_solution);
+/* Read H and S (and e, C when not NULL; the file must hold them) into caller arrays with leading dimensions; h or s
+ * may be NULL to skip them. */
+int xtbeig_hsbin_read(const char *path, int n, double *h, int ldh, double *s, int lds, double *e, double *c, int ldc);
+/* Write H, S and, when e and c are both given, the eigenpairs. */
+int xtbeig_hsbin_write(const char *path, int n, const double *h, int ldh, const double *s, int lds, const double *e,
+                       const double *c, int ldc);
+const char *xtbeig_hsbin_error(void);
+/* FNV-1a (64-bit) of `bytes` bytes, continuing from `hash` (start from 0xcbf29ce484222325): the reference's
+ * fingerprint of a variant's tridiagonal (d, e) bits, main.swift:253-266. */
+unsigned long long xtbeig_fnv1a(const void *data, long long bytes, unsigned long long hash);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,10 @@ int default_band(int n, bool fp32) {
 
 Handle::Handle(int dev) : device(dev) {
   XTB_HIP(hipSetDevice(device));
-  XTB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  // the handle's stream is an ordinary (blocking) stream: work on the legacy default stream (a caller's memset or
+  // copy of an input, a framework's default-stream kernels) is ordered with it without an explicit event; the second
+  // stream is internal, forked from and joined to this one by events
+  XTB_HIP(hipStreamCreateWithFlags(&stream, hipStreamDefault));
   own_stream = true;
   XTB_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
   XTB_HIP(hipEventCreateWithFlags(&ev_band, hipEventDisableTiming));

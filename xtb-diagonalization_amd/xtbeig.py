@@ -33,6 +33,11 @@ SIGNATURES = {
     "xtb_sstedc": (None, [c_int_p, vp, vp, vp, c_int_p, vp, c_int_p, vp, c_int_p, c_int_p]),
     "xtb_dsymmetrize_pad": (None, [c_int_p, vp, c_int_p, vp, c_int_p]),
     "xtb_dcongruence_upper": (None, [c_int_p, vp, c_int_p, vp, c_int_p, vp, c_int_p]),
+    "xtbeig_hsbin_info": (ci, [ctypes.c_char_p, c_int_p, c_int_p]),
+    "xtbeig_hsbin_read": (ci, [ctypes.c_char_p, ci, vp, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_hsbin_write": (ci, [ctypes.c_char_p, ci, vp, ci, vp, ci, vp, vp, ci]),
+    "xtbeig_hsbin_error": (ctypes.c_char_p, []),
+    "xtbeig_fnv1a": (ctypes.c_ulonglong, [vp, ll, ctypes.c_ulonglong]),
     "xtb_dtrmm_back": (None, [c_int_p, c_int_p, vp, c_int_p, vp, c_int_p]),
     "xtbeig_dev_dsyevd_partial": (ci, [vp, ci, ci, vp, ci, vp, vp, ci]),
     "xtbeig_dev_set_geometry": (ci, [vp, ci, vp, ci]),
@@ -280,6 +285,45 @@ def dtrmm_back(x, c, m):
     n = x.shape[0]
     lib().xtb_dtrmm_back(_i(n), _i(m), _ptr(x), _i(x.shape[0]), _ptr(c), _i(c.shape[0]))
     return c
+
+
+FNV_OFFSET = 0xcbf29ce484222325
+
+
+def fnv1a(*arrays, hash=FNV_OFFSET):
+    """xtbeig_fnv1a over the bytes of the arrays in turn (the eigen-bench fingerprint of a tridiagonal's d, e)."""
+    for a in arrays:
+        a = np.ascontiguousarray(a)
+        hash = lib().xtbeig_fnv1a(a.ctypes.data_as(vp), a.nbytes, hash)
+    return hash
+
+
+def hsbin_read(path):
+    """(h, s, e, c) through the library's C reader; e and c are None when the file holds only H and S."""
+    n, sol = ctypes.c_int(0), ctypes.c_int(0)
+    if lib().xtbeig_hsbin_info(str(path).encode(), ctypes.byref(n), ctypes.byref(sol)):
+        raise ValueError(lib().xtbeig_hsbin_error().decode())
+    n = n.value
+    h = np.zeros((n, n), order="F")
+    s = np.zeros((n, n), order="F")
+    e = np.zeros(n) if sol.value else None
+    c = np.zeros((n, n), order="F") if sol.value else None
+    rc = lib().xtbeig_hsbin_read(str(path).encode(), n, _ptr(h), n, _ptr(s), n,
+                                 _ptr(e) if e is not None else None, _ptr(c) if c is not None else None, n)
+    if rc:
+        raise ValueError(lib().xtbeig_hsbin_error().decode())
+    return h, s, e, c
+
+
+def hsbin_write(path, h, s, e=None, c=None):
+    h = np.asfortranarray(h, dtype=np.float64)
+    s = np.asfortranarray(s, dtype=np.float64)
+    n = h.shape[0]
+    ea = np.ascontiguousarray(e, dtype=np.float64) if e is not None else None
+    cf = np.asfortranarray(c, dtype=np.float64) if c is not None else None
+    if lib().xtbeig_hsbin_write(str(path).encode(), n, _ptr(h), n, _ptr(s), n, _ptr(ea) if ea is not None else None,
+                                _ptr(cf) if cf is not None else None, n):
+        raise ValueError(lib().xtbeig_hsbin_error().decode())
 
 
 def dsymmetrize_pad(src, ldd=None, n=None):
