@@ -174,10 +174,11 @@ def host_cores():
 
 def kernels(torch, np, workloads, xtbeig, hnd, h, s, which, reps=5):
     """The hot kernels at n = 1204, each run `reps` times with nothing else in flight, for counter collection
-    (rocprofv3 --pmc): FP64 and FP32 GEMM n^3 (gemm_kernel) and the symmetric congruence product (gemm_upper_kernel),
-    the dense-to-band reduction at b = 8 in both precisions (panel kernels), the FP32 bulge chase at b = 8, and the
-    one-stage reduction (trd_symv_kernel and the dlatrd column kernels). `which`: the groups to run (gemm, band,
-    chase, sytrd). Prints one line per stage."""
+    (rocprofv3 --pmc): FP64 and FP32 GEMM n^3 (gemm_kernel), the dense-to-band reduction at b = 8 in both
+    precisions (panel kernels), the FP32 bulge chase at b = 8, the FP64 and FP32+refine generalized solves of the
+    bench workload (gemm_upper_kernel, the refinement and check kernels, every stage in place), and the one-stage
+    reduction (trd_symv_kernel and the dlatrd column kernels). `which`: the groups to run (gemm, band, chase, solve,
+    sytrd). Prints one line per stage."""
     n = N
     for dt in ((torch.float64, torch.float32) if "gemm" in which else ()):
         a = torch.randn(n, n, dtype=dt, device="cuda")
@@ -211,6 +212,18 @@ def kernels(torch, np, workloads, xtbeig, hnd, h, s, which, reps=5):
                 hnd.chase(n, bw, ABc, V2, D, E)
                 hnd.synchronize()
             log(f"chase b={bw} {dt} x{reps}")
+    if "solve" in which:
+        hd = torch.from_numpy(np.ascontiguousarray(h.T)).cuda()
+        sd = torch.from_numpy(np.ascontiguousarray(s.T)).cuda()
+        w = torch.zeros(n, dtype=torch.float64, device="cuda")
+        c = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+        for prec in (xtbeig.FP64, xtbeig.FP32_REFINE):
+            for _ in range(reps):
+                hnd.sygvd(hd, sd, w, c, prec)
+                st = hnd.synchronize()
+                if st != [0, 0, 0, 0]:
+                    raise RuntimeError(f"sygvd status {st}")
+            log(f"sygvd precision {prec} x{reps}")
     if "sytrd" not in which:
         return
     import ctypes
@@ -228,6 +241,86 @@ def kernels(torch, np, workloads, xtbeig, hnd, h, s, which, reps=5):
             raise RuntimeError(xtbeig.last_error())
         hnd.synchronize()
     log("one-stage dsytrd x2")
+
+
+def rocprof_summary(path, top=40):
+    """Markdown tables of a rocprofv3 --output-format csv directory: kernel statistics (from --stats) and, for a
+    --pmc run, per kernel and grid size the dispatch count, mean duration and mean of each counter, plus MFMA rates (MOPS x 512
+    flops; MFMA busy cycles over 256 CUs x 4 SIMDs x the per-XCD GRBM_GUI_ACTIVE) and the largest single
+    __amd_rocclr_copyBuffer / fill dispatch (the --pmc-calibration copy of 256 MiB) for scaling FETCH_SIZE and
+    WRITE_SIZE. Host-side post-processing only."""
+    import collections
+    import csv
+    import glob
+    out = []
+    for f in glob.glob(os.path.join(path, "**", "*kernel_stats.csv"), recursive=True):
+        rows = list(csv.DictReader(open(f)))
+        tot = sum(float(r["TotalDurationNs"]) for r in rows)
+        out.append(f"Kernel time summed over all dispatches: {tot / 1e6:.2f} ms.\n")
+        out.append("| kernel | calls | total us | avg us | % |\n|---|---:|---:|---:|---:|")
+        for r in rows[:top]:
+            out.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e3:.1f} | "
+                       f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    for f in glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True):
+        per = collections.defaultdict(dict)  # (kernel, grid, dispatch) -> counter -> value
+        dur = {}
+        for r in csv.DictReader(open(f)):
+            key = (_short(r["Kernel_Name"]), r.get("Grid_Size", "?"), r["Dispatch_Id"])
+            per[key][r["Counter_Name"]] = per[key].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            if r.get("Start_Timestamp"):
+                dur[key] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+        names = sorted({c for v in per.values() for c in v})
+        for by_grid, title, rows in ((False, "per kernel", top), (True, "per kernel and grid size", top // 2)):
+            agg = collections.defaultdict(list)
+            for key, v in per.items():
+                agg[f"{key[0]} [grid {key[1]}]" if by_grid else key[0]].append((v, dur.get(key)))
+            out.append(f"\nCounters {title} (means over dispatches):\n")
+            out.append("| kernel | dispatches | mean us | " + " | ".join(names) + " | derived |")
+            out.append("|---|---:|---:|" + "---:|" * len(names) + "---|")
+            order = sorted(agg, key=lambda k: -sum((d or 0) for _, d in agg[k]))
+            for k in order[:rows]:
+                out.append(_counter_row(k, agg[k], names))
+    return "\n".join(out)
+
+
+def _counter_row(k, vs, names):
+    """One markdown row: means of each counter over the dispatches vs, and the rates derived from them."""
+    ds = [d for _, d in vs if d]
+    mean_ns = sum(ds) / len(ds) if ds else float("nan")
+    means = {c: sum(v.get(c, 0.0) for v, _ in vs) / len(vs) for c in names}
+    derived = []
+    for prec, peak in (("F64", MI355X_FP64_TFLOPS), ("F32", MI355X_FP32_TFLOPS)):
+        mops = means.get(f"SQ_INSTS_VALU_MFMA_MOPS_{prec}", 0.0)
+        if mops > 0 and ds:
+            tf = mops * 512 / mean_ns / 1e3
+            derived.append(f"{prec}: {mops * 512 / 1e9:.3f} GFLOP, {tf:.1f} TF/s = {tf / peak:.1%} of peak")
+    if means.get("SQ_VALU_MFMA_BUSY_CYCLES", 0) > 0 and means.get("GRBM_GUI_ACTIVE", 0) > 0:
+        util = means["SQ_VALU_MFMA_BUSY_CYCLES"] / (means["GRBM_GUI_ACTIVE"] / 8 * 1024)
+        derived.append(f"MFMA busy {util:.1%}")
+    if "SQ_INSTS_LDS_LOAD_BANDWIDTH" in means and ds:
+        lds_b = 64 * (means["SQ_INSTS_LDS_LOAD_BANDWIDTH"] + means.get("SQ_INSTS_LDS_STORE_BANDWIDTH", 0))
+        derived.append(f"LDS {lds_b / 1e6:.1f} MB, {lds_b / mean_ns:.1f} GB/s")
+    if ("copyBuffer" in k or "Fill" in k) and any(c in names for c in ("FETCH_SIZE", "WRITE_SIZE")):
+        big = max(vs, key=lambda t: t[0].get("FETCH_SIZE", 0.0) + t[0].get("WRITE_SIZE", 0.0))[0]
+        derived.append("largest dispatch: " + ", ".join(f"{c} {big[c]:.1f} KiB" for c in
+                                                        ("FETCH_SIZE", "WRITE_SIZE") if c in big))
+    return (f"| `{k}` | {len(vs)} | {mean_ns / 1e3:.2f} | " + " | ".join(f"{means[c]:.1f}" for c in names) + " | " +
+            "; ".join(derived) + " |")
+
+
+def _short(name):
+    """A kernel's name without its namespace and argument list."""
+    name = name.replace("void ", "").replace("xtbeig::(anonymous namespace)::", "")
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    return name[:cut][:110]
 
 
 def components(torch, np, workloads, xtbeig, hnd, h, s):
@@ -388,8 +481,13 @@ def main():
                     help="time each entry point of the path alone at n = 1204 (one JSON line each) and exit")
     ap.add_argument("--kernels", default=None, metavar="GROUPS",
                     help="run the hot kernels alone a few times each (for rocprofv3 counter runs) and exit; GROUPS "
-                         "is a comma-separated subset of gemm,band,chase,sytrd")
+                         "is a comma-separated subset of gemm,band,chase,solve,sytrd")
+    ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
+                    help="print markdown tables of a rocprofv3 csv output directory and exit (no GPU use)")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        print(rocprof_summary(args.rocprof_summary))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the eigensolves are "
               "independent per GPU and would shard by problem with no exchange.", file=sys.stderr)

@@ -1,4 +1,5 @@
-"""Build recipe for libxtbeig.so (gfx950): every csrc/*.hip through hipcc, linked into one shared library in-tree.
+"""Build recipe for libxtbeig.so (gfx950): every csrc/*.hip through hipcc, linked into one shared library in-tree;
+then libxtbeig_lapack.so, the opt-in library of LAPACK Fortran names forwarding to it (gcc).
 
 Usage: python xtb-diagonalization_amd/build.py [--jobs N] [--verbose]
 """
@@ -12,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libxtbeig.so")
+INTERPOSE_LIB = os.path.join(HERE, "libxtbeig_lapack.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -53,7 +55,22 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_interpose()
     return LIB
+
+
+def build_interpose():
+    """libxtbeig_lapack.so: the opt-in LAPACK-name forwards (interpose/lapack_names.c), linked against libxtbeig.so
+    and finding it beside itself at run time."""
+    src = os.path.join(HERE, "interpose", "lapack_names.c")
+    tmp = INTERPOSE_LIB + ".tmp"
+    cmd = [os.environ.get("CC", "gcc"), "-O2", "-std=c99", "-Wall", "-Werror", "-shared", "-fPIC", src, "-o", tmp,
+           "-L" + HERE, "-lxtbeig", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"interposing library failed to build:\n{r.stderr}")
+    os.replace(tmp, INTERPOSE_LIB)
+    return INTERPOSE_LIB
 
 
 if __name__ == "__main__":

@@ -110,6 +110,66 @@ int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T
   XTB_LAUNCH_CHECK();
   return splits;
 }
+// C = op(A) B for a thin B (n <= 64 columns: one column tile): the K range split in chunks of kc, one grid layer per
+// chunk writing its partial product to part + z M N (ld M), then the partials summed in chunk order (fixed, so the
+// result does not depend on scheduling). A grid of cdiv(M, 64) x 1 tiles alone would leave most CUs idle.
+namespace {
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void gemm_thin_kernel(int M, int N, int K, int kc, const T* __restrict__ A,
+                                                        int lda, const T* __restrict__ B, int ldb,
+                                                        T* __restrict__ part) {
+  __shared__ T As[BK][BM + PAD];
+  __shared__ T Bs[BK][BN + PAD];
+  const int z = blockIdx.y;
+  const int k0 = z * kc, kz = min(kc, K - k0);
+  const T* a0 = TA ? A + k0 : A + (int64_t)k0 * lda;
+  gemm_tile<T, TA, false>(M, N, kz, T(1), a0, lda, B + k0, ldb, T(0), part + (int64_t)z * M * N, M,
+                          blockIdx.x * BM, 0, As, Bs);
+}
+template <typename T>
+__global__ void sum_parts_kernel(int M, int N, int splits, const T* __restrict__ part, T* __restrict__ C, int ldc) {
+  const int64_t mn = (int64_t)M * N;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mn) return;
+  T acc = part[q];
+  for (int z = 1; z < splits; ++z) acc += part[z * mn + q];
+  const int i = (int)(q % M), j = (int)(q / M);
+  C[idx(i, j, ldc)] = acc;
+}
+}  // namespace
+
+size_t gemm_thin_parts(int m, int n, int k) {
+  const int splits = cdiv(k, THIN_KC);
+  return splits > 1 ? (size_t)splits * m * n : 0;
+}
+
+template <typename T>
+void gemm_thin(hipStream_t s, char ta, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+               T* part) {
+  if (n > BN) throw std::runtime_error("gemm_thin: more than 64 columns");
+  if (m <= 0 || n <= 0) return;
+  const int splits = cdiv(k, THIN_KC);
+  if (splits <= 1) {  // one chunk: the plain product, no partials
+    gemm<T>(s, ta, 'N', m, n, k, T(1), A, lda, B, ldb, T(0), C, ldc);
+    return;
+  }
+  if (ta == 'T' || ta == 't')
+    hipLaunchKernelGGL((gemm_thin_kernel<T, true>), dim3(cdiv(m, BM), splits), dim3(256), 0, s, m, n, k, THIN_KC, A,
+                       lda, B, ldb, part);
+  else
+    hipLaunchKernelGGL((gemm_thin_kernel<T, false>), dim3(cdiv(m, BM), splits), dim3(256), 0, s, m, n, k, THIN_KC,
+                       A, lda, B, ldb, part);
+  XTB_LAUNCH_CHECK();
+  const int64_t mn = (int64_t)m * n;
+  hipLaunchKernelGGL(sum_parts_kernel<T>, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, m, n, splits, part, C,
+                     ldc);
+  XTB_LAUNCH_CHECK();
+}
+template void gemm_thin<double>(hipStream_t, char, int, int, int, const double*, int, const double*, int, double*,
+                                int, double*);
+template void gemm_thin<float>(hipStream_t, char, int, int, int, const float*, int, const float*, int, float*, int,
+                               float*);
+
 template <typename T>
 void gemm_upper_tn(hipStream_t s, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* C,
                    int ldc, int batch, int64_t sA, int64_t sB, int64_t sC) {

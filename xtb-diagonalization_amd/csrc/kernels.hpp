@@ -20,6 +20,15 @@ void gemm_upper_tn(hipStream_t s, int n, int k, T alpha, const T* A, int lda, co
 template <typename T>
 int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int kc);
 
+// C (m x n, n <= 64) = op(A) B, op(A) m x k ('N' or 'T'), B k x n: K split in chunks of THIN_KC so that a thin
+// product still fills the GPU, partials summed in a fixed order. part: gemm_thin_parts(m, n, k) elements of scratch
+// (none when k <= THIN_KC: then one plain product).
+constexpr int THIN_KC = 64;
+size_t gemm_thin_parts(int m, int n, int k);
+template <typename T>
+void gemm_thin(hipStream_t s, char ta, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+               T* part);
+
 // util.hip ---------------------------------------------------------------------------------------------------------
 // dst(m x n) = src (with conversion); both column-major.
 template <typename S, typename D>

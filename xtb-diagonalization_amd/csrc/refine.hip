@@ -470,6 +470,7 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   // no step: the eigenpairs stay as the FP32 solve left them (already in ascending order), unchecked
   if (iterations <= 0) return;
   if (n > SCAN_THREADS * SCAN_CHUNK) throw std::runtime_error("refine_oa: n too large for the cluster scan");
+  if (gemm_thin_parts(n, 2 * CHECK_VECS, n) > (size_t)n * n) throw std::runtime_error("refine_oa: check scratch");
   const size_t nn = (size_t)n * n;
   double* sc = work;
   double* hc = sc + nn;
@@ -535,10 +536,11 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   // check: Y = C [V | Lambda V], S Y, H Y1, C^T (S Y1)
   hipLaunchKernelGGL(check_vectors_kernel, dim3(cdiv(n, 256), CHECK_VECS), dim3(256), 0, s, n, w, pv);
   XTB_LAUNCH_CHECK();
-  gemm<double>(s, 'N', 'N', n, 2 * CHECK_VECS, n, 1.0, c, ldc, pv, n, 0.0, py, n);
-  gemm<double>(s, 'N', 'N', n, 2 * CHECK_VECS, n, 1.0, shm, n, py, n, 0.0, psy, n);
-  gemm<double>(s, 'N', 'N', n, CHECK_VECS, n, 1.0, shm + nn, n, py, n, 0.0, phy, n);
-  gemm<double>(s, 'T', 'N', n, CHECK_VECS, n, 1.0, c, ldc, psy, n, 0.0, po, n);
+  // thin products, K split across the GPU; the partials go to r (n x n, free once the steps are done)
+  gemm_thin<double>(s, 'N', n, 2 * CHECK_VECS, n, c, ldc, pv, n, py, n, r);
+  gemm_thin<double>(s, 'N', n, 2 * CHECK_VECS, n, shm, n, py, n, psy, n, r);
+  gemm_thin<double>(s, 'N', n, CHECK_VECS, n, shm + nn, n, py, n, phy, n, r);
+  gemm_thin<double>(s, 'T', n, CHECK_VECS, n, c, ldc, psy, n, po, n, r);
   hipLaunchKernelGGL(check_partial_kernel, dim3(CHECK_BLOCKS), dim3(256), 0, s, n, phy, psy, po, pv, shm + nn, n,
                      part);
   XTB_LAUNCH_CHECK();

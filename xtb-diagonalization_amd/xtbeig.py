@@ -127,6 +127,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run xtb-diagonalization_amd/build.py (no CPU fallback)")
+        # torch's wheel carries its own HIP and HSA runtimes; loading it first makes libxtbeig.so bind to those
+        # same copies (by soname), so that torch's device pointers and streams are valid in the library's calls.
+        # Loaded the other way round, the process would hold two runtimes and the second would see no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
