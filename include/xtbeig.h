@@ -241,9 +241,11 @@ int xtbeig_dev_sygvd_graph(xtbeig_handle_t handle, int precision, int n, const d
 int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, const int *n, const double *const *h,
                              const int *ldh, const double *const *s, const int *lds, double *const *w,
                              double *const *c, const int *ldc, int slots, int use_graph, float *latency_ms);
-/* Band-width autotuner: for every n bucket of the handle's table that meets [nmin, nmax] (2 <= nmin <= nmax <=
- * 2048), times the candidate band widths 8, 16 and 32 on a synthetic pair of the bucket's middle size and keeps the
- * fastest in the table, which later solves on the handle use (an explicit xtbeig_set_tuning band overrides it).
+/* Autotuner: first the GEMM tile widths over [nmin, nmax] (xtbeig_autotune_gemm, FP64, and FP32 too for precision
+ * XTBEIG_FP32_REFINE), then for every n bucket of the handle's table that meets [nmin, nmax] (2 <= nmin <= nmax <=
+ * 2048), the candidate band widths (also the panel width of the dense-to-band reduction) 8, 16 and 32 timed on a
+ * synthetic pair of the bucket's middle size, the fastest kept in the table, which later solves on the handle use
+ * (an explicit xtbeig_set_tuning band overrides it).
  * Outputs per tuned bucket (arrays of xtbeig_band_buckets() entries, may be NULL): its first n, the band, the time
  * in ms; *count the number of buckets tuned. Reference counterpart: SBROptions.recommended's fixed table
  * (SBROptions.swift:73-82). */
@@ -254,6 +256,27 @@ int xtbeig_autotune(xtbeig_handle_t handle, int precision, int nmin, int nmax, i
 int xtbeig_get_band_table(xtbeig_handle_t handle, int precision, int *bucket_start, int *band);
 int xtbeig_set_band_table(xtbeig_handle_t handle, int precision, const int *band);
 int xtbeig_band_buckets(void);
+
+/* GEMM tile shape. Every product runs in tiles of 64 rows by 64 or 128 columns. The width of a product (m x n,
+ * batched b times) is the tuned width of its bucket of 64 x 64 tile counts (b ceil(m / 64) ceil(n / 64), half-octave
+ * buckets: the count sets how the grid fills the CUs), or, for an untuned bucket, a round model (ceil(tiles / CUs)
+ * rounds of one tile each). Results do not depend on the width (each element sums the same products in the same
+ * order). Process-wide, per precision. No reference counterpart (the reference calls the host BLAS).
+ * xtbeig_set_gemm_tile: force one width for every product (64 or 128); 0 restores the choice above.
+ * xtbeig_gemm_tile_width: the width a product of m x n, batch times, gets (-1 on invalid sizes).
+ * xtbeig_gemm_tile_buckets: the number of buckets; xtbeig_get/set_gemm_tile_table: the first tile count of each
+ * bucket and its width (0: not tuned), to save and restore them.
+ * xtbeig_autotune_gemm: for each bucket that the square products of n in [nmin, nmax] reach (1 <= nmin <= nmax <=
+ * 2048), times an n x n x n product of about the bucket's middle tile count in both widths on the handle's device
+ * and stores the faster (scratch of 3 nmax^2 elements allocated for the call); the buckets tuned (first tile count,
+ * width; arrays of xtbeig_gemm_tile_buckets() entries, may be NULL) and their count (may be NULL) are returned. */
+int xtbeig_set_gemm_tile(int width);
+int xtbeig_gemm_tile_width(int m, int n, int batch, int precision);
+int xtbeig_gemm_tile_buckets(void);
+int xtbeig_get_gemm_tile_table(int precision, int *bucket_tiles, int *width);
+int xtbeig_set_gemm_tile_table(int precision, const int *width);
+int xtbeig_autotune_gemm(xtbeig_handle_t handle, int precision, int nmin, int nmax, int *bucket_tiles, int *width,
+                         int *count);
 
 /* SCC precision schedule, per handle (mirrors xTB's xtb_setSCCPrecision / xtb_getSCCPrecision contract, reference
  * LOG.md:356-370 and scripts/test_sccprecision.py:79-161): fp32_rmsq = 0 runs every cycle in FP64 (the default);

@@ -1021,6 +1021,80 @@ int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, c
   });
 }
 
+int xtbeig_set_gemm_tile(int width) {
+  if (width != 0 && width != 64 && width != 128) return set_error("xtbeig_set_gemm_tile: width must be 0, 64 or 128");
+  xtbeig::set_gemm_tile(width);
+  return 0;
+}
+
+int xtbeig_gemm_tile_width(int m, int n, int batch, int precision) {
+  if (m < 1 || n < 1 || batch < 1) {
+    set_error("xtbeig_gemm_tile_width: m, n and batch must be >= 1");
+    return -1;
+  }
+  return xtbeig::gemm_tile_width(m, n, batch, precision != XTBEIG_FP32_REFINE);
+}
+
+int xtbeig_gemm_tile_buckets(void) { return xtbeig::GEMM_TILE_BUCKETS; }
+
+int xtbeig_get_gemm_tile_table(int precision, int* bucket_tiles, int* width) {
+  if (!width) return set_error("xtbeig_get_gemm_tile_table: null width");
+  xtbeig::get_gemm_tile_table(precision != XTBEIG_FP32_REFINE, width);
+  if (bucket_tiles)
+    for (int q = 0; q < xtbeig::GEMM_TILE_BUCKETS; ++q) bucket_tiles[q] = (int)xtbeig::gemm_tile_bucket_start(q);
+  return 0;
+}
+
+int xtbeig_set_gemm_tile_table(int precision, const int* width) {
+  if (!width) return set_error("xtbeig_set_gemm_tile_table: null width");
+  for (int q = 0; q < xtbeig::GEMM_TILE_BUCKETS; ++q)
+    if (width[q] != 0 && width[q] != 64 && width[q] != 128)
+      return set_error("xtbeig_set_gemm_tile_table: widths must be 0, 64 or 128");
+  xtbeig::set_gemm_tile_table(precision != XTBEIG_FP32_REFINE, width);
+  return 0;
+}
+
+extern "C++" {
+namespace {
+// The tile widths of one precision timed on the handle's device and stream (scratch of 3 nmax^2 elements allocated
+// for the call), stored for gemm()'s choice. Returns the number of buckets tuned.
+template <typename T>
+int autotune_gemm_tiles_on(Handle& h, int nmin, int nmax, int* bucket_tiles, int* width) {
+  hipStream_t s = h.stream;
+  XTB_HIP(hipSetDevice(h.device));
+  const size_t nn = (size_t)nmax * nmax;
+  T* buf = nullptr;
+  XTB_HIP(hipMalloc(&buf, 3 * nn * sizeof(T)));
+  int count = 0;
+  try {
+    xtbeig::set_zero<T>(s, nmax, 3 * nmax, buf, nmax);  // operand values do not change the timing
+    count = xtbeig::autotune_gemm_tiles<T>(s, nmin, nmax, buf, buf + nn, buf + 2 * nn, bucket_tiles, width);
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    throw;
+  }
+  XTB_HIP(hipStreamSynchronize(s));
+  XTB_HIP(hipFree(buf));
+  return count;
+}
+}  // namespace
+}  // extern "C++"
+
+int xtbeig_autotune_gemm(xtbeig_handle_t handle, int precision, int nmin, int nmax, int* bucket_tiles, int* width,
+                         int* count) {
+  if (!handle) return set_error("xtbeig_autotune_gemm: null handle");
+  if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
+    return set_error("xtbeig_autotune_gemm: unknown precision");
+  if (nmin < 1 || nmax < nmin || nmax > xtbeig::MAX_N)
+    return set_error(("xtbeig_autotune_gemm: need 1 <= nmin <= nmax <= " + std::to_string(xtbeig::MAX_N)).c_str());
+  return guarded([&] {
+    const int c = precision == XTBEIG_FP64 ? autotune_gemm_tiles_on<double>(handle->h, nmin, nmax, bucket_tiles, width)
+                                           : autotune_gemm_tiles_on<float>(handle->h, nmin, nmax, bucket_tiles, width);
+    if (count) *count = c;
+  });
+}
+
 int xtbeig_autotune(xtbeig_handle_t handle, int precision, int nmin, int nmax, int* bucket_start, int* band,
                     float* ms, int* count) {
   if (!handle) return set_error("xtbeig_autotune: null handle");
@@ -1033,6 +1107,10 @@ int xtbeig_autotune(xtbeig_handle_t handle, int precision, int nmin, int nmax, i
     if (!(xtbeig::bucket_edge(q + 1) - 1 < nmin || xtbeig::bucket_edge(q) > nmax)) ++cnt;
   if (count) *count = cnt;
   return guarded([&] {
+    // the GEMM tile costs first (both precisions for the mixed path, whose congruence and refinement are FP64), so
+    // that the band widths are timed with the products as they will run
+    autotune_gemm_tiles_on<double>(handle->h, nmin, nmax, nullptr, nullptr);
+    if (precision == XTBEIG_FP32_REFINE) autotune_gemm_tiles_on<float>(handle->h, nmin, nmax, nullptr, nullptr);
     xtbeig::autotune_bands(handle->h, precision == XTBEIG_FP32_REFINE, nmin, nmax, bucket_start, band, ms);
   });
 }

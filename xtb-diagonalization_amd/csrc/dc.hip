@@ -619,8 +619,7 @@ __global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, cons
                                                          const double* __restrict__ s_all, double* __restrict__ q_all,
                                                          int ldq) {
   using namespace gemm_detail;
-  __shared__ double As[BK][BM + PAD];
-  __shared__ double Bs[BK][BN + PAD];
+  __shared__ __attribute__((aligned(16))) double lds[tile_lds_elems<double>()];
   const int node = blockIdx.z >> 1, half = blockIdx.z & 1;
   const Node nd = nodes[node];
   const NodeOut o = out[node];
@@ -634,7 +633,7 @@ __global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, cons
   const double* A = q2_all + nd.q2off + (half ? (size_t)n1 * n12 : 0);
   const double* B = s_all + nd.q2off + (half ? o.ctot[0] : 0);
   double* C = q_all + idx(nd.off + (half ? n1 : 0), nd.off, ldq);
-  gemm_tile<double, false, false>(rows, k, inner, 1.0, A, rows, B, nd.sz, 0.0, C, ldq, m0, c0, As, Bs);
+  gemm_tile<double, false, false>(rows, k, inner, 1.0, A, rows, B, nd.sz, 0.0, C, ldq, m0, c0, lds);
 }
 
 // dlamrg of the K merged (ascending) and n - K deflated (descending) eigenvalues.

@@ -9,26 +9,35 @@
 #include "gemm_tile.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cmath>
+#include <type_traits>
+
 namespace xtbeig {
 
 namespace {
 
 using namespace gemm_detail;
 
-constexpr int GEMM_THREADS = 512;  // eight wavefronts per 64 x 64 tile, two per SIMD
+// Tile shapes of gemm_kernel: 64 rows by TBN columns, eight wavefronts (two per SIMD): 32 x 16 each for TBN 64,
+// 32 x 32 each for TBN 128.
+template <int TBN> struct Shape;
+template <> struct Shape<64> { static constexpr int NT = 512; };
+template <> struct Shape<128> { static constexpr int NT = 512; };
 
-template <typename T, bool TA, bool TB>
-__global__ __launch_bounds__(GEMM_THREADS) void gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A,
-                                                            int lda, int64_t sA, const T* __restrict__ B, int ldb,
-                                                            int64_t sB, T beta, T* __restrict__ C, int ldc,
-                                                            int64_t sC) {
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
+template <typename T, bool TA, bool TB, int TBN>
+__global__ __launch_bounds__(Shape<TBN>::NT) void gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A,
+                                                              int lda, int64_t sA, const T* __restrict__ B, int ldb,
+                                                              int64_t sB, T beta, T* __restrict__ C, int ldc,
+                                                              int64_t sC) {
+  __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T, TBN>()];
   A += blockIdx.z * sA;
   B += blockIdx.z * sB;
   C += blockIdx.z * sC;
-  gemm_tile<T, TA, TB, GEMM_THREADS>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.x * BM, blockIdx.y * BN,
-                                     As, Bs);
+  gemm_tile<T, TA, TB, Shape<TBN>::NT, TBN>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.x * BM,
+                                            blockIdx.y * TBN, lds);
 }
 
 // The tiles (I, J) with I <= J of an n x n C only, one workgroup each (tile q = J (J + 1) / 2 + I).
@@ -36,8 +45,7 @@ template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_upper_kernel(int N, int K, T alpha, const T* __restrict__ A, int lda,
                                                          int64_t sA, const T* __restrict__ B, int ldb, int64_t sB,
                                                          T beta, T* __restrict__ C, int ldc, int64_t sC) {
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
+  __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T>()];
   A += blockIdx.y * sA;
   B += blockIdx.y * sB;
   C += blockIdx.y * sC;
@@ -46,16 +54,24 @@ __global__ __launch_bounds__(256) void gemm_upper_kernel(int N, int K, T alpha, 
   while (J * (J + 1) / 2 > q) --J;
   while ((J + 1) * (J + 2) / 2 <= q) ++J;
   const int I = q - J * (J + 1) / 2;
-  gemm_tile<T, TA, TB>(N, N, K, alpha, A, lda, B, ldb, beta, C, ldc, I * BM, J * BN, As, Bs);
+  gemm_tile<T, TA, TB>(N, N, K, alpha, A, lda, B, ldb, beta, C, ldc, I * BM, J * BN, lds);
+}
+
+template <typename T, bool TA, bool TB, int TBN>
+void launch_w(hipStream_t s, int m, int n, int k, T alpha, const T* A, int lda, int64_t sA, const T* B, int ldb,
+              int64_t sB, T beta, T* C, int ldc, int64_t sC, int batch) {
+  dim3 grid(cdiv(m, BM), cdiv(n, TBN), batch);
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, TBN>), grid, dim3(Shape<TBN>::NT), 0, s, m, n, k, alpha, A, lda, sA, B,
+                     ldb, sB, beta, C, ldc, sC);
+  XTB_LAUNCH_CHECK();
 }
 
 template <typename T, bool TA, bool TB>
 void launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, int lda, int64_t sA, const T* B, int ldb,
             int64_t sB, T beta, T* C, int ldc, int64_t sC, int batch) {
-  dim3 grid(cdiv(m, BM), cdiv(n, BN), batch);
-  hipLaunchKernelGGL((gemm_kernel<T, TA, TB>), grid, dim3(GEMM_THREADS), 0, s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta,
-                     C, ldc, sC);
-  XTB_LAUNCH_CHECK();
+  const int w = gemm_tile_width(m, n, batch, std::is_same<T, double>::value);
+  if (w == 128) launch_w<T, TA, TB, 128>(s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
+  else launch_w<T, TA, TB, 64>(s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
 }
 
 // C = beta C when K = 0 (alpha op(A) op(B) vanishes).
@@ -67,6 +83,119 @@ __global__ void scale_kernel(int M, int N, T beta, T* C, int ldc, int64_t sC) {
 }
 
 }  // namespace
+
+namespace {
+std::atomic<int> tile_override{0};
+// tuned width per bucket of the product's count of 64 x 64 tiles (0: not tuned), [0] FP64, [1] FP32
+std::atomic<int> tile_table[2][GEMM_TILE_BUCKETS];
+
+int device_cus() {
+  static std::atomic<int> cus[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int c = cus[dev].load(std::memory_order_relaxed);
+  if (c == 0) {
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cus[dev].store(c, std::memory_order_relaxed);
+  }
+  return c;
+}
+
+// The round model's cost per tile column of the 128-wide tile relative to the 64-wide one (x 1000), from n x n x n
+// products at n = 600, 1204 and 1788 on MI355X: the 64-wide tiles run two workgroups per CU and win in FP64 where
+// the round counts tie; in FP32 the 128-wide tile is the cheaper per column.
+constexpr int MODEL_COST_128[2] = {1000, 950};
+}  // namespace
+
+int gemm_tile_bucket(int64_t tiles64) {
+  int q = 0;
+  while (q + 1 < GEMM_TILE_BUCKETS && gemm_tile_bucket_start(q + 1) <= tiles64) ++q;
+  return q;
+}
+// bucket q starts at ceil(2^(q / 2)) tiles of 64 x 64 (half-octave buckets), at least one past bucket q - 1's start
+int64_t gemm_tile_bucket_start(int q) {
+  int64_t start = 1;
+  for (int r = 1; r <= q; ++r) start = std::max(start + 1, (int64_t)std::ceil(std::pow(2.0, 0.5 * r) - 1e-9));
+  return start;
+}
+
+void set_gemm_tile(int width) { tile_override.store(width, std::memory_order_relaxed); }
+int get_gemm_tile() { return tile_override.load(std::memory_order_relaxed); }
+void set_gemm_tile_table(bool fp64, const int* width) {
+  for (int q = 0; q < GEMM_TILE_BUCKETS; ++q) tile_table[fp64 ? 0 : 1][q].store(width[q], std::memory_order_relaxed);
+}
+void get_gemm_tile_table(bool fp64, int* width) {
+  for (int q = 0; q < GEMM_TILE_BUCKETS; ++q) width[q] = tile_table[fp64 ? 0 : 1][q].load(std::memory_order_relaxed);
+}
+
+// The tile width of an m x n product batched `batch` times: a forced width, else the tuned width of the product's
+// bucket (by its count of 64 x 64 tiles, which sets how the grid fills the CUs), else a round model: a launch of T
+// tiles on C compute units takes ceil(T / C) rounds of one tile each, a tile of width w costing w x its cost per
+// column. The width only partitions C's columns among workgroups: every element sums the same products in the same
+// order whatever the width, so the results do not depend on it.
+int gemm_tile_width(int m, int n, int batch, bool fp64) {
+  const int ov = tile_override.load(std::memory_order_relaxed);
+  if (ov == 64 || ov == 128) return ov;
+  const int64_t t64 = (int64_t)batch * cdiv(m, BM) * cdiv(n, 64);
+  const int tuned = tile_table[fp64 ? 0 : 1][gemm_tile_bucket(t64)].load(std::memory_order_relaxed);
+  if (tuned == 64 || tuned == 128) return tuned;
+  const int64_t cu = device_cus();
+  const int64_t t128 = (int64_t)batch * cdiv(m, BM) * cdiv(n, 128);
+  const int64_t c64 = (t64 + cu - 1) / cu * 64 * 1000;
+  const int64_t c128 = (t128 + cu - 1) / cu * 128 * MODEL_COST_128[fp64 ? 0 : 1];
+  return c128 < c64 ? 128 : 64;
+}
+
+// Autotuning of the tile width: for each bucket of 64 x 64 tile counts that the square products of n in [nmin, nmax]
+// reach, an n x n x n product with about the bucket's middle tile count, timed with events on stream s in both
+// widths (best of 3 after a warm-up), the faster stored. a, b, c: nmax x nmax scratch each. Returns the buckets tuned
+// (first tile count, width) through the output arrays (GEMM_TILE_BUCKETS entries each, may be null).
+template <typename T>
+int autotune_gemm_tiles(hipStream_t s, int nmin, int nmax, T* a, T* b, T* c, int* bucket_tiles, int* widths) {
+  const bool fp64 = std::is_same<T, double>::value;
+  const int qlo = gemm_tile_bucket((int64_t)cdiv(nmin, 64) * cdiv(nmin, 64));
+  const int qhi = gemm_tile_bucket((int64_t)cdiv(nmax, 64) * cdiv(nmax, 64));
+  hipEvent_t e0, e1;
+  XTB_HIP(hipEventCreate(&e0));
+  XTB_HIP(hipEventCreate(&e1));
+  const int keep = get_gemm_tile();
+  int count = 0;
+  for (int q = qlo; q <= qhi; ++q) {
+    const double lo = (double)gemm_tile_bucket_start(q);
+    const double hi = (double)(q + 1 < GEMM_TILE_BUCKETS ? gemm_tile_bucket_start(q + 1) - 1 : 4 * lo);
+    const int side = std::max(1, (int)std::lround(std::sqrt(std::sqrt(lo * hi))));  // tiles per side
+    const int n = std::min(nmax, std::max(nmin, 64 * side - 8));
+    int best_w = 64;
+    float best_ms = 1e30f;
+    for (int w : {64, 128}) {
+      set_gemm_tile(w);
+      float tmin = 1e30f;
+      for (int r = 0; r < 4; ++r) {
+        XTB_HIP(hipEventRecord(e0, s));
+        gemm<T>(s, 'N', 'N', n, n, n, T(1), a, n, b, n, T(0), c, n);
+        XTB_HIP(hipEventRecord(e1, s));
+        XTB_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        XTB_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (r > 0) tmin = std::min(tmin, t);
+      }
+      if (tmin < best_ms) {
+        best_ms = tmin;
+        best_w = w;
+      }
+    }
+    tile_table[fp64 ? 0 : 1][q].store(best_w, std::memory_order_relaxed);
+    if (bucket_tiles) bucket_tiles[count] = (int)gemm_tile_bucket_start(q);
+    if (widths) widths[count] = best_w;
+    ++count;
+  }
+  set_gemm_tile(keep);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return count;
+}
+template int autotune_gemm_tiles<double>(hipStream_t, int, int, double*, double*, double*, int*, int*);
+template int autotune_gemm_tiles<float>(hipStream_t, int, int, float*, float*, float*, int*, int*);
 
 template <typename T>
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb,
@@ -92,12 +221,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void gemm_splitk_kernel(int M, int N, int K, int kc, const T* __restrict__ A,
                                                           int lda, const T* __restrict__ B, int ldb,
                                                           T* __restrict__ C) {
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
+  __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T>()];
   const int z = blockIdx.z;
   const int k0 = z * kc, kz = min(kc, K - k0);
   gemm_tile<T, false, false>(M, N, kz, T(1), A + (int64_t)k0 * lda, lda, B + k0, ldb, T(0),
-                             C + (int64_t)z * M * N, M, blockIdx.x * BM, blockIdx.y * BN, As, Bs);
+                             C + (int64_t)z * M * N, M, blockIdx.x * BM, blockIdx.y * BN, lds);
 }
 }  // namespace
 
@@ -118,13 +246,12 @@ template <typename T, bool TA>
 __global__ __launch_bounds__(256) void gemm_thin_kernel(int M, int N, int K, int kc, const T* __restrict__ A,
                                                         int lda, const T* __restrict__ B, int ldb,
                                                         T* __restrict__ part) {
-  __shared__ T As[BK][BM + PAD];
-  __shared__ T Bs[BK][BN + PAD];
+  __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T>()];
   const int z = blockIdx.y;
   const int k0 = z * kc, kz = min(kc, K - k0);
   const T* a0 = TA ? A + k0 : A + (int64_t)k0 * lda;
   gemm_tile<T, TA, false>(M, N, kz, T(1), a0, lda, B + k0, ldb, T(0), part + (int64_t)z * M * N, M,
-                          blockIdx.x * BM, 0, As, Bs);
+                          blockIdx.x * BM, 0, lds);
 }
 template <typename T>
 __global__ void sum_parts_kernel(int M, int N, int splits, const T* __restrict__ part, T* __restrict__ C, int ldc) {

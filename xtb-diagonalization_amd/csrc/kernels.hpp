@@ -11,6 +11,20 @@ template <typename T>
 void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb,
           T beta, T* C, int ldc, int batch = 1, int64_t sA = 0, int64_t sB = 0, int64_t sC = 0);
 
+// The tile width (64 or 128 columns, 64 rows) gemm() uses for an m x n product batched `batch` times, and its
+// controls: a fixed width for every product (0: chosen per product), and the tuned width per bucket of the product's
+// count of 64 x 64 tiles (half-octave buckets; 0: not tuned, a round model decides), per precision, process-wide.
+constexpr int GEMM_TILE_BUCKETS = 24;
+int gemm_tile_bucket(int64_t tiles64);
+int64_t gemm_tile_bucket_start(int q);
+int gemm_tile_width(int m, int n, int batch, bool fp64);
+void set_gemm_tile(int width);
+int get_gemm_tile();
+void set_gemm_tile_table(bool fp64, const int* width);
+void get_gemm_tile_table(bool fp64, int* width);
+template <typename T>
+int autotune_gemm_tiles(hipStream_t s, int nmin, int nmax, T* a, T* b, T* c, int* bucket_tiles, int* widths);
+
 // C = alpha A^T B + beta C (C n x n) over C's 64 x 64 tiles on and above the diagonal only; the tiles below it are
 // not written (a syrk-shaped update whose consumer reads the upper triangle).
 template <typename T>

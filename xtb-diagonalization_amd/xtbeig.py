@@ -98,6 +98,12 @@ SIGNATURES = {
     "xtbeig_synchronize": (ci, [vp, c_int_p]),
     "xtbeig_refine_check": (ci, [vp, c_dbl_p]),
     "xtbeig_last_error": (ctypes.c_char_p, []),
+    "xtbeig_set_gemm_tile": (ci, [ci]),
+    "xtbeig_gemm_tile_width": (ci, [ci, ci, ci, ci]),
+    "xtbeig_gemm_tile_buckets": (ci, []),
+    "xtbeig_get_gemm_tile_table": (ci, [ci, c_int_p, c_int_p]),
+    "xtbeig_set_gemm_tile_table": (ci, [ci, c_int_p]),
+    "xtbeig_autotune_gemm": (ci, [vp, ci, ci, ci, c_int_p, c_int_p, c_int_p]),
     "xtbeig_dev_sygvd": (ci, [vp, ci, ci, vp, ci, vp, ci, vp, vp, ci]),
     "xtbeig_dev_dsyevd": (ci, [vp, ci, vp, ci, vp, vp, ci]),
     "xtbeig_dev_ssyevd": (ci, [vp, ci, vp, ci, vp, vp, ci]),
@@ -149,6 +155,33 @@ def _i(v):
 
 def _ptr(a):
     return a.ctypes.data_as(vp)
+
+
+def set_gemm_tile(width):
+    """Force one GEMM tile width (64 or 128) for every product of the process; 0 restores the per-product choice."""
+    if lib().xtbeig_set_gemm_tile(int(width)) != 0:
+        raise ValueError(last_error())
+
+
+def gemm_tile_width(m, n, batch=1, precision=0):
+    return lib().xtbeig_gemm_tile_width(int(m), int(n), int(batch), int(precision))
+
+
+def gemm_tile_table(precision=0):
+    """[(first 64 x 64 tile count of the bucket, width or 0)] for every bucket."""
+    nb = lib().xtbeig_gemm_tile_buckets()
+    st = (ctypes.c_int * nb)()
+    wd = (ctypes.c_int * nb)()
+    if lib().xtbeig_get_gemm_tile_table(int(precision), st, wd) != 0:
+        raise RuntimeError(last_error())
+    return [(st[q], wd[q]) for q in range(nb)]
+
+
+def set_gemm_tile_table(precision, widths):
+    nb = lib().xtbeig_gemm_tile_buckets()
+    w = (ctypes.c_int * nb)(*[int(x) for x in widths])
+    if lib().xtbeig_set_gemm_tile_table(int(precision), w) != 0:
+        raise ValueError(last_error())
 
 
 def last_error():
@@ -505,6 +538,16 @@ class Handle:
         cnt = ctypes.c_int(0)
         self._check(lib().xtbeig_autotune(self._h, precision, nmin, nmax, st, bd, ms, ctypes.byref(cnt)))
         return [(st[q], bd[q], ms[q]) for q in range(cnt.value)]
+
+    def autotune_gemm(self, precision, nmin, nmax):
+        """Times the GEMM tile widths per bucket of tile counts reached by n in [nmin, nmax] and stores the faster;
+        returns [(first 64 x 64 tile count of the bucket, width)]."""
+        nb = lib().xtbeig_gemm_tile_buckets()
+        st = (ctypes.c_int * nb)()
+        wd = (ctypes.c_int * nb)()
+        cnt = ctypes.c_int(0)
+        self._check(lib().xtbeig_autotune_gemm(self._h, precision, nmin, nmax, st, wd, ctypes.byref(cnt)))
+        return [(st[q], wd[q]) for q in range(cnt.value)]
 
     def band_table(self, precision):
         nb = lib().xtbeig_band_buckets()
