@@ -122,7 +122,7 @@ int xtbeig_destroy(xtbeig_handle_t handle);
 /* Enqueue on the caller's HIP stream (hipStream_t passed as void*) instead of the handle's own. */
 int xtbeig_set_stream(xtbeig_handle_t handle, void *stream);
 void *xtbeig_get_stream(xtbeig_handle_t handle);
-/* Band width of the two-stage reduction (0: the built-in table per n; at most 32) and the refinement iterations
+/* Band width of the two-stage reduction (0: the built-in table per n; at most 64) and the refinement iterations
  * (0 to 8; 0 returns the FP32 eigenpairs unrefined and unchecked). */
 int xtbeig_set_tuning(xtbeig_handle_t handle, int band, int refine_iterations);
 /* Test setting: cap the divide and conquer's secular-equation iterations at secular_maxit (0 gives up after the

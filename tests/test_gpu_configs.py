@@ -31,7 +31,7 @@ def dev(torch, a):
 
 # ---- configuration 2 ------------------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("band", [32, 0])
+@pytest.mark.parametrize("band", [32, 64, 0])
 def test_config2_sequence_n1788(torch_cuda, band):
     torch = torch_cuda
     n, steps = 1788, 20

@@ -1,5 +1,5 @@
-"""The generalized solve at the workload sizes (n = 1204; n = 1788 with the band-32 configuration and with the
-built-in band) and at sizes on either side of the 64-row tiles and of n = 1204, in both precisions, against host
+"""The generalized solve at the workload sizes (n = 1204; n = 1788 with the band-32 and band-64 configurations and
+with the built-in band; band 48 at n = 1204) and at sizes on either side of the 64-row tiles and of n = 1204, in both precisions, against host
 LAPACK (scipy eigh) with the FP64 gates: eig err <= 1e-12 ||H||, residual <= 1e-13, ||C^T S C - I|| <= 1e-12 n."""
 import numpy as np
 import pytest
@@ -21,7 +21,8 @@ def host(t):
 
 @pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("n,band", [(1204, 0), (1788, 32), (1788, 0), (63, 0), (65, 0), (127, 0), (129, 0),
-                                    (257, 0), (1203, 0), (1205, 0), (600, 16), (700, 32)])
+                                    (257, 0), (1203, 0), (1205, 0), (600, 16), (700, 32),
+                                    (1788, 64), (1204, 48), (600, 64), (100, 64)])
 def test_sygvd_sizes(handle, torch_cuda, n, band, mixed):
     torch = torch_cuda
     h, s = workloads.generalized_pair(n, 1001 + n)

@@ -78,20 +78,26 @@ __global__ __launch_bounds__(64 * BULGE_WAVES) void bulge_apply_kernel(int n, in
 #pragma unroll
       for (int jj = 0; jj < BULGE_G; ++jj) {
         const int j = REV ? jj : BULGE_G - 1 - jj;
-        T wr[B];
+        // the reflector in registers for B <= 32; wider ones are read from LDS twice (a 64-entry copy beside the
+        // 79-row window would not fit the register file)
+        constexpr int BR = B <= 32 ? B : 1;
+        T wr[BR];
+        if constexpr (B <= 32) {
 #pragma unroll
-        for (int r = 0; r < B; ++r) wr[r] = ws[j * WS + r];
+          for (int r = 0; r < B; ++r) wr[r] = ws[j * WS + r];
+        }
+        auto w_at = [&](int r) { if constexpr (B <= 32) return wr[r]; else return ws[j * WS + r]; };
         T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
 #pragma unroll
         for (int r = 0; r < B; r += 4) {
-          d0 += wr[r] * z[j + r];
-          d1 += wr[r + 1] * z[j + r + 1];
-          d2 += wr[r + 2] * z[j + r + 2];
-          d3 += wr[r + 3] * z[j + r + 3];
+          d0 += w_at(r) * z[j + r];
+          d1 += w_at(r + 1) * z[j + r + 1];
+          d2 += w_at(r + 2) * z[j + r + 2];
+          d3 += w_at(r + 3) * z[j + r + 3];
         }
         const T dot = (d0 + d1) + (d2 + d3);
 #pragma unroll
-        for (int r = 0; r < B; ++r) z[j + r] -= wr[r] * dot;
+        for (int r = 0; r < B; ++r) z[j + r] -= w_at(r) * dot;
         // compiler-only fence: the next reflector's LDS reads are not hoisted above this one (register pressure)
         __asm__ volatile("" ::: "memory");
       }
@@ -395,12 +401,12 @@ void launch_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldz
 
 }  // namespace
 
-int bulge_width_class(int b) { return b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : 32; }
+int bulge_width_class(int b) { return b <= 4 ? 4 : b <= 8 ? 8 : b <= 16 ? 16 : b <= 32 ? 32 : 64; }
 
 BackPlan::BackPlan(int n_, int b_) : n(n_), b(b_), g(BULGE_G) {
-  if (b > 32) throw std::runtime_error("back-transform: band width above 32");
+  if (b > 64) throw std::runtime_error("back-transform: band width above 64");
   // bulge blocks in a valid application order: k ascending, s0 descending. A block's footprint is the kernel's
-  // window, g + B - 1 rows (B = b rounded up to 4, 8, 16 or 32), running into the zero padding below row n.
+  // window, g + B - 1 rows (B = b rounded up to 4, 8, 16, 32 or 64), running into the zero padding below row n.
   std::vector<BulgeBlock> bl;
   const int sweeps = n - 2;
   const int foot = g + bulge_width_class(b) - 1;
@@ -463,7 +469,8 @@ BackPlan::~BackPlan() {
 }
 
 int zt_ld(int m) { return cdiv(std::max(m, 1), 64) * 64; }
-int zt_rows(int n) { return n + 64; }
+// the widest bulge window, BULGE_G + 64 - 1 rows from a block's first row (< n), stays inside the zero rows
+int zt_rows(int n) { return n + BULGE_G + 64; }
 
 template <typename T>
 size_t back_band_workspace(int n, int b, int m) {
@@ -480,7 +487,8 @@ void back_transform_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt,
     case 4: launch_bulges<T, 4>(s, p, v2, zt, ldzt, m, transposed); break;
     case 8: launch_bulges<T, 8>(s, p, v2, zt, ldzt, m, transposed); break;
     case 16: launch_bulges<T, 16>(s, p, v2, zt, ldzt, m, transposed); break;
-    default: launch_bulges<T, 32>(s, p, v2, zt, ldzt, m, transposed); break;
+    case 32: launch_bulges<T, 32>(s, p, v2, zt, ldzt, m, transposed); break;
+    default: launch_bulges<T, 64>(s, p, v2, zt, ldzt, m, transposed); break;
   }
 }
 

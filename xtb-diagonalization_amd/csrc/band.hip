@@ -716,23 +716,25 @@ __global__ __launch_bounds__(256) void panel_update_factor_kernel(int m, T* __re
 }
 
 // The per-panel reductions over the m rows run on cdiv(m, PR) workgroups, PR rows each; k x k partial products per
-// row block go to a partial buffer (ld 32, 1024 elements per block) and are summed by the consumer in block order.
+// row block go to a partial buffer (ld KM, KM^2 elements per block) and are summed by the consumer in block order.
+// KM: the widest panel of the general path (band widths up to 64).
 constexpr int PR = 64;
+constexpr int KM = 64;
+constexpr int KM1 = KM + 1;
 
-// Rows i0 ..< i0 + PR of the m x k (ld m) matrix a into sh (PR x 33), zeros past m and k.
+// Rows i0 ..< i0 + PR of the m x k (ld m) matrix a into sh (PR x KM1), zeros past m and k.
 template <typename T>
-__device__ inline void stage_rows(int m, int k, int i0, const T* __restrict__ a, T (*sh)[33]) {
-  for (int e = threadIdx.x; e < PR * 32; e += blockDim.x) {
+__device__ inline void stage_rows(int m, int k, int i0, const T* __restrict__ a, T (*sh)[KM1]) {
+  for (int e = threadIdx.x; e < PR * KM; e += blockDim.x) {
     const int i = e % PR, c = e / PR;
     sh[i][c] = (i0 + i < m && c < k) ? a[(int64_t)(i0 + i) + (int64_t)c * m] : T(0);
   }
 }
 
-// out[r + 32 c] = sum_i A(i, r) B(i, c) over the staged rows (k x k; sym: r <= c only).
+// out[r + KM c] = sum_i A(i, r) B(i, c) over the staged rows (k x k; sym: r <= c only).
 template <typename T>
-__device__ inline void rows_atb(int k, const T (*A)[33], const T (*B)[33], T* __restrict__ out, bool sym) {
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
+__device__ inline void rows_atb(int k, const T (*A)[KM1], const T (*B)[KM1], T* __restrict__ out, bool sym) {
+  for (int u = 0; u < KM * KM / 256; ++u) {
     const int o = threadIdx.x + 256 * u;
     if (o < k * k) {
       const int r = o % k, c = o / k;
@@ -743,20 +745,20 @@ __device__ inline void rows_atb(int k, const T (*A)[33], const T (*B)[33], T* __
           s0 += A[i][r] * B[i][c];
           s1 += A[i + 1][r] * B[i + 1][c];
         }
-        out[r + 32 * c] = s0 + s1;
+        out[r + KM * c] = s0 + s1;
       }
     }
   }
 }
 
-// k x k (ld 32) sum of the nblk partials into sh (32 x 33).
+// k x k (ld KM) sum of the nblk partials into sh (KM x KM1).
 template <typename T>
-__device__ inline void sum_partials(int k, int nblk, const T* __restrict__ part, T (*sh)[33], bool sym) {
+__device__ inline void sum_partials(int k, int nblk, const T* __restrict__ part, T (*sh)[KM1], bool sym) {
   for (int o = threadIdx.x; o < k * k; o += blockDim.x) {
     const int r = o % k, c = o / k;
     if (sym && r > c) continue;
     T sum = 0;
-    for (int q = 0; q < nblk; ++q) sum += part[(int64_t)q * 1024 + r + 32 * c];
+    for (int q = 0; q < nblk; ++q) sum += part[(int64_t)q * KM * KM + r + KM * c];
     sh[r][c] = sum;
   }
 }
@@ -764,10 +766,10 @@ __device__ inline void sum_partials(int k, int nblk, const T* __restrict__ part,
 // Gp[blk] = V(rows)^T V(rows), upper triangle.
 template <typename T>
 __global__ __launch_bounds__(256) void panel_gram_kernel(int m, int k, const T* __restrict__ v, T* __restrict__ gp) {
-  __shared__ T Vs[PR][33];
+  __shared__ T Vs[PR][KM1];
   stage_rows<T>(m, k, blockIdx.x * PR, v, Vs);
   __syncthreads();
-  rows_atb<T>(k, Vs, Vs, gp + (int64_t)blockIdx.x * 1024, true);
+  rows_atb<T>(k, Vs, Vs, gp + (int64_t)blockIdx.x * KM * KM, true);
 }
 
 // G = sum of the Gram partials; the forward columnwise T (I - V T V^T = H_1 ... H_k; T(0:c, c) = -tau_c T(0:c, 0:c)
@@ -776,13 +778,13 @@ template <typename T>
 __global__ __launch_bounds__(256) void panel_tvt_kernel(int m, int k, const T* __restrict__ v,
                                                         const T* __restrict__ tau, const T* __restrict__ gp, int nblk,
                                                         T* __restrict__ tt, int ldt, T* __restrict__ vt) {
-  __shared__ T G[32][33];
-  __shared__ T Ts[32][33];
-  __shared__ T Vs[PR][33];
+  __shared__ T G[KM][KM1];
+  __shared__ T Ts[KM][KM1];
+  __shared__ T Vs[PR][KM1];
   const int t = threadIdx.x, i0 = blockIdx.x * PR;
   sum_partials<T>(k, nblk, gp, G, true);
   stage_rows<T>(m, k, i0, v, Vs);
-  for (int e = t; e < 32 * 33; e += blockDim.x) (&Ts[0][0])[e] = T(0);
+  for (int e = t; e < KM * KM1; e += blockDim.x) (&Ts[0][0])[e] = T(0);
   __syncthreads();
   for (int c = 0; c < k; ++c) {
     if (t < c) {
@@ -812,11 +814,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void panel_xy_kernel(int m, int k, const T* __restrict__ vt,
                                                        const T* __restrict__ xpart, int nsplit, T* __restrict__ x,
                                                        T* __restrict__ yp) {
-  __shared__ T Xs[PR][33];
-  __shared__ T Ws[PR][33];
+  __shared__ T Xs[PR][KM1];
+  __shared__ T Ws[PR][KM1];
   const int t = threadIdx.x, i0 = blockIdx.x * PR;
   const int64_t mk = (int64_t)m * k;
-  for (int e = t; e < PR * 32; e += blockDim.x) {
+  for (int e = t; e < PR * KM; e += blockDim.x) {
     const int i = e % PR, c = e / PR;
     T sum = 0;
     if (i0 + i < m && c < k) {
@@ -828,15 +830,15 @@ __global__ __launch_bounds__(256) void panel_xy_kernel(int m, int k, const T* __
   }
   stage_rows<T>(m, k, i0, vt, Ws);
   __syncthreads();
-  rows_atb<T>(k, Ws, Xs, yp + (int64_t)blockIdx.x * 1024, false);
+  rows_atb<T>(k, Ws, Xs, yp + (int64_t)blockIdx.x * KM * KM, false);
 }
 
 // Y = sum of the Yp partials; W(rows) = X(rows) - V(rows) Y / 2, in place of x.
 template <typename T>
 __global__ __launch_bounds__(256) void panel_w_kernel(int m, int k, const T* __restrict__ v, const T* __restrict__ yp,
                                                       int nblk, T* __restrict__ x) {
-  __shared__ T Y[32][33];
-  __shared__ T Vs[PR][33];
+  __shared__ T Y[KM][KM1];
+  __shared__ T Vs[PR][KM1];
   const int t = threadIdx.x, i0 = blockIdx.x * PR;
   sum_partials<T>(k, nblk, yp, Y, false);
   stage_rows<T>(m, k, i0, v, Vs);
@@ -890,8 +892,9 @@ constexpr int SPLIT_ROWS = 128;  // K per split of the X = A2 VT product
 template <typename T>
 size_t band_workspace(int n, int b) {
   // tau (b) | buf [V W V] (3 n b) | VT (n b) | split-K partials of X (ceil(n / 64) n b: the fused path splits K by
-  // 64 (FP64) or 128 (FP32), the general path by SPLIT_ROWS) | k x k partials of the row blocks (ceil(n / PR) 1024)
-  return (size_t)b + 4 * (size_t)n * b + (size_t)((n + 63) / 64) * n * b + (size_t)((n + PR - 1) / PR) * 1024 + 64;
+  // 64 (FP64) or 128 (FP32), the general path by SPLIT_ROWS) | k x k partials of the row blocks (ceil(n / PR) KM^2)
+  return (size_t)b + 4 * (size_t)n * b + (size_t)((n + 63) / 64) * n * b + (size_t)((n + PR - 1) / PR) * KM * KM +
+         64;
 }
 
 template <typename T, int NB, int R>
@@ -994,7 +997,7 @@ bool update_factor(hipStream_t s, int m, int b, T* a2, int lda, const T* v, int 
 
 template <typename T>
 void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T* ab, T* work) {
-  if (b > 32 || b < 1) throw std::runtime_error("reduce_to_band: band width must be in [1, 32]");
+  if (b > KM || b < 1) throw std::runtime_error("reduce_to_band: band width must be in [1, 64]");
   T* tau = work;
   T* buf = tau + b;
   T* vt = buf + 3 * (size_t)n * b;

@@ -15,7 +15,7 @@ namespace xtbeig {
 constexpr int NSTATUS = 4;
 // the widest band the dense-to-band panels, the chase and the back-transforms take, and the largest n of a solve
 // (the panel factorization holds a column of up to 2048 rows below the band in one workgroup's registers)
-constexpr int MAX_BAND = 32;
+constexpr int MAX_BAND = 64;
 constexpr int MAX_N = 2048;
 // the largest n of the allocation-free Cholesky entry point (its scratch array is sized for it)
 constexpr int NOALLOC_MAX_N = 2048;
