@@ -60,6 +60,38 @@ def timed_solves(torch, hnd, hd, sd, w, c, prec, steps, warmup):
     return t0.elapsed_time(t1) / steps, wall / steps * 1e3
 
 
+def concurrent_throughput(torch, np, workloads, xtbeig, hnd, h, s, wref, count=32, slots=16):
+    """`count` solves of the workload, all in flight at once through xtbeig_dev_sygvd_batched (`slots` sub-handle
+    streams, HIP's default hardware queues), one warm-up pass then two timed, the rate of the faster; the first and
+    last solve of the timed pass checked against the oracle. Returns {precision: {...}}."""
+    n = h.shape[0]
+    hd = torch.from_numpy(np.ascontiguousarray(h.T)).cuda()
+    sd = torch.from_numpy(np.ascontiguousarray(s.T)).cuda()
+    ws = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(count)]
+    cs = [torch.zeros((n, n), dtype=torch.float64, device="cuda") for _ in range(count)]
+    out = {}
+    for name, prec in (("fp64", xtbeig.FP64), ("fp32_refine", xtbeig.FP32_REFINE)):
+        hnd.sygvd_batched([hd] * count, [sd] * count, ws, cs, prec, slots=slots)
+        hnd.synchronize()
+        best = None
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            hnd.sygvd_batched([hd] * count, [sd] * count, ws, cs, prec, slots=slots)
+            st = hnd.synchronize()
+            dt = time.perf_counter() - t0
+            if st != [0, 0, 0, 0]:
+                raise RuntimeError(f"batched status {st}")
+            best = dt if best is None else min(best, dt)
+        ok = all(workloads.passes(workloads.gates(h, s, ws[q].cpu().numpy(), cs[q].cpu().numpy().T, wref))
+                 for q in (0, count - 1))
+        out[name] = {"solves_per_s": count / best, "wall_ms": best * 1e3, "checked_pass": ok}
+        log(f"throughput {name}", out[name])
+    out["config"] = {"solves_in_flight": count, "slots": slots, "api": "xtbeig_dev_sygvd_batched",
+                     "timing": "host wall clock around the call, inputs resident in HBM"}
+    return out
+
+
 def kernel_roofline(torch, xtbeig, hnd, h, s, b):
     """The bulge chase of the FP32+refine solve (one workgroup, band resident in LDS) is its largest kernel. It is
     timed alone, with HIP events on the handle's stream, on the FP32 band that the library's own stage 1 makes from
@@ -241,6 +273,60 @@ def kernels(torch, np, workloads, xtbeig, hnd, h, s, which, reps=5):
             raise RuntimeError(xtbeig.last_error())
         hnd.synchronize()
     log("one-stage dsytrd x2")
+
+
+def band_sweep(torch, np, workloads, xtbeig, hnd, sizes=(1204, 1788), bands=(8, 16, 32, 48, 64)):
+    """The generalized solve (FP64 and FP32+refine, workload pair of seed 1001 + n) and its band-dependent stages
+    alone (dense -> band, chase, bulge back-transform of m = n columns, each precision; standard matrix of seed 11)
+    for each band width, HIP events on the handle's stream, median of 3 after a warm-up. One JSON line per (n, b)."""
+    stream = torch.cuda.ExternalStream(hnd.stream_ptr())
+
+    def ev_time(fn, reps=3):
+        ts = []
+        for r in range(reps + 1):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            st = hnd.synchronize()
+            if st != [0, 0, 0, 0]:
+                raise RuntimeError(f"status {st}")
+            if r:
+                ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    with torch.cuda.stream(stream):
+        for n in sizes:
+            h, s = workloads.generalized_pair(n, 1001 + n)
+            hd = torch.from_numpy(np.ascontiguousarray(h.T)).cuda()
+            sd = torch.from_numpy(np.ascontiguousarray(s.T)).cuda()
+            w = torch.zeros(n, dtype=torch.float64, device="cuda")
+            c = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+            a, _ = workloads.standard_matrix(n, 11)
+            for b in bands:
+                row = {"n": n, "band": b}
+                hnd.set_tuning(b, 2)
+                for name, prec in (("sygvd_fp64_ms", xtbeig.FP64), ("sygvd_fp32_refine_ms", xtbeig.FP32_REFINE)):
+                    row[name] = round(ev_time(lambda: hnd.sygvd(hd, sd, w, c, prec)), 2)
+                for dt, tag in ((torch.float64, "f64"), (torch.float32, "f32")):
+                    A0 = torch.from_numpy(np.ascontiguousarray(a.T)).cuda().to(dt)
+                    A = A0.clone()
+                    V1 = torch.zeros((n, n), dtype=dt, device="cuda")
+                    T1 = torch.zeros((n, b), dtype=dt, device="cuda")
+                    AB = torch.zeros(xtbeig.band_elems(n, b), dtype=dt, device="cuda")
+                    row[f"dense_to_band_{tag}_ms"] = round(
+                        ev_time(lambda: (A.copy_(A0), hnd.reduce_to_band(A, b, V1, T1, AB))), 2)
+                    AB0 = AB.clone()
+                    V2 = torch.zeros((n, n), dtype=dt, device="cuda")
+                    D = torch.zeros(n, dtype=dt, device="cuda")
+                    E = torch.zeros(n, dtype=dt, device="cuda")
+                    row[f"chase_{tag}_ms"] = round(ev_time(lambda: (AB.copy_(AB0), hnd.chase(n, b, AB, V2, D, E))), 2)
+                    row[f"chase_{tag}_band_in_lds"] = bool(xtbeig.lib().xtbeig_chase_in_lds(n, b, int(dt == torch.float32)))
+                    Z = torch.eye(n, dtype=dt, device="cuda")
+                    row[f"bulge_back_{tag}_ms"] = round(ev_time(lambda: hnd.back_bulges(n, b, V2, Z, n)), 2)
+                print(json.dumps(row), flush=True)
+            hnd.set_tuning(0, 2)
 
 
 def rocprof_summary(path, top=40):
@@ -482,6 +568,9 @@ def main():
     ap.add_argument("--kernels", default=None, metavar="GROUPS",
                     help="run the hot kernels alone a few times each (for rocprofv3 counter runs) and exit; GROUPS "
                          "is a comma-separated subset of gemm,band,chase,solve,sytrd")
+    ap.add_argument("--bands", action="store_true",
+                    help="time the solve and its band-dependent stages for band widths 8 to 64 at n = 1204 and 1788 "
+                         "(one JSON line each) and exit")
     ap.add_argument("--rocprof-summary", default=None, metavar="DIR",
                     help="print markdown tables of a rocprofv3 csv output directory and exit (no GPU use)")
     args = ap.parse_args()
@@ -516,6 +605,9 @@ def main():
     if args.kernels:
         kernels(torch, np, workloads, xtbeig, hnd, h, s, set(args.kernels.split(",")))
         return
+    if args.bands:
+        band_sweep(torch, np, workloads, xtbeig, hnd)
+        return
     band = xtbeig.default_band(N, fp32=True)
 
     res = {}
@@ -534,6 +626,10 @@ def main():
     unrefined = workloads.gates(h, s, w.cpu().numpy(), c.cpu().numpy().T, wref)
     hnd.set_tuning(0, 2)
     log("fp32 before refinement", unrefined)
+
+    log("concurrent throughput ...")
+    thru = concurrent_throughput(torch, np, workloads, xtbeig, hnd, h, s, wref)
+    thru["ratio_fp32_refine_vs_fp64"] = thru["fp32_refine"]["solves_per_s"] / thru["fp64"]["solves_per_s"]
 
     log("kernel roofline ...")
     roof = kernel_roofline(torch, xtbeig, hnd, h, s, band)
@@ -589,6 +685,7 @@ def main():
                         "gates": mixed["gates"]},
         "fp32_before_refinement": {"gates": unrefined, "pass": workloads.passes(unrefined)},
         "speedup_fp32_refine_vs_fp64": fp64["ms"] / mixed["ms"],
+        "throughput_concurrent": thru,
         "nominal_flop_rate_TFLOPs": {"fp64": flops / (fp64["ms"] * 1e-3) / 1e12,
                                      "fp32_refine": flops / (mixed["ms"] * 1e-3) / 1e12},
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

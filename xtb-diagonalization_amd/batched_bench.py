@@ -23,8 +23,6 @@ def main():
     ap.add_argument("--reps", type=int, default=2, help="timed passes (the first pass after the warm-up pass)")
     ap.add_argument("--check", type=int, default=8)
     args = ap.parse_args()
-    # one hardware queue per slot (HIP's default is 4 per process)
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(args.slots, 16))))
     import numpy as np
     import scipy.linalg as sl
     import torch

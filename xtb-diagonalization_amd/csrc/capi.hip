@@ -1021,6 +1021,11 @@ int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, c
   });
 }
 
+int xtbeig_chase_in_lds(int n, int b, int fp32) {
+  if (n < 1 || b < 1 || b > xtbeig::MAX_BAND) return 0;
+  return xtbeig::chase_in_lds(n, b, fp32 != 0) ? 1 : 0;
+}
+
 int xtbeig_set_gemm_tile(int width) {
   if (width != 0 && width != 64 && width != 128) return set_error("xtbeig_set_gemm_tile: width must be 0, 64 or 128");
   xtbeig::set_gemm_tile(width);

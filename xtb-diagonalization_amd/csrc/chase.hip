@@ -554,12 +554,18 @@ void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* st
 
 bool chase_in_lds(int n, int b, bool fp32) {
   constexpr int WAVES = 8;
-  size_t need;
-  if (b == 8) need = fp32 ? chase_lanes_lds<8, 2, true, float>(n, WAVES) : chase_lanes_lds<8, 2, true, double>(n, WAVES);
-  else if (b == 16)
-    need = fp32 ? chase_lanes_lds<16, 2, true, float>(n, WAVES) : chase_lanes_lds<16, 2, true, double>(n, WAVES);
-  else return false;
-  return need + sizeof(int) * WAVES * (64 / (2 * b)) <= 160 * 1024;  // as launch_chase_lanes decides
+  if (n > 2 && (b == 8 || b == 16)) {
+    size_t need;
+    if (b == 8)
+      need = fp32 ? chase_lanes_lds<8, 2, true, float>(n, WAVES) : chase_lanes_lds<8, 2, true, double>(n, WAVES);
+    else
+      need = fp32 ? chase_lanes_lds<16, 2, true, float>(n, WAVES) : chase_lanes_lds<16, 2, true, double>(n, WAVES);
+    return need + sizeof(int) * WAVES * (64 / (2 * b)) <= 160 * 1024;  // as launch_chase_lanes decides
+  }
+  // as launch_chase decides
+  const size_t el = fp32 ? sizeof(float) : sizeof(double);
+  const int waves = (b > 16 && b <= 32) ? 8 : 16;
+  return 2 * el * waves * (size_t)b + el * (size_t)band_elems(n, b) + sizeof(int) * waves <= 160 * 1024;
 }
 
 template void chase_bulges<double>(hipStream_t, int, int, double*, double*, double*, double*, int*);

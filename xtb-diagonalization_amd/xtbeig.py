@@ -99,6 +99,7 @@ SIGNATURES = {
     "xtbeig_refine_check": (ci, [vp, c_dbl_p]),
     "xtbeig_last_error": (ctypes.c_char_p, []),
     "xtbeig_set_gemm_tile": (ci, [ci]),
+    "xtbeig_chase_in_lds": (ci, [ci, ci, ci]),
     "xtbeig_gemm_tile_width": (ci, [ci, ci, ci, ci]),
     "xtbeig_gemm_tile_buckets": (ci, []),
     "xtbeig_get_gemm_tile_table": (ci, [ci, c_int_p, c_int_p]),
