@@ -134,6 +134,9 @@ int xtbeig_set_test_limits(xtbeig_handle_t handle, int secular_maxit);
 /* The built-in band width for an n x n solve (fp32 nonzero: the FP32 reduction of the mixed path). No reference
  * counterpart beyond SBROptions.recommended (SBROptions.swift:73-82), which picks b per n the same way. */
 int xtbeig_default_band(int n, int fp32);
+/* 1 when the bulge chase of an n x n band of width b keeps the band in one CU's LDS (else it works on the band in
+ * device memory, several times slower), 0 otherwise or for invalid sizes. No reference counterpart. */
+int xtbeig_chase_in_lds(int n, int b, int fp32);
 /* Wait for the handle's stream; status[0..3] = Cholesky info, bulge-chase stall flag, divide-and-conquer info, and
  * the refinement check of an FP32 + refinement solve (nonzero: the refined eigenpairs failed it, or a cluster's
  * Rayleigh-Ritz step met a Gram block that is not positive definite) (status: 4 ints, may be NULL). Returns 0 when all
@@ -234,10 +237,11 @@ int xtbeig_dev_sormtr_upper(xtbeig_handle_t handle, int n, int m, const float *a
 int xtbeig_dev_sygvd_graph(xtbeig_handle_t handle, int precision, int n, const double *h, int ldh, const double *s,
                            int lds, double *w, double *c, int ldc);
 /* count independent generalized solves of uneven n (arrays of count device pointers and sizes, in host memory) on up
- * to `slots` concurrent streams of sub-handles, longest first onto the least-loaded slot; use_graph: each solve
- * through the slot's graph cache. Waits for completion; latency_ms (count, may be NULL) receives each solve's time
- * on its stream. Status words of the slots are merged into the handle's (xtbeig_synchronize). For batches of
- * independent problems (256 of n in [124, 1204], say); no reference counterpart. */
+ * to `slots` concurrent streams of sub-handles, longest first onto the least-loaded slot; every workspace and plan
+ * of an eager batch is set up before the first solve is enqueued. use_graph: each solve through the slot's graph
+ * cache. Waits for completion; latency_ms (count, may be NULL) receives each solve's time on its stream. Status words
+ * of the slots are merged into the handle's (xtbeig_synchronize). For batches of independent problems (256 of n in
+ * [124, 1204], say); no reference counterpart. */
 int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, const int *n, const double *const *h,
                              const int *ldh, const double *const *s, const int *lds, double *const *w,
                              double *const *c, const int *ldc, int slots, int use_graph, float *latency_ms);

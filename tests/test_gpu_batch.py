@@ -101,6 +101,6 @@ def test_autotune_table(torch_cuda):
     other.set_band_table(xtbeig.FP64, bands)
     assert other.band_table(xtbeig.FP64)[1] == bands
     with pytest.raises(RuntimeError):
-        other.set_band_table(xtbeig.FP64, [64] * len(bands))
+        other.set_band_table(xtbeig.FP64, [65] * len(bands))
     other.close()
     hnd.close()

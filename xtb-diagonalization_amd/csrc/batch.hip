@@ -12,7 +12,8 @@
 // factorizations on few), so a batch runs up to `slots` solves at once on sub-handles with their own streams and
 // arenas. Problems are assigned longest first (cost ~ n^3) to the least-loaded slot; the slots wait for the caller's
 // stream before starting and the caller's stream waits for all slots at the end. HIP events around every solve give
-// the per-solve latency.
+// the per-solve latency. For eager solves the arenas and plans of every problem are set up before anything is
+// enqueued; with use_graph each solve goes through its slot's graph cache.
 //
 // Autotuner. For each n bucket of the tuned table within [nmin, nmax] the candidate band widths are timed on a
 // synthetic symmetric pair of the bucket's geometric-middle size (two timed solves after a warm-up, the minimum
@@ -135,6 +136,22 @@ void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, 
     const int sl = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     lists[sl].push_back(q);
     load[sl] += std::pow((double)n[q], 3.0);
+  }
+  // Everything an eager solve would allocate, up front: each slot's arena for its largest problem and the D&C and
+  // back-transformation plans of every size it will see, so that the enqueue loop below never waits on the host.
+  if (!use_graph) {
+    for (int sl = 0; sl < slots; ++sl) {
+      Handle& sub = *h.subs[sl];
+      size_t need = 0;
+      for (int q : lists[sl]) {
+        if (n[q] <= 1) continue;
+        const int bq = std::max(1, std::min(sub.band_for(n[q], mixed), n[q] - 1));
+        need = std::max(need, sygvd_bytes(n[q], bq, mixed, n[q]));
+        (void)sub.plan(n[q]);
+        (void)sub.backplan(n[q], bq);
+      }
+      if (need) sub.reserve(need);
+    }
   }
   hipEvent_t ready;
   XTB_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));

@@ -114,6 +114,8 @@ void syevd_device(Handle& h, int n, const T* a, int lda, double* w, T* z, int ld
 // until the next call on the handle.
 void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
                   int ldc, bool mixed, int b, const double** u_out = nullptr);
+// arena bytes of a generalized solve (m eigenvector columns)
+size_t sygvd_bytes(int n, int b, bool mixed, int m);
 void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz);
 // The cycle's generalized solve on the handle's geometry (set_geometry): all n eigenvalues, the first m eigenvectors.
 void sygvd_geometry(Handle& h, int n, int m, const double* hm, int ldh, double* w, double* cm, int ldc, bool mixed,
