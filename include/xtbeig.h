@@ -89,7 +89,7 @@ void xtbeig_ssyevd(const char *jobz, const char *uplo, const int *n, float *a, c
 /* dsygvd(itype = 1, jobz, uplo, ...): A x = lambda B x with B symmetric positive definite. On exit a holds the
  * B-orthonormal eigenvectors ('V') and b the Cholesky factor U (B = U^T U, upper) as LAPACK leaves it. info > n:
  * B's leading minor of order info - n is not positive definite. The xTB fact_solve generalized solve (reference
- * spec: scripts/kernels/eigbench.f90:101-168). Sizes up to n = 2048 + band width (the built-in band is 8 or 16);
+ * spec: scripts/kernels/eigbench.f90:101-168). Sizes up to n = 2048 + band width (the built-in band is 8);
  * a larger n returns info = 2 n + 1 with the reason in xtbeig_last_error(), before any work is done. */
 void xtbeig_dsygvd(const int *itype, const char *jobz, const char *uplo, const int *n, double *a, const int *lda,
                    double *b, const int *ldb, double *w, double *work, const int *lwork, int *iwork,

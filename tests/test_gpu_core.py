@@ -119,7 +119,9 @@ def test_reduce_to_band(handle, torch_cuda, prec, b, n):
                                       # the bulge back-transform's LDS strip with 2 wavefronts, and past its LDS
                                       ("d", 16, 700), ("d", 16, 1600),
                                       # band widths past 32: the scalar chase and the 64-wide bulge window
-                                      ("d", 64, 300), ("s", 48, 250), ("d", 48, 600), ("s", 64, 65)])
+                                      ("d", 64, 300), ("s", 48, 250), ("d", 48, 600), ("s", 64, 65),
+                                      # the lane chase with its band split between device memory and LDS
+                                      ("d", 8, 1300), ("d", 8, 2000), ("s", 16, 1300)])
 def test_chase(handle, torch_cuda, prec, b, n):
     torch = torch_cuda
     rng = np.random.default_rng(5)

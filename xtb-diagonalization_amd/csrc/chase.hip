@@ -358,27 +358,127 @@ __device__ __forceinline__ bool chase_bulge_lanes(int n, T* __restrict__ ab, T* 
   return true;
 }
 
+// Band element access by flat index f = i + (2b - 1) j (column j in flat [2b j, 2b j + 2b)) for a band split at a
+// column boundary fs between device memory (f < fs) and an LDS copy of the rest.
+template <typename T>
+struct PtrBand {  // the whole band behind one pointer (device memory or LDS, so each access keeps its address space)
+  T* p;
+  __device__ inline T& at(int f) const { return p[f]; }
+};
+template <typename T>
+struct SplitBand {
+  T* g;
+  T* l;  // LDS element f - fs
+  int fs;
+  __device__ inline T& at(int f) const { return f < fs ? g[f] : l[f - fs]; }
+};
+
+// The same bulge through an accessor, for bulges whose columns straddle the split between device memory and LDS
+// (pointer offsets from one base would cross the two address spaces).
+template <typename T, int B, int H, typename Acc>
+__device__ __forceinline__ bool chase_bulge_lanes_acc(int n, const Acc& ab, T* __restrict__ v2s, int rs, bool first,
+                                                      int cl, int h, const LaneOffsets& o, T* __restrict__ qs) {
+  constexpr int U = B / H;
+  constexpr int LD = 2 * B - 1;
+  if (rs >= n) return false;
+  // left block: columns rs - B ..< rs; for the first bulge of a sweep only column s is nonzero (the columns before it
+  // are already tridiagonal), and every lane reads that column
+  const int u0 = h * U;
+  const int pos = rs * (2 * B);               // A(rs, rs)
+  const int xp = pos + (first ? o.x0 : o.x1);  // x = A[R, c] at xp + u
+  const int lp = pos + (first ? o.l0 : o.l1);  // left block column at lp + j
+  const int bp = pos + o.bl;                   // block-below row re + cl, columns rs + u, at bp + j LD
+  const int dr = pos + o.dr;                   // diagonal block row rs + cl, columns rs + u <= rs + cl, at dr + j LD
+  const int dc = pos + o.dc;                   // diagonal block column rs + cl, rows rs + u >= rs + cl, at dc + j
+  T L[U], Bl[U], D[U], w[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    w[j] = ab.at(xp + u0 + j);
+    L[j] = ab.at(lp + j);
+    Bl[j] = ab.at(bp + j * LD);
+    // both reads stay inside the band storage (for u > cl, dr + j LD is another stored element)
+    const T r = ab.at(dr + j * LD), q = ab.at(dc + j);
+    D[j] = (u0 + j <= cl) ? r : q;
+  }
+  const T alpha = ab.at(xp);
+  const T xc = ab.at(xp + cl);  // this lane's element of x, for its own w
+  T xn2 = (u0 == 0) ? T(0) : w[0] * w[0];
+#pragma unroll
+  for (int j = 1; j < U; ++j) xn2 += w[j] * w[j];
+  if (H > 1) xn2 = part_sum<B, H>(xn2);
+  T beta, sq, s2;
+  larfg_scalars(alpha, xn2, beta, sq, s2);  // tau = 0 gives w = 0: every update below leaves its block unchanged
+#pragma unroll
+  for (int j = 0; j < U; ++j) w[j] = (j == 0 && u0 == 0) ? sq : s2 * w[j];
+  const T wc = (cl == 0) ? sq : s2 * xc;
+  if (h == 0 && rs + cl < n) v2s[rs + cl] = wc;
+  T dl = 0, y = 0, p = 0;
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    dl += w[j] * L[j];  // left block: w^T A[R, lcol]
+    y += Bl[j] * w[j];  // block below: A[re + cl, R] w
+    p += D[j] * w[j];   // diagonal block: (A_RR w)[cl]
+  }
+  if (H > 1) {
+    dl = part_sum<B, H>(dl);
+    y = part_sum<B, H>(y);
+    p = part_sum<B, H>(p);
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    L[j] -= w[j] * dl;
+    Bl[j] -= y * w[j];
+  }
+  const T half = group_sum_lanes<B>(wc * p) * T(0.5);
+  const T qc = p - half * wc;
+  qs[cl] = qc;  // every part stores the same value
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int j = 0; j < U; ++j) D[j] -= wc * qs[u0 + j] + qc * w[j];
+  __builtin_amdgcn_wave_barrier();
+  if (!first || cl == 0) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) ab.at(lp + j) = (cl == 0) ? ((j == 0 && u0 == 0) ? beta : T(0)) : L[j];
+  }
+#pragma unroll
+  for (int j = 0; j < U; ++j) {
+    ab.at(bp + j * LD) = Bl[j];
+    if (u0 + j <= cl) ab.at(dr + j * LD) = D[j];  // the lower triangle of the diagonal block, one lane per row
+  }
+  __threadfence_block();
+  return true;
+}
+
 // Band storage positions outside the matrix (rows >= n in the last columns' slots, the b extra columns) -> 0.
-template <typename T, int B>
-__device__ inline void clear_band_padding(int n, T* ab, int64_t nel) {
+template <typename T, int B, typename Acc>
+__device__ inline void clear_band_padding(int n, const Acc& ab, int64_t nel) {
   const int64_t first = (int64_t)2 * B * max(0, n - 2 * B);
   for (int64_t q = first + threadIdx.x; q < nel; q += blockDim.x) {
     const int64_t j = q / (2 * B), i = j + q % (2 * B);
-    if (i >= n) ab[q] = T(0);
+    if (i >= n) ab.at((int)q) = T(0);
   }
 }
 
-template <int B, int H, bool LDS, typename T>
-inline size_t chase_lanes_lds(int n, int waves) {
+// LDS bytes of the lane chase: the q slots, and the band from flat element fs on (fs = 0: all of it; none for a band
+// left in device memory).
+template <int B, int H, typename T>
+inline size_t chase_lanes_lds(int n, int waves, bool band_in_lds, int64_t fs = 0) {
   const size_t groups = (size_t)waves * (64 / (H * B));
-  return sizeof(T) * (groups * B + (LDS ? (size_t)band_elems(n, B) : 0));
+  return sizeof(T) * (groups * B + (band_in_lds ? (size_t)(band_elems(n, B) - fs) : 0));
 }
 
-// dynamic LDS: the band (when LDS-resident) first, then one q slot of B elements per group
-template <typename T, int B, int H, bool LDS, int WAVES>
+// Where the band lives during the lane chase: all in device memory, all in LDS, or split at flat element fs (a
+// column boundary) with the columns past it in LDS. The split keeps the part of the band where most bulge steps
+// happen on chip: position p is passed by p sweeps, so with the last columns in LDS the first (n - jsplit)^2 / n^2 of
+// the steps never leave the CU.
+enum { BAND_DEVICE = 0, BAND_LDS = 1, BAND_SPLIT = 2 };
+
+// dynamic LDS: the band's on-chip part (BAND_LDS: all of it; BAND_SPLIT: flat elements fs ..< nel) first, then one
+// q slot of B elements per group
+template <typename T, int B, int H, int MODE, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __restrict__ ab_global, T* __restrict__ v2,
                                                                  int ldv2, T* __restrict__ d, T* __restrict__ e,
-                                                                 int* __restrict__ status) {
+                                                                 int* __restrict__ status, int fs) {
   constexpr int GS = H * B;         // lanes per group
   constexpr int SPW = 64 / GS;      // sweeps per wavefront
   constexpr int NG = WAVES * SPW;   // sweep groups: group g takes sweeps g, g + NG, ...
@@ -387,22 +487,24 @@ __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __res
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int gl = lane % GS, cl = gl % B, h = gl / B, g = wid * SPW + lane / GS;
   const int64_t nel = band_elems(n, B);
+  if (MODE != BAND_SPLIT) fs = (MODE == BAND_LDS) ? 0 : (int)nel;
+  const int64_t on_chip = nel - fs;  // band elements in LDS
   T* const sm = reinterpret_cast<T*>(smem_raw);
-  T* const qs = sm + (LDS ? nel : 0) + g * B;  // this group's q exchange slot
-  T* ab = ab_global;
-  if (LDS) {
-    for (int64_t q = t; q < nel; q += blockDim.x) sm[q] = ab_global[q];
-    ab = sm;
-    __syncthreads();
-  }
-  clear_band_padding<T, B>(n, ab, nel);
+  T* const qs = sm + on_chip + g * B;  // this group's q exchange slot
+  for (int64_t q = t; q < on_chip; q += blockDim.x) sm[q] = ab_global[fs + q];
+  __syncthreads();
+  const SplitBand<T> sab{ab_global, sm, fs};  // any element (BAND_SPLIT only: a generic pointer per access)
+  T* const lab = sm - fs;                      // elements from fs on, on chip (lab[f], f >= fs)
+  if constexpr (MODE == BAND_SPLIT) clear_band_padding<T, B>(n, sab, nel);
+  else if constexpr (MODE == BAND_LDS) clear_band_padding<T, B>(n, PtrBand<T>{sm}, nel);
+  else clear_band_padding<T, B>(n, PtrBand<T>{ab_global}, nel);
   if (t < NG) prog[t] = -1;
   __syncthreads();
-  const Band<T> A{ab, band_ld(B)};
   const int sweeps = max(0, n - 2);
   const int KB = (n + B - 1) / B + 8;  // progress code: s * KB + bulges done (monotonic per group)
   const int pg = (g + NG - 1) % NG;    // the group of sweep s - 1
   const LaneOffsets off = lane_offsets<B, H>(cl, h);
+  const int js = fs / (2 * B);         // first column on chip
   bool failed = false;
   // per sweep: s, its first row rs = s + 1 + k B, the progress code s KB + k this group publishes after bulge k - 1,
   // the code (s - 1) KB + k + 3 it waits for before bulge k, and v2's column s
@@ -414,7 +516,15 @@ __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __res
     if (ready && s > 0) ready = ld_acquire(&prog[pg]) >= wait;
     if (ready) {
       spins = 0;
-      if (chase_bulge_lanes<T, B, H>(n, ab, v2s, rs, k == 0, cl, h, off, qs)) {
+      // a bulge at rs touches columns rs - B ..< rs + B: on chip, in device memory, or across the split
+      bool more;
+      if (MODE == BAND_LDS || (MODE == BAND_SPLIT && rs - B >= js))
+        more = chase_bulge_lanes<T, B, H>(n, lab, v2s, rs, k == 0, cl, h, off, qs);
+      else if (MODE == BAND_DEVICE || rs + B <= js)
+        more = chase_bulge_lanes<T, B, H>(n, ab_global, v2s, rs, k == 0, cl, h, off, qs);
+      else
+        more = chase_bulge_lanes_acc<T, B, H>(n, sab, v2s, rs, k == 0, cl, h, off, qs);
+      if (more) {
         ++k;
         ++code;
         ++wait;
@@ -437,14 +547,17 @@ __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __res
   }
   if (failed && gl == 0) atomicExch(status, 1);
   __syncthreads();
-  for (int i = t; i < n; i += blockDim.x) {
-    d[i] = A(i, i);
-    if (i + 1 < n) e[i] = A(i + 1, i);
-  }
-  if (LDS) {
-    __syncthreads();
-    for (int64_t q = t; q < nel; q += blockDim.x) ab_global[q] = ab[q];
-  }
+  auto diag = [&](const auto& band) {
+    for (int i = t; i < n; i += blockDim.x) {
+      d[i] = band.at((2 * B) * i);
+      if (i + 1 < n) e[i] = band.at((2 * B) * i + 1);
+    }
+  };
+  if constexpr (MODE == BAND_SPLIT) diag(sab);
+  else if constexpr (MODE == BAND_LDS) diag(PtrBand<T>{sm});
+  else diag(PtrBand<T>{ab_global});
+  __syncthreads();
+  for (int64_t q = t; q < on_chip; q += blockDim.x) ab_global[fs + q] = sm[q];
 }
 
 template <typename T, int B, bool LDS, int CHASE_WAVES>
@@ -527,15 +640,28 @@ void launch_chase(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* st
 template <typename T, int B, int H, int WAVES = 8>
 void launch_chase_lanes(hipStream_t s, int n, T* ab, T* v2, T* d, T* e, int* status) {
   constexpr int NG = WAVES * (64 / (H * B));
-  const size_t lds = chase_lanes_lds<B, H, true, T>(n, WAVES);
-  if (lds + sizeof(int) * NG <= 160 * 1024) {
-    allow_lds(chase_lanes_kernel<T, B, H, true, WAVES>, lds);
-    hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, true, WAVES>), dim3(1), dim3(64 * WAVES), lds, s, n, ab, v2, n, d,
-                       e, status);
+  constexpr size_t LDS_MAX = 160 * 1024 - sizeof(int) * NG;  // dynamic LDS beside the progress flags
+  const int64_t nel = band_elems(n, B);
+  const size_t all = chase_lanes_lds<B, H, T>(n, WAVES, true);
+  if (all <= LDS_MAX) {
+    allow_lds(chase_lanes_kernel<T, B, H, BAND_LDS, WAVES>, all);
+    hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_LDS, WAVES>), dim3(1), dim3(64 * WAVES), all, s, n, ab, v2,
+                       n, d, e, status, 0);
   } else {
-    const size_t qlds = chase_lanes_lds<B, H, false, T>(n, WAVES);
-    hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, false, WAVES>), dim3(1), dim3(64 * WAVES), qlds, s, n, ab, v2, n,
-                       d, e, status);
+    // the last columns on chip: fs, a column boundary, leaves at most LDS_MAX bytes for them and the q slots
+    const size_t qbytes = chase_lanes_lds<B, H, T>(n, WAVES, false);
+    const int64_t cap = (int64_t)((LDS_MAX - qbytes) / sizeof(T));
+    int64_t fs = nel - cap;
+    fs = (fs + 2 * B - 1) / (2 * B) * (2 * B);
+    if (fs < nel - 4 * B * (2 * B)) {  // worth it only with a few columns on chip
+      const size_t lds = chase_lanes_lds<B, H, T>(n, WAVES, true, fs);
+      allow_lds(chase_lanes_kernel<T, B, H, BAND_SPLIT, WAVES>, lds);
+      hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_SPLIT, WAVES>), dim3(1), dim3(64 * WAVES), lds, s, n, ab,
+                         v2, n, d, e, status, (int)fs);
+    } else {
+      hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_DEVICE, WAVES>), dim3(1), dim3(64 * WAVES), qbytes, s, n,
+                         ab, v2, n, d, e, status, (int)nel);
+    }
   }
   XTB_LAUNCH_CHECK();
 }
@@ -557,9 +683,9 @@ bool chase_in_lds(int n, int b, bool fp32) {
   if (n > 2 && (b == 8 || b == 16)) {
     size_t need;
     if (b == 8)
-      need = fp32 ? chase_lanes_lds<8, 2, true, float>(n, WAVES) : chase_lanes_lds<8, 2, true, double>(n, WAVES);
+      need = fp32 ? chase_lanes_lds<8, 2, float>(n, WAVES, true) : chase_lanes_lds<8, 2, double>(n, WAVES, true);
     else
-      need = fp32 ? chase_lanes_lds<16, 2, true, float>(n, WAVES) : chase_lanes_lds<16, 2, true, double>(n, WAVES);
+      need = fp32 ? chase_lanes_lds<16, 2, float>(n, WAVES, true) : chase_lanes_lds<16, 2, double>(n, WAVES, true);
     return need + sizeof(int) * WAVES * (64 / (2 * b)) <= 160 * 1024;  // as launch_chase_lanes decides
   }
   // as launch_chase decides

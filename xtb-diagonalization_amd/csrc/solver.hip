@@ -17,13 +17,14 @@
 
 namespace xtbeig {
 
-// Band width per n: 8 while the chase of that band is LDS-resident (FP64 up to n = 1255, FP32 up to about 2500), else
-// 16. At n = 1204 the band of 8 is the faster in both precisions: the chase is cheaper per sweep, and the twice as
-// many dense-to-band panels cost less than that saves.
+// Band width per n: 8 in both precisions. The chase of a band of 8 is the cheaper per sweep whether its band fits one
+// CU's LDS (FP64 up to n = 1255, FP32 up to about 2500) or not (then its last columns stay on chip): at n = 1788 in
+// FP64, 14.7 ms at band 8 against 31.3 ms at band 16 with both bands in device memory; and the twice as many
+// dense-to-band panels cost less than that saves.
 int default_band(int n, bool fp32) {
+  (void)fp32;
   if (n <= 3) return 1;
-  const int b = chase_in_lds(n, 8, fp32) ? 8 : 16;
-  return std::min(b, std::max(2, n - 1));
+  return std::min(8, std::max(2, n - 1));
 }
 
 Handle::Handle(int dev) : device(dev) {
