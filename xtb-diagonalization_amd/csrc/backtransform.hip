@@ -44,7 +44,9 @@ __global__ __launch_bounds__(64 * BULGE_WAVES) void bulge_apply_kernel(int n, in
                                                                        const int* __restrict__ stage_off,
                                                                        const int* __restrict__ order, int nstages,
                                                                        const T* __restrict__ v2, int ldv2,
-                                                                       T* __restrict__ zt, int ldzt) {
+                                                                       T* __restrict__ zt, int ldzt, Zb zb) {
+  v2 = zb.at(v2);
+  zt = zb.at(zt);
   constexpr int H = BULGE_G + B - 1;
   constexpr int WE = BULGE_G * B;  // reflector entries of one block
   // reflector j at ws[j WS ..< j WS + B]; the gap keeps the compiler from merging reads across reflectors
@@ -120,7 +122,9 @@ __global__ __launch_bounds__(64 * W) void bulge_strip_kernel(int n, int b, int r
                                                              const BulgeBlock* __restrict__ sorted,
                                                              const int* __restrict__ stage_off,
                                                              const T* __restrict__ v2, int ldv2,
-                                                             T* __restrict__ zt, int ldzt) {
+                                                             T* __restrict__ zt, int ldzt, Zb zb) {
+  v2 = zb.at(v2);
+  zt = zb.at(zt);
   constexpr int G = BULGE_G, H = G + B - 1, WE = G * B;
   constexpr int GPW = 64 / C, SLOTS = W * GPW;
   constexpr int SS = WE + 16 / (int)sizeof(T);  // slot stride: the slots of one wavefront start 4 banks apart
@@ -255,7 +259,10 @@ __global__ __launch_bounds__(64 * W) void bulge_strip_kernel(int n, int b, int r
 // the group's K columns).
 template <typename T>
 __global__ __launch_bounds__(256) void group_t_kernel(int n, int b, int ppg, int last, const T* __restrict__ v1,
-                                                      const T* __restrict__ t1, T* __restrict__ tg) {
+                                                      const T* __restrict__ t1, T* __restrict__ tg, Zb zb) {
+  v1 = zb.at(v1);
+  t1 = zb.at(t1);
+  tg = zb.at(tg);
   extern __shared__ char smem_raw[];  // three GROUP_K x (GROUP_K + 1) arrays
   using Row = T[GROUP_K + 1];
   Row* G = reinterpret_cast<Row*>(smem_raw);
@@ -335,7 +342,10 @@ __global__ __launch_bounds__(256) void group_t_kernel(int n, int b, int ppg, int
 constexpr int COMBINE_ROWS = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void group_combine_kernel(int m, int K, int splits, const T* __restrict__ P,
-                                                            const T* __restrict__ tg, T* __restrict__ w2) {
+                                                            const T* __restrict__ tg, T* __restrict__ w2, Zb zb) {
+  P = zb.at(P);
+  tg = zb.at(tg);
+  w2 = zb.at(w2);
   __shared__ T Ts[GROUP_K][GROUP_K + 1];
   __shared__ T Ws[COMBINE_ROWS][GROUP_K + 1];
   const int i0 = blockIdx.x * COMBINE_ROWS, t = threadIdx.x;
@@ -371,8 +381,8 @@ bool launch_strip(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldzt
   const size_t lds = sizeof(T) * ((size_t)W * (64 / C) * SS + (size_t)C * rs) + sizeof(int) * (p.nstages + 1);
   if (lds > LDS_BYTES) return false;
   allow_lds(bulge_strip_kernel<T, B, REV, C, W>, lds);
-  hipLaunchKernelGGL((bulge_strip_kernel<T, B, REV, C, W>), dim3(cdiv(m, C)), dim3(64 * W), lds, s, p.n, p.b, p.rows,
-                     rs, p.nstages, p.sorted_dev, p.stage_off_dev, v2, p.n, zt, ldzt);
+  hipLaunchKernelGGL((bulge_strip_kernel<T, B, REV, C, W>), dim3(cdiv(m, C), 1, zgrid()), dim3(64 * W), lds, s, p.n, p.b,
+                     p.rows, rs, p.nstages, p.sorted_dev, p.stage_off_dev, v2, p.n, zt, ldzt, zb());
   XTB_LAUNCH_CHECK();
   return true;
 }
@@ -391,11 +401,11 @@ void launch_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldz
   if (transposed ? launch_strips<T, B, true>(s, p, v2, zt, ldzt, m) : launch_strips<T, B, false>(s, p, v2, zt, ldzt, m))
     return;
   if (transposed)
-    hipLaunchKernelGGL((bulge_apply_kernel<T, B, true>), dim3(cdiv(m, 64)), dim3(64 * BULGE_WAVES), 0, s, p.n, p.b, m,
-                       p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt);
+    hipLaunchKernelGGL((bulge_apply_kernel<T, B, true>), dim3(cdiv(m, 64), 1, zgrid()), dim3(64 * BULGE_WAVES), 0, s, p.n,
+                       p.b, m, p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt, zb());
   else
-    hipLaunchKernelGGL((bulge_apply_kernel<T, B, false>), dim3(cdiv(m, 64)), dim3(64 * BULGE_WAVES), 0, s, p.n, p.b,
-                       m, p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt);
+    hipLaunchKernelGGL((bulge_apply_kernel<T, B, false>), dim3(cdiv(m, 64), 1, zgrid()), dim3(64 * BULGE_WAVES), 0, s,
+                       p.n, p.b, m, p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt, zb());
   XTB_LAUNCH_CHECK();
 }
 
@@ -502,7 +512,8 @@ void back_transform_band(hipStream_t s, const BackPlan& p, const T* v1, const T*
   T* parts = w2 + (size_t)m * GROUP_K;
   const size_t lds = 3 * sizeof(T) * GROUP_K * (GROUP_K + 1);
   allow_lds(group_t_kernel<T>, lds);
-  hipLaunchKernelGGL(group_t_kernel<T>, dim3(p.ngroups), dim3(256), lds, s, n, b, p.ppg, p.last, v1, t1, tg);
+  hipLaunchKernelGGL(group_t_kernel<T>, dim3(p.ngroups, 1, zgrid()), dim3(256), lds, s, n, b, p.ppg, p.last, v1, t1,
+                     tg, zb());
   XTB_LAUNCH_CHECK();
   for (int g = p.ngroups - 1; g >= 0; --g) {
     const int j0 = g * p.ppg * b, r0 = j0 + b, h = n - r0;
@@ -511,8 +522,8 @@ void back_transform_band(hipStream_t s, const BackPlan& p, const T* v1, const T*
     const int K = (q - 1) * b + std::min(b, n - (j0 + (q - 1) * b + b));
     const T* V = v1 + (int64_t)r0 + (int64_t)j0 * n;
     const int splits = gemm_splitk<T>(s, m, K, h, zt + (int64_t)r0 * ldzt, ldzt, V, n, parts, SPLIT_KC);
-    hipLaunchKernelGGL(group_combine_kernel<T>, dim3(cdiv(m, COMBINE_ROWS)), dim3(256), 0, s, m, K, splits, parts,
-                       tg + (size_t)g * GROUP_K * GROUP_K, w2);
+    hipLaunchKernelGGL(group_combine_kernel<T>, dim3(cdiv(m, COMBINE_ROWS), 1, zgrid()), dim3(256), 0, s, m, K, splits,
+                       parts, tg + (size_t)g * GROUP_K * GROUP_K, w2, zb());
     XTB_LAUNCH_CHECK();
     gemm<T>(s, 'N', 'T', m, h, K, T(-1), w2, m, V, n, T(1), zt + (int64_t)r0 * ldzt, ldzt);
   }

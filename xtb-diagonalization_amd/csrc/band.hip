@@ -47,7 +47,11 @@ __device__ inline void block_sums(const T (&v)[NV], T* red, T* tot) {
 template <typename T, int SB, int R>
 __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* __restrict__ p, int lda,
                                                               T* __restrict__ v1, int ldv1, T* __restrict__ buf,
-                                                              T* __restrict__ tau) {
+                                                              T* __restrict__ tau, Zb zb) {
+  p = zb.at(p);
+  v1 = zb.at(v1);
+  buf = zb.at(buf);
+  tau = zb.at(tau);
   __shared__ T red[4 * SB];
   __shared__ T tot[SB];
   __shared__ T sh_alpha;
@@ -333,7 +337,11 @@ __device__ __forceinline__ void factor_panel(int m, T (&x)[R][NB], T* __restrict
 template <typename T, int NB, int R>
 __global__ __launch_bounds__(256) void panel_factor_kernel(int m, T* __restrict__ p, int lda, T* __restrict__ v1,
                                                            int ldv1, T* __restrict__ tau, T* __restrict__ tt,
-                                                           int ldt) {
+                                                           int ldt, Zb zb) {
+  p = zb.at(p);
+  v1 = zb.at(v1);
+  tau = zb.at(tau);
+  tt = zb.at(tt);
   const int t = threadIdx.x;
   T x[R][NB];
 #pragma unroll
@@ -357,7 +365,11 @@ template <> struct XSplit<double> { static constexpr int K = 64; };
 
 template <typename T>
 __global__ __launch_bounds__(256) void panel_xpart_kernel(int m, int k, const T* __restrict__ a2, int lda,
-                                                          const T* __restrict__ v, int ldv, T* __restrict__ xpart) {
+                                                          const T* __restrict__ v, int ldv, T* __restrict__ xpart,
+                                                          Zb zb) {
+  a2 = zb.at(a2);
+  v = zb.at(v);
+  xpart = zb.at(xpart);
   using namespace gemm_detail;
   using acc_t = typename Mfma<T>::acc_t;
   constexpr int KC = XSplit<T>::K;
@@ -407,7 +419,12 @@ __global__ __launch_bounds__(256) void panel_xpart_kernel(int m, int k, const T*
 template <typename T>
 __global__ __launch_bounds__(256) void panel_xfin_kernel(int m, int k, const T* __restrict__ xpart, int nks,
                                                          const T* __restrict__ tt, int ldt, const T* __restrict__ v,
-                                                         int ldv, T* __restrict__ x, T* __restrict__ ypart) {
+                                                         int ldv, T* __restrict__ x, T* __restrict__ ypart, Zb zb) {
+  xpart = zb.at(xpart);
+  tt = zb.at(tt);
+  v = zb.at(v);
+  x = zb.at(x);
+  ypart = zb.at(ypart);
   __shared__ T Xs[64][17];
   __shared__ T Vr[64][17];
   __shared__ T Ts[16][17];
@@ -613,7 +630,12 @@ template <typename T>
 __global__ __launch_bounds__(256) void panel_update_kernel(int m, int k, T* __restrict__ a2, int lda,
                                                            const T* __restrict__ v, int ldv, const T* __restrict__ x,
                                                            const T* __restrict__ ypart, int nrb,
-                                                           const T* __restrict__ tt, int ldt) {
+                                                           const T* __restrict__ tt, int ldt, Zb zb) {
+  a2 = zb.at(a2);
+  v = zb.at(v);
+  x = zb.at(x);
+  ypart = zb.at(ypart);
+  tt = zb.at(tt);
   __shared__ UpdateLds<T> L;
   update_tile<T>(m, k, a2, lda, v, ldv, x, ypart, nrb, tt, ldt, blockIdx.x * 64, blockIdx.y * 64, 0, L);
 }
@@ -632,7 +654,15 @@ __global__ __launch_bounds__(256) void panel_update_factor_kernel(int m, T* __re
                                                                   const T* __restrict__ tt, int ldt,
                                                                   T* __restrict__ v1n, int ldv1,
                                                                   T* __restrict__ taun, T* __restrict__ ttn,
-                                                                  int ldtn) {
+                                                                  int ldtn, Zb zb) {
+  a2 = zb.at(a2);
+  v = zb.at(v);
+  x = zb.at(x);
+  ypart = zb.at(ypart);
+  tt = zb.at(tt);
+  v1n = zb.at(v1n);
+  taun = zb.at(taun);
+  ttn = zb.at(ttn);
   __shared__ UpdateLds<T> L;
   const int t = threadIdx.x;
   if (blockIdx.x > 0) {
@@ -765,7 +795,10 @@ __device__ inline void sum_partials(int k, int nblk, const T* __restrict__ part,
 
 // Gp[blk] = V(rows)^T V(rows), upper triangle.
 template <typename T>
-__global__ __launch_bounds__(256) void panel_gram_kernel(int m, int k, const T* __restrict__ v, T* __restrict__ gp) {
+__global__ __launch_bounds__(256) void panel_gram_kernel(int m, int k, const T* __restrict__ v, T* __restrict__ gp,
+                                                         Zb zb) {
+  v = zb.at(v);
+  gp = zb.at(gp);
   __shared__ T Vs[PR][KM1];
   stage_rows<T>(m, k, blockIdx.x * PR, v, Vs);
   __syncthreads();
@@ -777,7 +810,12 @@ __global__ __launch_bounds__(256) void panel_gram_kernel(int m, int k, const T* 
 template <typename T>
 __global__ __launch_bounds__(256) void panel_tvt_kernel(int m, int k, const T* __restrict__ v,
                                                         const T* __restrict__ tau, const T* __restrict__ gp, int nblk,
-                                                        T* __restrict__ tt, int ldt, T* __restrict__ vt) {
+                                                        T* __restrict__ tt, int ldt, T* __restrict__ vt, Zb zb) {
+  v = zb.at(v);
+  tau = zb.at(tau);
+  gp = zb.at(gp);
+  tt = zb.at(tt);
+  vt = zb.at(vt);
   __shared__ T G[KM][KM1];
   __shared__ T Ts[KM][KM1];
   __shared__ T Vs[PR][KM1];
@@ -813,7 +851,11 @@ __global__ __launch_bounds__(256) void panel_tvt_kernel(int m, int k, const T* _
 template <typename T>
 __global__ __launch_bounds__(256) void panel_xy_kernel(int m, int k, const T* __restrict__ vt,
                                                        const T* __restrict__ xpart, int nsplit, T* __restrict__ x,
-                                                       T* __restrict__ yp) {
+                                                       T* __restrict__ yp, Zb zb) {
+  vt = zb.at(vt);
+  xpart = zb.at(xpart);
+  x = zb.at(x);
+  yp = zb.at(yp);
   __shared__ T Xs[PR][KM1];
   __shared__ T Ws[PR][KM1];
   const int t = threadIdx.x, i0 = blockIdx.x * PR;
@@ -836,7 +878,10 @@ __global__ __launch_bounds__(256) void panel_xy_kernel(int m, int k, const T* __
 // Y = sum of the Yp partials; W(rows) = X(rows) - V(rows) Y / 2, in place of x.
 template <typename T>
 __global__ __launch_bounds__(256) void panel_w_kernel(int m, int k, const T* __restrict__ v, const T* __restrict__ yp,
-                                                      int nblk, T* __restrict__ x) {
+                                                      int nblk, T* __restrict__ x, Zb zb) {
+  v = zb.at(v);
+  yp = zb.at(yp);
+  x = zb.at(x);
   __shared__ T Y[KM][KM1];
   __shared__ T Vs[PR][KM1];
   const int t = threadIdx.x, i0 = blockIdx.x * PR;
@@ -854,7 +899,9 @@ __global__ __launch_bounds__(256) void panel_w_kernel(int m, int k, const T* __r
 
 // Lower band of a (diagonals 0 .. b) into band storage ab, diagonals b + 1 .. 2b - 1 zero.
 template <typename T>
-__global__ void extract_band_kernel(int n, int b, const T* __restrict__ a, int lda, T* __restrict__ ab) {
+__global__ void extract_band_kernel(int n, int b, const T* __restrict__ a, int lda, T* __restrict__ ab, Zb zb) {
+  a = zb.at(a);
+  ab = zb.at(ab);
   const int j = blockIdx.x, ld = band_ld(b);
   for (int dd = threadIdx.x; dd <= ld; dd += blockDim.x) {
     const int i = j + dd;
@@ -864,7 +911,8 @@ __global__ void extract_band_kernel(int n, int b, const T* __restrict__ a, int l
 
 template <typename T, int SB, int R>
 void launch_qr(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, T* buf, T* tau) {
-  hipLaunchKernelGGL((panel_qr_kernel<T, SB, R>), dim3(1), dim3(QR_THREADS), 0, s, m, nb, p, lda, v1, ldv1, buf, tau);
+  hipLaunchKernelGGL((panel_qr_kernel<T, SB, R>), dim3(1, 1, zgrid()), dim3(QR_THREADS), 0, s, m, nb, p, lda, v1, ldv1,
+                     buf, tau, zb());
 }
 
 template <typename T>
@@ -899,7 +947,8 @@ size_t band_workspace(int n, int b) {
 
 template <typename T, int NB, int R>
 void launch_factor(hipStream_t s, int m, T* p, int lda, T* v1, int ldv1, T* tau, T* tt, int ldt) {
-  hipLaunchKernelGGL((panel_factor_kernel<T, NB, R>), dim3(1), dim3(256), 0, s, m, p, lda, v1, ldv1, tau, tt, ldt);
+  hipLaunchKernelGGL((panel_factor_kernel<T, NB, R>), dim3(1, 1, zgrid()), dim3(256), 0, s, m, p, lda, v1, ldv1, tau, tt,
+                     ldt, zb());
   XTB_LAUNCH_CHECK();
 }
 
@@ -947,8 +996,8 @@ bool panel_factor(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, 
 template <typename T, int NB, int R>
 void launch_update_factor(hipStream_t s, int m, T* a2, int lda, const T* v, int ldv, const T* x, const T* kpart,
                           int nrb, const T* tt, int ldt, T* v1n, int ldv1, T* tau, T* ttn) {
-  hipLaunchKernelGGL((panel_update_factor_kernel<T, NB, R>), dim3(1 + nrb * nrb), dim3(256), 0, s, m, a2, lda, v, ldv,
-                     x, kpart, nrb, tt, ldt, v1n, ldv1, tau, ttn, NB);
+  hipLaunchKernelGGL((panel_update_factor_kernel<T, NB, R>), dim3(1 + nrb * nrb, 1, zgrid()), dim3(256), 0, s, m, a2, lda,
+                     v, ldv, x, kpart, nrb, tt, ldt, v1n, ldv1, tau, ttn, NB, zb());
   XTB_LAUNCH_CHECK();
 }
 
@@ -1019,17 +1068,19 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
       // A2 -= V W^T + W V^T with Y and W formed per tile, with the next panel's factorization when it can ride along
       T* xb = buf;
       const int nks = cdiv(m, XSplit<T>::K), nrb = cdiv(m, 64);
-      hipLaunchKernelGGL(panel_xpart_kernel<T>, dim3(nrb, nks), dim3(256), 0, s, m, k, a2, lda, v, n, xpart);
+      hipLaunchKernelGGL(panel_xpart_kernel<T>, dim3(nrb, nks, zgrid()), dim3(256), 0, s, m, k, a2, lda, v, n,
+                         xpart, zb());
       XTB_LAUNCH_CHECK();
-      hipLaunchKernelGGL(panel_xfin_kernel<T>, dim3(nrb), dim3(256), 0, s, m, k, xpart, nks, tt, b, v, n, xb, kpart);
+      hipLaunchKernelGGL(panel_xfin_kernel<T>, dim3(nrb, 1, zgrid()), dim3(256), 0, s, m, k, xpart, nks, tt, b, v,
+                         n, xb, kpart, zb());
       XTB_LAUNCH_CHECK();
       const int jn = j + b;
       factored = jn < n - b - 1 && k == b &&
                  update_factor<T>(s, m, b, a2, lda, v, n, xb, kpart, nrb, tt, v1 + idx(jn + b, jn, n), n, tau,
                                   t1 + idx(0, jn, b));
       if (!factored) {
-        hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb, nrb), dim3(256), 0, s, m, k, a2, lda, v, n, xb, kpart,
-                           nrb, tt, b);
+        hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb, nrb, zgrid()), dim3(256), 0, s, m, k, a2, lda, v, n, xb,
+                           kpart, nrb, tt, b, zb());
         XTB_LAUNCH_CHECK();
       }
       continue;
@@ -1037,20 +1088,23 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
     factored = false;
     {
       launch_panel_qr<T>(s, m, b, panel, lda, v, n, buf, tau);
-      hipLaunchKernelGGL(panel_gram_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, kpart);  // V^T V parts
+      hipLaunchKernelGGL(panel_gram_kernel<T>, dim3(nblk, 1, zgrid()), dim3(256), 0, s, m, k, buf, kpart,
+                         zb());  // V^T V parts
       XTB_LAUNCH_CHECK();
-      hipLaunchKernelGGL(panel_tvt_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, tau, kpart, nblk, tt, b,
-                         vt);  // T, VT = V T
+      hipLaunchKernelGGL(panel_tvt_kernel<T>, dim3(nblk, 1, zgrid()), dim3(256), 0, s, m, k, buf, tau, kpart, nblk,
+                         tt, b, vt, zb());  // T, VT = V T
       XTB_LAUNCH_CHECK();
     }
     const int nsplit = gemm_splitk<T>(s, m, k, m, a2, lda, vt, m, xpart, SPLIT_ROWS);  // A2 VT in parts
-    hipLaunchKernelGGL(panel_xy_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, vt, xpart, nsplit, x, kpart);  // X, Y
+    hipLaunchKernelGGL(panel_xy_kernel<T>, dim3(nblk, 1, zgrid()), dim3(256), 0, s, m, k, vt, xpart, nsplit, x,
+                       kpart, zb());  // X, Y
     XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(panel_w_kernel<T>, dim3(nblk), dim3(256), 0, s, m, k, buf, kpart, nblk, x);  // W
+    hipLaunchKernelGGL(panel_w_kernel<T>, dim3(nblk, 1, zgrid()), dim3(256), 0, s, m, k, buf, kpart, nblk, x,
+                       zb());  // W
     XTB_LAUNCH_CHECK();
     gemm<T>(s, 'N', 'T', m, m, 2 * k, T(-1), buf, m, buf + (size_t)m * k, m, T(1), a2, lda);  // A2 -= [V W][W V]^T
   }
-  hipLaunchKernelGGL(extract_band_kernel<T>, dim3(n), dim3(128), 0, s, n, b, a, lda, ab);
+  hipLaunchKernelGGL(extract_band_kernel<T>, dim3(n, 1, zgrid()), dim3(128), 0, s, n, b, a, lda, ab, zb());
   XTB_LAUNCH_CHECK();
 }
 

@@ -478,8 +478,13 @@ enum { BAND_DEVICE = 0, BAND_LDS = 1, BAND_SPLIT = 2 };
 template <typename T, int B, int H, int MODE, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __restrict__ ab_global, T* __restrict__ v2,
                                                                  int ldv2, T* __restrict__ d, T* __restrict__ e,
-                                                                 int* __restrict__ status, int fs) {
+                                                                 int* __restrict__ status, int fs, Zb zb) {
   constexpr int GS = H * B;         // lanes per group
+  ab_global = zb.at(ab_global);
+  v2 = zb.at(v2);
+  d = zb.at(d);
+  e = zb.at(e);
+  status = zb.at(status);
   constexpr int SPW = 64 / GS;      // sweeps per wavefront
   constexpr int NG = WAVES * SPW;   // sweep groups: group g takes sweeps g, g + NG, ...
   extern __shared__ char smem_raw[];
@@ -563,8 +568,14 @@ __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __res
 template <typename T, int B, bool LDS, int CHASE_WAVES>
 __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T* __restrict__ ab_global,
                                                                  T* __restrict__ v2, int ldv2, T* __restrict__ d,
-                                                                 T* __restrict__ e, int* __restrict__ status) {
+                                                                 T* __restrict__ e, int* __restrict__ status,
+                                                                 Zb zb) {
   // dynamic LDS: per-wave w and q vectors (b each), then the band when it is LDS-resident
+  ab_global = zb.at(ab_global);
+  v2 = zb.at(v2);
+  d = zb.at(d);
+  e = zb.at(e);
+  status = zb.at(status);
   extern __shared__ char smem_raw[];
   __shared__ int prog[CHASE_WAVES];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -628,11 +639,11 @@ void launch_chase(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* st
   const size_t band_bytes = sizeof(T) * (size_t)band_elems(n, b);
   if (vec_bytes + band_bytes + sizeof(int) * CHASE_WAVES <= 160 * 1024) {
     allow_lds(chase_kernel<T, B, true, CHASE_WAVES>, vec_bytes + band_bytes);
-    hipLaunchKernelGGL((chase_kernel<T, B, true, CHASE_WAVES>), dim3(1), dim3(64 * CHASE_WAVES), vec_bytes + band_bytes, s, n, b,
-                       ab, v2, n, d, e, status);
+    hipLaunchKernelGGL((chase_kernel<T, B, true, CHASE_WAVES>), dim3(1, 1, zgrid()), dim3(64 * CHASE_WAVES),
+                       vec_bytes + band_bytes, s, n, b, ab, v2, n, d, e, status, zb());
   } else {
-    hipLaunchKernelGGL((chase_kernel<T, B, false, CHASE_WAVES>), dim3(1), dim3(64 * CHASE_WAVES), vec_bytes, s, n, b, ab, v2, n,
-                       d, e, status);
+    hipLaunchKernelGGL((chase_kernel<T, B, false, CHASE_WAVES>), dim3(1, 1, zgrid()), dim3(64 * CHASE_WAVES), vec_bytes,
+                       s, n, b, ab, v2, n, d, e, status, zb());
   }
   XTB_LAUNCH_CHECK();
 }
@@ -645,8 +656,8 @@ void launch_chase_lanes(hipStream_t s, int n, T* ab, T* v2, T* d, T* e, int* sta
   const size_t all = chase_lanes_lds<B, H, T>(n, WAVES, true);
   if (all <= LDS_MAX) {
     allow_lds(chase_lanes_kernel<T, B, H, BAND_LDS, WAVES>, all);
-    hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_LDS, WAVES>), dim3(1), dim3(64 * WAVES), all, s, n, ab, v2,
-                       n, d, e, status, 0);
+    hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_LDS, WAVES>), dim3(1, 1, zgrid()), dim3(64 * WAVES), all, s,
+                       n, ab, v2, n, d, e, status, 0, zb());
   } else {
     // the last columns on chip: fs, a column boundary, leaves at most LDS_MAX bytes for them and the q slots
     const size_t qbytes = chase_lanes_lds<B, H, T>(n, WAVES, false);
@@ -656,11 +667,11 @@ void launch_chase_lanes(hipStream_t s, int n, T* ab, T* v2, T* d, T* e, int* sta
     if (fs < nel - 4 * B * (2 * B)) {  // worth it only with a few columns on chip
       const size_t lds = chase_lanes_lds<B, H, T>(n, WAVES, true, fs);
       allow_lds(chase_lanes_kernel<T, B, H, BAND_SPLIT, WAVES>, lds);
-      hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_SPLIT, WAVES>), dim3(1), dim3(64 * WAVES), lds, s, n, ab,
-                         v2, n, d, e, status, (int)fs);
+      hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_SPLIT, WAVES>), dim3(1, 1, zgrid()), dim3(64 * WAVES), lds,
+                         s, n, ab, v2, n, d, e, status, (int)fs, zb());
     } else {
-      hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_DEVICE, WAVES>), dim3(1), dim3(64 * WAVES), qbytes, s, n,
-                         ab, v2, n, d, e, status, (int)nel);
+      hipLaunchKernelGGL((chase_lanes_kernel<T, B, H, BAND_DEVICE, WAVES>), dim3(1, 1, zgrid()), dim3(64 * WAVES),
+                         qbytes, s, n, ab, v2, n, d, e, status, (int)nel, zb());
     }
   }
   XTB_LAUNCH_CHECK();

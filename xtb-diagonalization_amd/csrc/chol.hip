@@ -53,7 +53,11 @@ __device__ inline gemm_detail::Mfma<double>::acc_t mfma_block(const double* pa, 
 // pivot k (then nothing is written).
 __global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* __restrict__ a, int lda,
                                                          double* __restrict__ u, int ldu, double* __restrict__ x,
-                                                         int ldx, int* info) {
+                                                         int ldx, int* info, Zb zb) {
+  a = zb.at(a);
+  u = zb.at(u);
+  x = zb.at(x);
+  info = zb.at(info);
   constexpr int LD = NB + 1;
   __shared__ double Am[NB][LD];
   __shared__ double Ym[NB][LD];
@@ -213,7 +217,8 @@ void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu,
   const bool beside = s2 != nullptr && ev != nullptr && ev_join != nullptr;
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = std::min(NB, n - k0), r = n - k0 - kb;
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, n, k0, a, lda, u, ldu, x, ldx, info);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1, 1, zgrid()), dim3(256), 0, s, n, k0, a, lda, u, ldu, x, ldx, info,
+                       zb());
     XTB_LAUNCH_CHECK();
     if (beside && k0 > 0) {
       // X(0:k0, block) = -(X(0:k0, 0:k0) U(0:k0, block)) X_kk on s2: U's block column came from the row panels of

@@ -49,6 +49,48 @@ inline void allow_lds(K kernel, size_t bytes) {
 __host__ __device__ inline int64_t idx(int64_t i, int64_t j, int64_t ld) { return i + j * ld; }
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// ---- problem batches ------------------------------------------------------------------------------------------------
+// A batched solve enqueues the pipeline once for `count` problems of one size whose buffers lie in equal arena slices
+// `stride` bytes apart. Every kernel of the pipeline carries the problem in the slow part of its grid's z index
+// (z = problem * zin + the kernel's own z, zin its own z extent) and moves each pointer into that problem's slice with
+// Zb::at before use. A single solve is the batch of one: stride 0, at() returns its argument, the grid is unchanged.
+// Tables shared by every problem of the batch (the divide-and-conquer tree, the bulge back-transformation plan) are
+// not moved.
+struct Zb {
+  size_t stride;  // bytes between consecutive problems' slices
+  int zin;        // the kernel's own z extent
+  __device__ inline int prob() const { return (int)blockIdx.z / zin; }
+  __device__ inline int z() const { return (int)blockIdx.z % zin; }
+  template <typename P>
+  __device__ inline P* at(P* p) const {
+    return p ? (P*)((uintptr_t)p + (size_t)prob() * stride) : p;
+  }
+};
+// The batch that the calling host thread is enqueuing (count 1, stride 0 outside a BatchScope).
+struct Batch {
+  int count = 1;
+  size_t stride = 0;
+};
+Batch& current_batch();
+inline Zb zb(int zin = 1) { return Zb{current_batch().stride, zin}; }
+// grid z extent of a launch whose own z extent is zin
+inline unsigned zgrid(int zin = 1) {
+  const long long z = (long long)zin * current_batch().count;
+  if (z > 65535) throw std::runtime_error("batched launch: grid z extent " + std::to_string(z) + " past 65535");
+  return (unsigned)z;
+}
+// Sets the calling thread's batch for its lifetime (nested scopes restore the outer one).
+struct BatchScope {
+  Batch saved;
+  BatchScope(int count, size_t stride) : saved(current_batch()) {
+    current_batch().count = count;
+    current_batch().stride = stride;
+  }
+  ~BatchScope() { current_batch() = saved; }
+  BatchScope(const BatchScope&) = delete;
+  BatchScope& operator=(const BatchScope&) = delete;
+};
+
 // ---- wavefront (64-lane) reductions -------------------------------------------------------------
 // DPP moves within rows of 16 lanes (quad permutes, half-row and row mirrors), for 32-bit lanes; 64-bit values move
 // as two halves.

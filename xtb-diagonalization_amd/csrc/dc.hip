@@ -81,7 +81,11 @@ __device__ inline void lamrg(int n1, int n2, const double* a, int s1, int s2, in
 
 // Scale by the max norm, apply every tear (pre-order, parent before children), Q = I.
 __global__ __launch_bounds__(1024) void prepare_kernel(int n, double* d, double* e, const int* __restrict__ split_pre,
-                                                       double* rho, double* scale) {
+                                                       double* rho, double* scale, Zb zb) {
+  d = zb.at(d);
+  e = zb.at(e);
+  rho = zb.at(rho);
+  scale = zb.at(scale);
   __shared__ double red[16];
   const int t = threadIdx.x;
   double m = 0;
@@ -115,7 +119,9 @@ __global__ __launch_bounds__(1024) void prepare_kernel(int n, double* d, double*
   }
 }
 
-__global__ void unscale_kernel(int n, double* d, const double* scale) {
+__global__ void unscale_kernel(int n, double* d, const double* scale, Zb zb) {
+  d = zb.at(d);
+  scale = zb.at(scale);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] *= *scale;
 }
@@ -156,8 +162,20 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
                                                       double* __restrict__ rho_lvl, double* __restrict__ dlamda_all,
                                                       double* __restrict__ w_all, int* __restrict__ indx_all,
                                                       int* __restrict__ indxc_all, int* __restrict__ rotp_all,
-                                                      double* __restrict__ rotcs_all, NodeOut* __restrict__ out) {
+                                                      double* __restrict__ rotcs_all, NodeOut* __restrict__ out,
+                                                      Zb zb) {
 #pragma clang fp contract(off)  // dlaed2's scalar recurrences rounded as written (no fused multiply-add)
+  d_all = zb.at(d_all);
+  q_all = zb.at(q_all);
+  rho_pre = zb.at(rho_pre);
+  rho_lvl = zb.at(rho_lvl);
+  dlamda_all = zb.at(dlamda_all);
+  w_all = zb.at(w_all);
+  indx_all = zb.at(indx_all);
+  indxc_all = zb.at(indxc_all);
+  rotp_all = zb.at(rotp_all);
+  rotcs_all = zb.at(rotcs_all);
+  out = zb.at(out);
   extern __shared__ char smem[];
   const Node nd = nodes[blockIdx.x];
   const int n = nd.sz, n1 = nd.n1, n2 = n - n1, off = nd.off;
@@ -448,7 +466,11 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
 
 // The deflation's Givens rotations on Q's columns, in order; each thread owns one row of the merge's block.
 __global__ void rotate_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ rotp_all,
-                              const double* __restrict__ rotcs_all, double* __restrict__ q_all, int ldq) {
+                              const double* __restrict__ rotcs_all, double* __restrict__ q_all, int ldq, Zb zb) {
+  out = zb.at(out);
+  rotp_all = zb.at(rotp_all);
+  rotcs_all = zb.at(rotcs_all);
+  q_all = zb.at(q_all);
   const Node nd = nodes[blockIdx.y];
   const int nrot = out[blockIdx.y].nrot;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -466,7 +488,11 @@ __global__ void rotate_kernel(const Node* nodes, const NodeOut* out, const int* 
 // Q2 = [ n1 x n12 | n2 x n23 | n x ctot3 ]: column pos of the grouped order from Q's column INDX(pos) (dlaed2's
 // copies). One workgroup per column.
 __global__ void q2_gather_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ indx_all,
-                                 const double* __restrict__ q_all, int ldq, double* __restrict__ q2_all) {
+                                 const double* __restrict__ q_all, int ldq, double* __restrict__ q2_all, Zb zb) {
+  out = zb.at(out);
+  indx_all = zb.at(indx_all);
+  q_all = zb.at(q_all);
+  q2_all = zb.at(q2_all);
   const Node nd = nodes[blockIdx.y];
   const int pos = blockIdx.x;
   if (pos >= nd.sz) return;
@@ -501,7 +527,17 @@ __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const No
                                                      const double* __restrict__ w_all, const int* __restrict__ indxc_all,
                                                      double* __restrict__ d_all, int* __restrict__ orig_all,
                                                      double* __restrict__ tau_all, double* __restrict__ s_all,
-                                                     int* info, int maxit) {
+                                                     int* info, int maxit, Zb zb) {
+  out = zb.at(out);
+  rho_lvl = zb.at(rho_lvl);
+  dlamda_all = zb.at(dlamda_all);
+  w_all = zb.at(w_all);
+  indxc_all = zb.at(indxc_all);
+  d_all = zb.at(d_all);
+  orig_all = zb.at(orig_all);
+  tau_all = zb.at(tau_all);
+  s_all = zb.at(s_all);
+  info = zb.at(info);
   extern __shared__ double poles[];  // dlamda (k), then z (k): every root reads all of them each iteration
   constexpr int RPB = 64 / SEC_G;    // roots per workgroup
   const Node nd = nodes[blockIdx.y];
@@ -563,7 +599,13 @@ constexpr int GU_G = 16;  // lanes per z_i: each multiplies every GU_G-th factor
 
 __global__ void gu_kernel(const Node* nodes, const NodeOut* out, const double* __restrict__ dlamda_all,
                           const double* __restrict__ w_all, const int* __restrict__ orig_all,
-                          const double* __restrict__ tau_all, double* __restrict__ wnew_all) {
+                          const double* __restrict__ tau_all, double* __restrict__ wnew_all, Zb zb) {
+  out = zb.at(out);
+  dlamda_all = zb.at(dlamda_all);
+  w_all = zb.at(w_all);
+  orig_all = zb.at(orig_all);
+  tau_all = zb.at(tau_all);
+  wnew_all = zb.at(wnew_all);
   const Node nd = nodes[blockIdx.y];
   const int k = out[blockIdx.y].k;
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / GU_G, gl = threadIdx.x % GU_G;
@@ -588,7 +630,14 @@ __global__ __launch_bounds__(256) void vectors_kernel(const Node* nodes, const N
                                                       const int* __restrict__ orig_all,
                                                       const double* __restrict__ tau_all,
                                                       const double* __restrict__ wnew_all,
-                                                      const int* __restrict__ indxc_all, double* __restrict__ s_all) {
+                                                      const int* __restrict__ indxc_all, double* __restrict__ s_all, Zb zb) {
+  out = zb.at(out);
+  dlamda_all = zb.at(dlamda_all);
+  orig_all = zb.at(orig_all);
+  tau_all = zb.at(tau_all);
+  wnew_all = zb.at(wnew_all);
+  indxc_all = zb.at(indxc_all);
+  s_all = zb.at(s_all);
   const Node nd = nodes[blockIdx.y];
   const int k = out[blockIdx.y].k;
   const int lane = threadIdx.x & 63;
@@ -617,10 +666,14 @@ __global__ __launch_bounds__(256) void vectors_kernel(const Node* nodes, const N
 __global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, const NodeOut* out,
                                                          const double* __restrict__ q2_all,
                                                          const double* __restrict__ s_all, double* __restrict__ q_all,
-                                                         int ldq) {
+                                                         int ldq, Zb zb) {
+  out = zb.at(out);
+  q2_all = zb.at(q2_all);
+  s_all = zb.at(s_all);
+  q_all = zb.at(q_all);
   using namespace gemm_detail;
   __shared__ __attribute__((aligned(16))) double lds[tile_lds_elems<double>()];
-  const int node = blockIdx.z >> 1, half = blockIdx.z & 1;
+  const int node = zb.z() >> 1, half = zb.z() & 1;
   const Node nd = nodes[node];
   const NodeOut o = out[node];
   const int k = o.k;
@@ -641,7 +694,10 @@ __global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, cons
 // stored descending (read backwards), merged by ranks; on ties the first list's element comes first, as dlamrg.
 // K = 0 (every column deflated at once) leaves them ascending: the identity.
 __global__ __launch_bounds__(256) void lamrg_kernel(int count, const Node* nodes, const NodeOut* out,
-                                                    const double* __restrict__ d_all, int* __restrict__ perm_all) {
+                                                    const double* __restrict__ d_all, int* __restrict__ perm_all, Zb zb) {
+  out = zb.at(out);
+  d_all = zb.at(d_all);
+  perm_all = zb.at(perm_all);
   const Node nd = nodes[blockIdx.x];
   const int k = out[blockIdx.x].k, sz = nd.sz, n2 = sz - k;
   const double* a = d_all + nd.off;
@@ -681,11 +737,18 @@ __global__ __launch_bounds__(256) void lamrg_kernel(int count, const Node* nodes
 __global__ void gather_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ perm_all,
                               const double* __restrict__ q_all, int ldq, const double* __restrict__ q2_all,
                               const double* __restrict__ d_all, double* __restrict__ tmp_all,
-                              double* __restrict__ dtmp_all) {
-  const Node nd = nodes[blockIdx.z];
+                              double* __restrict__ dtmp_all, Zb zb) {
+  out = zb.at(out);
+  perm_all = zb.at(perm_all);
+  q_all = zb.at(q_all);
+  q2_all = zb.at(q2_all);
+  d_all = zb.at(d_all);
+  tmp_all = zb.at(tmp_all);
+  dtmp_all = zb.at(dtmp_all);
+  const Node nd = nodes[zb.z()];
   const int col = blockIdx.y;
   if (col >= nd.sz) return;
-  const NodeOut o = out[blockIdx.z];
+  const NodeOut o = out[zb.z()];
   const int n = nd.sz, n1 = nd.n1, n2 = n - n1, k = o.k;
   const int src = perm_all[nd.off + col] - 1;
   const double* from;
@@ -703,8 +766,12 @@ __global__ void gather_kernel(const Node* nodes, const NodeOut* out, const int* 
 
 __global__ void scatter_kernel(const Node* nodes, const double* __restrict__ tmp_all,
                                const double* __restrict__ dtmp_all, double* __restrict__ q_all, int ldq,
-                               double* __restrict__ d_all) {
-  const Node nd = nodes[blockIdx.z];
+                               double* __restrict__ d_all, Zb zb) {
+  tmp_all = zb.at(tmp_all);
+  dtmp_all = zb.at(dtmp_all);
+  q_all = zb.at(q_all);
+  d_all = zb.at(d_all);
+  const Node nd = nodes[zb.z()];
   const int col = blockIdx.y;
   if (col >= nd.sz) return;
   const double* tmp = tmp_all + nd.q2off;
@@ -817,13 +884,14 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
   const size_t lds = (size_t)maxsz * (6 * sizeof(double) + 7 * sizeof(int)) + 64;
   if (lds > 160 * 1024) throw std::runtime_error("stedc: n too large for the LDS-staged deflation");
   allow_lds(deflate_kernel, lds);
-  hipLaunchKernelGGL(deflate_kernel, dim3(cnt), dim3(256), lds, s, lvl, lvl_pre, d, z, ldz, W.rho_pre, W.rho_lvl,
-                     W.dlamda, W.w, W.indx, W.indxc, W.rotp, W.rotcs, W.outs);
+  hipLaunchKernelGGL(deflate_kernel, dim3(cnt, 1, zgrid()), dim3(256), lds, s, lvl, lvl_pre, d, z, ldz, W.rho_pre,
+                     W.rho_lvl, W.dlamda, W.w, W.indx, W.indxc, W.rotp, W.rotcs, W.outs, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(rotate_kernel, dim3(cdiv(maxsz, 256), cnt), dim3(256), 0, s, lvl, W.outs, W.rotp, W.rotcs, z,
-                     ldz);
+  hipLaunchKernelGGL(rotate_kernel, dim3(cdiv(maxsz, 256), cnt, zgrid()), dim3(256), 0, s, lvl, W.outs, W.rotp,
+                     W.rotcs, z, ldz, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(q2_gather_kernel, dim3(maxsz, cnt), dim3(256), 0, s, lvl, W.outs, W.indx, z, ldz, W.q2);
+  hipLaunchKernelGGL(q2_gather_kernel, dim3(maxsz, cnt, zgrid()), dim3(256), 0, s, lvl, W.outs, W.indx, z, ldz, W.q2,
+                     zb());
   XTB_LAUNCH_CHECK();
   if (probe) {
     XTB_HIP(hipMemcpyAsync(probe->d_defl, d, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
@@ -835,26 +903,25 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
     XTB_HIP(hipMemcpyAsync(probe->rho, W.rho_lvl, sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   allow_lds(secular_kernel, 2 * sizeof(double) * maxsz);
-  hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64 / SEC_G), cnt), dim3(64), 2 * sizeof(double) * maxsz, s, lvl,
-                     W.outs,
-                     W.rho_lvl, W.dlamda, W.w, W.indxc, d, W.orig, W.tau, W.sbuf, info, maxit);
+  hipLaunchKernelGGL(secular_kernel, dim3(cdiv(maxsz, 64 / SEC_G), cnt, zgrid()), dim3(64), 2 * sizeof(double) * maxsz,
+                     s, lvl, W.outs, W.rho_lvl, W.dlamda, W.w, W.indxc, d, W.orig, W.tau, W.sbuf, info, maxit, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz * GU_G, 64), cnt), dim3(64), 0, s, lvl, W.outs, W.dlamda, W.w, W.orig,
-                     W.tau, W.wnew);
+  hipLaunchKernelGGL(gu_kernel, dim3(cdiv(maxsz * GU_G, 64), cnt, zgrid()), dim3(64), 0, s, lvl, W.outs, W.dlamda, W.w,
+                     W.orig, W.tau, W.wnew, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(maxsz, 4), cnt), dim3(256), 0, s, lvl, W.outs, W.dlamda, W.orig,
-                     W.tau, W.wnew, W.indxc, W.sbuf);
+  hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(maxsz, 4), cnt, zgrid()), dim3(256), 0, s, lvl, W.outs, W.dlamda,
+                     W.orig, W.tau, W.wnew, W.indxc, W.sbuf, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), 2 * cnt), dim3(256), 0, s, lvl,
-                     W.outs, W.q2, W.sbuf, z, ldz);
+  hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), zgrid(2 * cnt)), dim3(256), 0, s, lvl,
+                     W.outs, W.q2, W.sbuf, z, ldz, zb(2 * cnt));
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(lamrg_kernel, dim3(cnt), dim3(256), 0, s, cnt, lvl, W.outs, d, W.perm);
+  hipLaunchKernelGGL(lamrg_kernel, dim3(cnt, 1, zgrid()), dim3(256), 0, s, cnt, lvl, W.outs, d, W.perm, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(gather_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, W.outs, W.perm, z, ldz,
-                     W.q2, d, W.sbuf, W.dtmp);
+  hipLaunchKernelGGL(gather_kernel, dim3(cdiv(maxsz, 256), maxsz, zgrid(cnt)), dim3(256), 0, s, lvl, W.outs, W.perm,
+                     z, ldz, W.q2, d, W.sbuf, W.dtmp, zb(cnt));
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(maxsz, 256), maxsz, cnt), dim3(256), 0, s, lvl, W.sbuf, W.dtmp, z,
-                     ldz, d);
+  hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(maxsz, 256), maxsz, zgrid(cnt)), dim3(256), 0, s, lvl, W.sbuf, W.dtmp,
+                     z, ldz, d, zb(cnt));
   XTB_LAUNCH_CHECK();
 }
 
@@ -872,14 +939,15 @@ void stedc(hipStream_t s, const DcPlan& plan, double* d, double* e, double* z, i
   const DcWork W = carve(n, work, iwork);
   const Node* nodes = plan.nodes_dev;
   set_identity(s, n, z, ldz);
-  hipLaunchKernelGGL(prepare_kernel, dim3(1), dim3(1024), 0, s, n, d, e, plan.split_pre_dev, W.rho_pre, W.scale);
+  hipLaunchKernelGGL(prepare_kernel, dim3(1, 1, zgrid()), dim3(1024), 0, s, n, d, e, plan.split_pre_dev, W.rho_pre,
+                     W.scale, zb());
   XTB_LAUNCH_CHECK();
   for (int L = (int)plan.lvls.size() - 1; L >= 0; --L) {
     const DcPlan::Level& lv = plan.lvls[L];
     run_level(s, n, nodes + lv.start, plan.pre_index_dev + lv.pre_start, lv.count, lv.maxsz, d, z, ldz, W, info,
               nullptr, maxit);
   }
-  hipLaunchKernelGGL(unscale_kernel, dim3(cdiv(n, 256)), dim3(256), 0, s, n, d, W.scale);
+  hipLaunchKernelGGL(unscale_kernel, dim3(cdiv(n, 256), 1, zgrid()), dim3(256), 0, s, n, d, W.scale, zb());
   XTB_LAUNCH_CHECK();
 }
 

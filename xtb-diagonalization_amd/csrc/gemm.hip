@@ -31,11 +31,12 @@ template <typename T, bool TA, bool TB, int TBN>
 __global__ __launch_bounds__(Shape<TBN>::NT) void gemm_kernel(int M, int N, int K, T alpha, const T* __restrict__ A,
                                                               int lda, int64_t sA, const T* __restrict__ B, int ldb,
                                                               int64_t sB, T beta, T* __restrict__ C, int ldc,
-                                                              int64_t sC) {
+                                                              int64_t sC, Zb zb) {
   __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T, TBN>()];
-  A += blockIdx.z * sA;
-  B += blockIdx.z * sB;
-  C += blockIdx.z * sC;
+  const int z = zb.z();
+  A = zb.at(A) + z * sA;
+  B = zb.at(B) + z * sB;
+  C = zb.at(C) + z * sC;
   gemm_tile<T, TA, TB, Shape<TBN>::NT, TBN>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, blockIdx.x * BM,
                                             blockIdx.y * TBN, lds);
 }
@@ -44,11 +45,11 @@ __global__ __launch_bounds__(Shape<TBN>::NT) void gemm_kernel(int M, int N, int 
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_upper_kernel(int N, int K, T alpha, const T* __restrict__ A, int lda,
                                                          int64_t sA, const T* __restrict__ B, int ldb, int64_t sB,
-                                                         T beta, T* __restrict__ C, int ldc, int64_t sC) {
+                                                         T beta, T* __restrict__ C, int ldc, int64_t sC, Zb zb) {
   __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T>()];
-  A += blockIdx.y * sA;
-  B += blockIdx.y * sB;
-  C += blockIdx.y * sC;
+  A = zb.at(A) + blockIdx.y * sA;
+  B = zb.at(B) + blockIdx.y * sB;
+  C = zb.at(C) + blockIdx.y * sC;
   const int q = blockIdx.x;
   int J = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
   while (J * (J + 1) / 2 > q) --J;
@@ -60,24 +61,24 @@ __global__ __launch_bounds__(256) void gemm_upper_kernel(int N, int K, T alpha, 
 template <typename T, bool TA, bool TB, int TBN>
 void launch_w(hipStream_t s, int m, int n, int k, T alpha, const T* A, int lda, int64_t sA, const T* B, int ldb,
               int64_t sB, T beta, T* C, int ldc, int64_t sC, int batch) {
-  dim3 grid(cdiv(m, BM), cdiv(n, TBN), batch);
+  dim3 grid(cdiv(m, BM), cdiv(n, TBN), zgrid(batch));
   hipLaunchKernelGGL((gemm_kernel<T, TA, TB, TBN>), grid, dim3(Shape<TBN>::NT), 0, s, m, n, k, alpha, A, lda, sA, B,
-                     ldb, sB, beta, C, ldc, sC);
+                     ldb, sB, beta, C, ldc, sC, zb(batch));
   XTB_LAUNCH_CHECK();
 }
 
 template <typename T, bool TA, bool TB>
 void launch(hipStream_t s, int m, int n, int k, T alpha, const T* A, int lda, int64_t sA, const T* B, int ldb,
             int64_t sB, T beta, T* C, int ldc, int64_t sC, int batch) {
-  const int w = gemm_tile_width(m, n, batch, std::is_same<T, double>::value);
+  const int w = gemm_tile_width(m, n, batch * current_batch().count, std::is_same<T, double>::value);
   if (w == 128) launch_w<T, TA, TB, 128>(s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
   else launch_w<T, TA, TB, 64>(s, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch);
 }
 
 // C = beta C when K = 0 (alpha op(A) op(B) vanishes).
 template <typename T>
-__global__ void scale_kernel(int M, int N, T beta, T* C, int ldc, int64_t sC) {
-  C += blockIdx.z * sC;
+__global__ void scale_kernel(int M, int N, T beta, T* C, int ldc, int64_t sC, Zb zb) {
+  C = zb.at(C) + zb.z() * sC;
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i < M) C[idx(i, j, ldc)] = beta == T(0) ? T(0) : beta * C[idx(i, j, ldc)];
 }
@@ -203,7 +204,8 @@ void gemm(hipStream_t s, char ta, char tb, int m, int n, int k, T alpha, const T
   if (m <= 0 || n <= 0 || batch <= 0) return;
   if (k <= 0) {
     if (beta == T(1)) return;
-    hipLaunchKernelGGL(scale_kernel<T>, dim3(cdiv(m, 256), n, batch), dim3(256), 0, s, m, n, beta, C, ldc, sC);
+    hipLaunchKernelGGL(scale_kernel<T>, dim3(cdiv(m, 256), n, zgrid(batch)), dim3(256), 0, s, m, n, beta, C, ldc, sC,
+                       zb(batch));
     XTB_LAUNCH_CHECK();
     return;
   }
@@ -220,9 +222,12 @@ namespace {
 template <typename T>
 __global__ __launch_bounds__(256) void gemm_splitk_kernel(int M, int N, int K, int kc, const T* __restrict__ A,
                                                           int lda, const T* __restrict__ B, int ldb,
-                                                          T* __restrict__ C) {
+                                                          T* __restrict__ C, Zb zb) {
   __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T>()];
-  const int z = blockIdx.z;
+  const int z = zb.z();
+  A = zb.at(A);
+  B = zb.at(B);
+  C = zb.at(C);
   const int k0 = z * kc, kz = min(kc, K - k0);
   gemm_tile<T, false, false>(M, N, kz, T(1), A + (int64_t)k0 * lda, lda, B + k0, ldb, T(0),
                              C + (int64_t)z * M * N, M, blockIdx.x * BM, blockIdx.y * BN, lds);
@@ -233,8 +238,8 @@ template <typename T>
 int gemm_splitk(hipStream_t s, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int kc) {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
   const int splits = cdiv(k, kc);
-  hipLaunchKernelGGL(gemm_splitk_kernel<T>, dim3(cdiv(m, BM), cdiv(n, BN), splits), dim3(256), 0, s, m, n, k, kc, A,
-                     lda, B, ldb, C);
+  hipLaunchKernelGGL(gemm_splitk_kernel<T>, dim3(cdiv(m, BM), cdiv(n, BN), zgrid(splits)), dim3(256), 0, s, m, n, k,
+                     kc, A, lda, B, ldb, C, zb(splits));
   XTB_LAUNCH_CHECK();
   return splits;
 }
@@ -245,16 +250,22 @@ namespace {
 template <typename T, bool TA>
 __global__ __launch_bounds__(256) void gemm_thin_kernel(int M, int N, int K, int kc, const T* __restrict__ A,
                                                         int lda, const T* __restrict__ B, int ldb,
-                                                        T* __restrict__ part) {
+                                                        T* __restrict__ part, Zb zb) {
   __shared__ __attribute__((aligned(16))) T lds[tile_lds_elems<T>()];
   const int z = blockIdx.y;
+  A = zb.at(A);
+  B = zb.at(B);
+  part = zb.at(part);
   const int k0 = z * kc, kz = min(kc, K - k0);
   const T* a0 = TA ? A + k0 : A + (int64_t)k0 * lda;
   gemm_tile<T, TA, false>(M, N, kz, T(1), a0, lda, B + k0, ldb, T(0), part + (int64_t)z * M * N, M,
                           blockIdx.x * BM, 0, lds);
 }
 template <typename T>
-__global__ void sum_parts_kernel(int M, int N, int splits, const T* __restrict__ part, T* __restrict__ C, int ldc) {
+__global__ void sum_parts_kernel(int M, int N, int splits, const T* __restrict__ part, T* __restrict__ C, int ldc,
+                                 Zb zb) {
+  part = zb.at(part);
+  C = zb.at(C);
   const int64_t mn = (int64_t)M * N;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= mn) return;
@@ -281,15 +292,15 @@ void gemm_thin(hipStream_t s, char ta, int m, int n, int k, const T* A, int lda,
     return;
   }
   if (ta == 'T' || ta == 't')
-    hipLaunchKernelGGL((gemm_thin_kernel<T, true>), dim3(cdiv(m, BM), splits), dim3(256), 0, s, m, n, k, THIN_KC, A,
-                       lda, B, ldb, part);
+    hipLaunchKernelGGL((gemm_thin_kernel<T, true>), dim3(cdiv(m, BM), splits, zgrid()), dim3(256), 0, s, m, n, k,
+                       THIN_KC, A, lda, B, ldb, part, zb());
   else
-    hipLaunchKernelGGL((gemm_thin_kernel<T, false>), dim3(cdiv(m, BM), splits), dim3(256), 0, s, m, n, k, THIN_KC,
-                       A, lda, B, ldb, part);
+    hipLaunchKernelGGL((gemm_thin_kernel<T, false>), dim3(cdiv(m, BM), splits, zgrid()), dim3(256), 0, s, m, n, k,
+                       THIN_KC, A, lda, B, ldb, part, zb());
   XTB_LAUNCH_CHECK();
   const int64_t mn = (int64_t)m * n;
-  hipLaunchKernelGGL(sum_parts_kernel<T>, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, m, n, splits, part, C,
-                     ldc);
+  hipLaunchKernelGGL(sum_parts_kernel<T>, dim3((unsigned)((mn + 255) / 256), 1, zgrid()), dim3(256), 0, s, m, n,
+                     splits, part, C, ldc, zb());
   XTB_LAUNCH_CHECK();
 }
 template void gemm_thin<double>(hipStream_t, char, int, int, int, const double*, int, const double*, int, double*,
@@ -307,8 +318,8 @@ void gemm_upper_tn(hipStream_t s, int n, int k, T alpha, const T* A, int lda, co
     return;
   }
   const int nt = cdiv(n, BM);
-  hipLaunchKernelGGL((gemm_upper_kernel<T, true, false>), dim3(nt * (nt + 1) / 2, batch), dim3(256), 0, s, n, k,
-                     alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+  hipLaunchKernelGGL((gemm_upper_kernel<T, true, false>), dim3(nt * (nt + 1) / 2, batch, zgrid()), dim3(256), 0, s,
+                     n, k, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, zb());
   XTB_LAUNCH_CHECK();
 }
 template void gemm_upper_tn<double>(hipStream_t, int, int, double, const double*, int, const double*, int, double,

@@ -43,6 +43,21 @@ template <typename T>
 void gemm_thin(hipStream_t s, char ta, int m, int n, int k, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
                T* part);
 
+// grouped.hip ------------------------------------------------------------------------------------------------------
+// A problem of size n_p < n solved padded to n: H = [H_p 0; 0 D], S = [S_p 0; 0 I] with D diagonal. The padded block
+// stays exactly decoupled through every stage (its reflectors are the identity, its merges deflate it), so its
+// eigenpairs are (D_kk, e_k) and the problem's own eigenvectors keep exact zeros in the padded rows.
+struct PadSpec {
+  const int* n;  // device, one per problem of the batch (not moved by Zb): each problem's own size
+  double* h;     // H as the solve holds it (moved by Zb), or null
+  int ldh;
+};
+// hp (n x n, symmetric, the problem's block in its leading n_p rows and columns and zeros elsewhere): the padded
+// diagonal D_kk (k >= n_p) set to distinct values inside the block's spectral range, spread evenly between its smallest
+// and largest diagonal entries (diagonal entries are Rayleigh quotients, so the scale of the matrix is unchanged);
+// the same values into h's padded diagonal when h is given.
+void pad_spectrum(hipStream_t s, int n, const int* nvec, double* hp, int ldhp, double* h, int ldh);
+
 // util.hip ---------------------------------------------------------------------------------------------------------
 // dst(m x n) = src (with conversion); both column-major.
 template <typename S, typename D>

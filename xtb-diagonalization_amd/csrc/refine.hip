@@ -53,7 +53,11 @@ constexpr int CHECK_BLOCKS = 128;
 // image, staged transposed in LDS, is written down the columns of tile (J, I).
 __global__ __launch_bounds__(256) void oa_prepare_kernel(int n, double* __restrict__ r, int ldr,
                                                          double* __restrict__ a, int lda, double* __restrict__ lam,
-                                                         int* __restrict__ reach) {
+                                                         int* __restrict__ reach, Zb zb) {
+  r = zb.at(r);
+  a = zb.at(a);
+  lam = zb.at(lam);
+  reach = zb.at(reach);
   const int I = blockIdx.x, J = blockIdx.y;
   if (I > J) return;
   __shared__ double tr[64][65], ta[64][65];
@@ -86,7 +90,12 @@ __global__ __launch_bounds__(256) void oa_prepare_kernel(int n, double* __restri
 // E_ij for every pair; an unresolved pair (i, j), i < j, extends reach_i to j.
 __global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, const double* __restrict__ a, int lda,
                                  const double* __restrict__ lam, double* __restrict__ e, int lde,
-                                 int* __restrict__ reach) {
+                                 int* __restrict__ reach, Zb zb) {
+  r = zb.at(r);
+  a = zb.at(a);
+  lam = zb.at(lam);
+  e = zb.at(e);
+  reach = zb.at(reach);
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i >= n) return;
   const double rij = r[idx(i, j, ldr)];
@@ -108,7 +117,10 @@ __global__ void oa_update_kernel(int n, const double* __restrict__ r, int ldr, c
 // SCAN_CHUNK consecutive columns.
 __global__ __launch_bounds__(SCAN_THREADS) void cluster_scan_kernel(int n, const int* __restrict__ reach,
                                                                     int* __restrict__ cstart,
-                                                                    int* __restrict__ csize) {
+                                                                    int* __restrict__ csize, Zb zb) {
+  reach = zb.at(reach);
+  cstart = zb.at(cstart);
+  csize = zb.at(csize);
   __shared__ int buf[SCAN_THREADS];
   const int t = threadIdx.x, i0 = t * SCAN_CHUNK;
   // 1. prefix maximum of reach -> boundary flags
@@ -181,7 +193,14 @@ __global__ __launch_bounds__(64) void cluster_solve_kernel(int n, const double* 
                                                            const double* __restrict__ a, int lda,
                                                            const int* __restrict__ cstart,
                                                            const int* __restrict__ csize, double* __restrict__ ybuf,
-                                                           double* __restrict__ lam, int* __restrict__ status) {
+                                                           double* __restrict__ lam, int* __restrict__ status, Zb zb) {
+  r = zb.at(r);
+  a = zb.at(a);
+  cstart = zb.at(cstart);
+  csize = zb.at(csize);
+  ybuf = zb.at(ybuf);
+  lam = zb.at(lam);
+  status = zb.at(status);
   const int a0 = blockIdx.x, t = threadIdx.x;
   if (cstart[a0] != a0) return;
   const int k = csize[a0];
@@ -312,7 +331,10 @@ __global__ __launch_bounds__(64) void cluster_solve_kernel(int n, const double* 
 // the cluster step instead.
 __global__ __launch_bounds__(256) void oa_diag_kernel(int n, const double* __restrict__ r, int ldr,
                                                       double* __restrict__ e, int lde,
-                                                      const int* __restrict__ csize) {
+                                                      const int* __restrict__ csize, Zb zb) {
+  r = zb.at(r);
+  e = zb.at(e);
+  csize = zb.at(csize);
   __shared__ double red[16];
   const int i = blockIdx.x, t = threadIdx.x;
   if (csize[i] >= 2 && csize[i] <= CLUSTER_MAX) return;
@@ -335,7 +357,11 @@ __global__ __launch_bounds__(256) void oa_diag_kernel(int n, const double* __res
 __global__ __launch_bounds__(256) void cluster_apply_kernel(int n, double* __restrict__ e, int lde,
                                                             const int* __restrict__ cstart,
                                                             const int* __restrict__ csize,
-                                                            const double* __restrict__ ybuf) {
+                                                            const double* __restrict__ ybuf, Zb zb) {
+  e = zb.at(e);
+  cstart = zb.at(cstart);
+  csize = zb.at(csize);
+  ybuf = zb.at(ybuf);
   const int a0 = blockIdx.y;
   if (cstart[a0] != a0) return;
   const int k = csize[a0];
@@ -363,7 +389,9 @@ __global__ __launch_bounds__(256) void cluster_apply_kernel(int n, double* __res
 
 // rank of lambda_i in ascending order (ties by index): RANK_G lanes per i, each counting every RANK_G-th j
 constexpr int RANK_G = 16;
-__global__ __launch_bounds__(256) void rank_kernel(int n, const double* __restrict__ lam, int* __restrict__ rank) {
+__global__ __launch_bounds__(256) void rank_kernel(int n, const double* __restrict__ lam, int* __restrict__ rank, Zb zb) {
+  lam = zb.at(lam);
+  rank = zb.at(rank);
   const int g = (blockIdx.x * blockDim.x + threadIdx.x) / RANK_G, gl = threadIdx.x % RANK_G;
   const int i = min(g, n - 1);  // lanes past n count for a real i and store nothing
   const double li = lam[i];
@@ -381,7 +409,12 @@ __global__ __launch_bounds__(256) void rank_kernel(int n, const double* __restri
 template <typename T>
 __global__ void scatter_columns_kernel(int n, const int* __restrict__ rank, const T* __restrict__ src, int lds,
                                        T* __restrict__ dst, int ldd, const double* __restrict__ lam,
-                                       double* __restrict__ w) {
+                                       double* __restrict__ w, Zb zb) {
+  rank = zb.at(rank);
+  src = zb.at(src);
+  dst = zb.at(dst);
+  lam = zb.at(lam);
+  w = zb.at(w);
   const int j = blockIdx.y;
   const int r = rank[j];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -390,7 +423,9 @@ __global__ void scatter_columns_kernel(int n, const int* __restrict__ rank, cons
 }
 
 // The check's probe vectors: pv = [V | Lambda V], V (n x CHECK_VECS) of signs from a hash of (i, k).
-__global__ void check_vectors_kernel(int n, const double* __restrict__ w, double* __restrict__ pv) {
+__global__ void check_vectors_kernel(int n, const double* __restrict__ w, double* __restrict__ pv, Zb zb) {
+  w = zb.at(w);
+  pv = zb.at(pv);
   const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (i >= n) return;
   uint32_t x = (uint32_t)i * 0x9E3779B1u + (uint32_t)k * 0x85EBCA77u + 0x165667B1u;
@@ -410,7 +445,13 @@ __global__ __launch_bounds__(256) void check_partial_kernel(int n, const double*
                                                             const double* __restrict__ o,
                                                             const double* __restrict__ pv,
                                                             const double* __restrict__ h, int ldh,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part, Zb zb) {
+  hy = zb.at(hy);
+  sy = zb.at(sy);
+  o = zb.at(o);
+  pv = zb.at(pv);
+  h = zb.at(h);
+  part = zb.at(part);
   __shared__ double red[16];
   const int64_t nv = (int64_t)n * CHECK_VECS;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x, t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -439,7 +480,10 @@ __global__ __launch_bounds__(256) void check_partial_kernel(int n, const double*
 // The check's verdict: ratios[0] = residual estimate / (1e-13 n ||H||_F / sqrt(n)), ratios[1] = orthogonality
 // estimate / (1e-12 n); the status word is set when either is above 1 (or not a number).
 __global__ void check_final_kernel(int n, int nblocks, const double* __restrict__ part, double* __restrict__ ratios,
-                                   int* __restrict__ status) {
+                                   int* __restrict__ status, Zb zb) {
+  part = zb.at(part);
+  ratios = zb.at(ratios);
+  status = zb.at(status);
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double sr = 0, so = 0, sh = 0;
   for (int b = 0; b < nblocks; ++b) {
@@ -491,8 +535,8 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   int* reach = iwork + n;
   int* cstart = reach + n;
   int* csize = cstart + n;
-  const dim3 eg(cdiv(n, 256), n);
-  const dim3 tg(cdiv(n, 64), cdiv(n, 64));
+  const dim3 eg(cdiv(n, 256), n, zgrid());
+  const dim3 tg(cdiv(n, 64), cdiv(n, 64), zgrid());
   copy_matrix<double, double>(s, n, n, sm, lds, shm, n);
   copy_matrix<double, double>(s, n, n, h, ldh, shm + nn, n);
   // the current C alternates between c (ldc) and cnew (ld n)
@@ -501,18 +545,19 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
   for (int it = 0; it < iterations; ++it) {
     gemm<double>(s, 'N', 'N', n, n, n, 1.0, shm, n, cur, ldcur, 0.0, sc, n, 2, nn, 0, nn);  // S C, H C
     gemm_upper_tn<double>(s, n, n, 1.0, cur, ldcur, sc, n, 0.0, r, n, 2, 0, nn, nn);  // C^T S C, C^T H C (upper)
-    hipLaunchKernelGGL(oa_prepare_kernel, tg, dim3(256), 0, s, n, r, n, a, n, lam, reach);
+    hipLaunchKernelGGL(oa_prepare_kernel, tg, dim3(256), 0, s, n, r, n, a, n, lam, reach, zb());
     XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n, reach);
+    hipLaunchKernelGGL(oa_update_kernel, eg, dim3(256), 0, s, n, r, n, a, n, lam, e, n, reach, zb());
     XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(cluster_scan_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, n, reach, cstart, csize);
+    hipLaunchKernelGGL(cluster_scan_kernel, dim3(1, 1, zgrid()), dim3(SCAN_THREADS), 0, s, n, reach, cstart, csize,
+                       zb());
     XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(cluster_solve_kernel, dim3(n), dim3(64), 0, s, n, r, n, a, n, cstart, csize, ybuf, lam,
-                       status);
+    hipLaunchKernelGGL(cluster_solve_kernel, dim3(n, 1, zgrid()), dim3(64), 0, s, n, r, n, a, n, cstart, csize, ybuf,
+                       lam, status, zb());
     XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(oa_diag_kernel, dim3(n), dim3(256), 0, s, n, r, n, e, n, csize);
+    hipLaunchKernelGGL(oa_diag_kernel, dim3(n, 1, zgrid()), dim3(256), 0, s, n, r, n, e, n, csize, zb());
     XTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(cluster_apply_kernel, eg, dim3(256), 0, s, n, e, n, cstart, csize, ybuf);
+    hipLaunchKernelGGL(cluster_apply_kernel, eg, dim3(256), 0, s, n, e, n, cstart, csize, ybuf, zb());
     XTB_LAUNCH_CHECK();
     double* nxt = (cur == c) ? cnew : c;
     const int ldnxt = (cur == c) ? n : ldc;
@@ -521,30 +566,31 @@ void refine_oa(hipStream_t s, int n, const double* h, int ldh, const double* sm,
     ldcur = ldnxt;
   }
   // sort: columns of the current C scattered into the other buffer in ascending order of lambda, ending in c
-  hipLaunchKernelGGL(rank_kernel, dim3(cdiv(n * RANK_G, 256)), dim3(256), 0, s, n, lam, rank);
+  hipLaunchKernelGGL(rank_kernel, dim3(cdiv(n * RANK_G, 256), 1, zgrid()), dim3(256), 0, s, n, lam, rank, zb());
   XTB_LAUNCH_CHECK();
   if (cur == cnew) {
-    hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, rank, cnew, n, c,
-                       ldc, lam, w);
+    hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n, zgrid()), dim3(256), 0, s, n, rank, cnew,
+                       n, c, ldc, lam, w, zb());
     XTB_LAUNCH_CHECK();
   } else {
-    hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, rank, c, ldc,
-                       cnew, n, lam, w);
+    hipLaunchKernelGGL(scatter_columns_kernel<double>, dim3(cdiv(n, 256), n, zgrid()), dim3(256), 0, s, n, rank, c,
+                       ldc, cnew, n, lam, w, zb());
     XTB_LAUNCH_CHECK();
     copy_matrix<double, double>(s, n, n, cnew, n, c, ldc);
   }
   // check: Y = C [V | Lambda V], S Y, H Y1, C^T (S Y1)
-  hipLaunchKernelGGL(check_vectors_kernel, dim3(cdiv(n, 256), CHECK_VECS), dim3(256), 0, s, n, w, pv);
+  hipLaunchKernelGGL(check_vectors_kernel, dim3(cdiv(n, 256), CHECK_VECS, zgrid()), dim3(256), 0, s, n, w, pv, zb());
   XTB_LAUNCH_CHECK();
   // thin products, K split across the GPU; the partials go to r (n x n, free once the steps are done)
   gemm_thin<double>(s, 'N', n, 2 * CHECK_VECS, n, c, ldc, pv, n, py, n, r);
   gemm_thin<double>(s, 'N', n, 2 * CHECK_VECS, n, shm, n, py, n, psy, n, r);
   gemm_thin<double>(s, 'N', n, CHECK_VECS, n, shm + nn, n, py, n, phy, n, r);
   gemm_thin<double>(s, 'T', n, CHECK_VECS, n, c, ldc, psy, n, po, n, r);
-  hipLaunchKernelGGL(check_partial_kernel, dim3(CHECK_BLOCKS), dim3(256), 0, s, n, phy, psy, po, pv, shm + nn, n,
-                     part);
+  hipLaunchKernelGGL(check_partial_kernel, dim3(CHECK_BLOCKS, 1, zgrid()), dim3(256), 0, s, n, phy, psy, po, pv,
+                     shm + nn, n, part, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(check_final_kernel, dim3(1), dim3(64), 0, s, n, CHECK_BLOCKS, part, ratios, status);
+  hipLaunchKernelGGL(check_final_kernel, dim3(1, 1, zgrid()), dim3(64), 0, s, n, CHECK_BLOCKS, part, ratios, status,
+                     zb());
   XTB_LAUNCH_CHECK();
 }
 

@@ -315,8 +315,11 @@ size_t sygvd_bytes(int n, int b, bool mixed, int m) {
 // Everything after S = U^T U: H' = X^T H X (FP64), the standard solve of H' (FP64, or FP32 and rounded), C = X Z for
 // the first m columns, and for the mixed path the FP64 refinement against (H, S) and the check. x: X = U^-1 (n x n,
 // ld ldx, zero below the diagonal); sm is read by the refinement only.
+// pad (grouped solves): problems of their own size pad->n[p] < n padded to n; H' gets the padded block's diagonal
+// (pad_spectrum) before the reduction, and H (writable there) the same diagonal for the refinement.
 static void sygvd_core(Handle& h, Carver& c, int n, const double* hm, int ldh, const double* sm, int lds,
-                       const double* x, int ldx, double* w, double* cm, int ldc, bool mixed, int b, int m) {
+                       const double* x, int ldx, double* w, double* cm, int ldc, bool mixed, int b, int m,
+                       const PadSpec* pad = nullptr) {
   hipStream_t st = h.stream;
   const size_t nn = (size_t)n * n;
   if (!mixed) {
@@ -326,6 +329,7 @@ static void sygvd_core(Handle& h, Carver& c, int n, const double* hm, int ldh, c
     gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, ldx, 0.0, hx, n);
     gemm_upper_tn<double>(st, n, n, 1.0, x, ldx, hx, n, 0.0, hp, n);  // X^T H X is symmetric: upper tiles, mirrored
     symmetrize<double>(st, 'U', n, hp, n);
+    if (pad) pad_spectrum(st, n, pad->n, hp, n, nullptr, 0);
     // C = X Z = (X Q1 Q2) Zdc, X Q1 Q2 formed beside the divide and conquer (into hx, free by then)
     syevd_core<double>(h, s, n, b, hp, n, w, cm, ldc, x, ldx, hx, m);
   } else {
@@ -348,6 +352,7 @@ static void sygvd_core(Handle& h, Carver& c, int n, const double* hm, int ldh, c
     gemm<double>(st, 'N', 'N', n, n, n, 1.0, hm, ldh, x, ldx, 0.0, hx64, n);
     gemm_upper_tn<double>(st, n, n, 1.0, x, ldx, hx64, n, 0.0, hp64, n);
     symmetrize<double>(st, 'U', n, hp64, n);
+    if (pad) pad_spectrum(st, n, pad->n, hp64, n, pad->h, pad->ldh);
     copy_matrix<double, float>(st, n, n, hp64, n, hp, n);
     // the refinement needs the whole basis: all n vectors are formed and refined, the first m returned
     syevd_core<float>(h, s, n, b, hp, n, w, c32, n, x32, n, pm);
@@ -361,12 +366,17 @@ static void sygvd_core(Handle& h, Carver& c, int n, const double* hm, int ldh, c
 void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm, int lds, double* w, double* cm,
                   int ldc, bool mixed, int b, const double** u_out) {
   if (n <= 0) return;
-  hipStream_t st = h.stream;
   if (b <= 0) b = h.band_for(n, mixed);
   b = std::max(1, std::min(b, std::max(1, n - 1)));
   check_size(n, b);
   h.reserve(sygvd_bytes(n, b, mixed, n));
-  Carver c(h.arena);
+  sygvd_in(h, h.arena, n, hm, ldh, sm, lds, w, cm, ldc, mixed, b, u_out, nullptr);
+}
+
+void sygvd_in(Handle& h, void* base, int n, const double* hm, int ldh, const double* sm, int lds, double* w,
+              double* cm, int ldc, bool mixed, int b, const double** u_out, const PadSpec* pad) {
+  hipStream_t st = h.stream;
+  Carver c(base);
   const size_t nn = (size_t)n * n;
   double* scopy = c.take<double>(nn);
   double* u = c.take<double>(nn);
@@ -375,7 +385,7 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
   copy_matrix<double, double>(st, n, n, sm, lds, scopy, n);
   potrf_inverse(st, n, scopy, n, u, n, x, n, tmp, h.dinfo + 0, h.stream2, h.ev_band, h.ev_join);
   if (u_out) *u_out = u;  // no later stage carves over it
-  sygvd_core(h, c, n, hm, ldh, sm, lds, x, n, w, cm, ldc, mixed, b, n);
+  sygvd_core(h, c, n, hm, ldh, sm, lds, x, n, w, cm, ldc, mixed, b, n, pad);
 }
 
 // ---- the geometry (S, U, X = U^-1) kept on the handle across the cycles of one geometry -----------------------------

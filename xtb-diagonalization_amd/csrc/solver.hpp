@@ -116,6 +116,16 @@ void sygvd_device(Handle& h, int n, const double* hm, int ldh, const double* sm,
                   int ldc, bool mixed, int b, const double** u_out = nullptr);
 // arena bytes of a generalized solve (m eigenvector columns)
 size_t sygvd_bytes(int n, int b, bool mixed, int m);
+// The generalized solve of sygvd_device (band b resolved and checked) on the workspace at base (sygvd_bytes(n, b,
+// mixed, n) bytes), with the handle's streams and status words; pad: see PadSpec.
+void sygvd_in(Handle& h, void* base, int n, const double* hm, int ldh, const double* sm, int lds, double* w,
+              double* cm, int ldc, bool mixed, int b, const double** u_out, const PadSpec* pad);
+// Grouped solves (grouped.hip): count problems padded to multiples of `granule` and solved group by group, every
+// kernel of a group's pipeline launched once over the whole group. status (count x NSTATUS ints, may be null): each
+// problem's status words; latency_ms (may be null): each problem's time to the end of its group.
+void sygvd_grouped(Handle& h, int count, const int* n, const double* const* hm, const int* ldh,
+                   const double* const* sm, const int* lds, double* const* w, double* const* cm, const int* ldc,
+                   bool mixed, int granule, int* status, float* latency_ms);
 void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz);
 // The cycle's generalized solve on the handle's geometry (set_geometry): all n eigenvalues, the first m eigenvectors.
 void sygvd_geometry(Handle& h, int n, int m, const double* hm, int ldh, double* w, double* cm, int ldc, bool mixed,

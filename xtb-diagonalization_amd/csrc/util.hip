@@ -8,7 +8,9 @@ namespace xtbeig {
 namespace {
 
 template <typename S, typename D>
-__global__ void copy_kernel(int m, int n, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd) {
+__global__ void copy_kernel(int m, int n, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd, Zb zb) {
+  src = zb.at(src);
+  dst = zb.at(dst);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int j = blockIdx.y; j < n; j += gridDim.y)
     if (i < m) dst[idx(i, j, ldd)] = D(src[idx(i, j, lds)]);
@@ -16,17 +18,20 @@ __global__ void copy_kernel(int m, int n, const S* __restrict__ src, int lds, D*
 
 // dst(i, j) = src(i, j) for i <= j (the upper triangle); dst's strictly lower triangle is not touched.
 template <typename T>
-__global__ void copy_upper_kernel(int n, const T* __restrict__ src, int lds, T* __restrict__ dst, int ldd) {
+__global__ void copy_upper_kernel(int n, const T* __restrict__ src, int lds, T* __restrict__ dst, int ldd, Zb zb) {
+  src = zb.at(src);
+  dst = zb.at(dst);
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (i <= j && i < n) dst[idx(i, j, ldd)] = src[idx(i, j, lds)];
 }
 
 // 32 x 32 tiles through LDS: tile (bi, bj) with bi <= bj reads the referenced triangle and writes both.
 template <typename T>
-__global__ void symmetrize_kernel(int n, T* a, int lda, bool upper) {
+__global__ void symmetrize_kernel(int n, T* a, int lda, bool upper, Zb zb) {
   __shared__ T tile[32][33];
   const int bi = blockIdx.x, bj = blockIdx.y;
   if (bi > bj) return;
+  a = zb.at(a);
   const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
   // Source tile: upper -> rows bi, cols bj; lower -> rows bj, cols bi (transposed role).
   for (int r = ty; r < 32; r += 8) {
@@ -46,7 +51,8 @@ __global__ void symmetrize_kernel(int n, T* a, int lda, bool upper) {
 }
 
 template <typename T>
-__global__ void fill_kernel(int m, int n, T* a, int lda, T diag) {
+__global__ void fill_kernel(int m, int n, T* a, int lda, T diag, Zb zb) {
+  a = zb.at(a);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int j = blockIdx.y; j < n; j += gridDim.y)
     if (i < m) a[idx(i, j, lda)] = (i == j) ? diag : T(0);
@@ -75,8 +81,11 @@ __global__ void symmetrize_pad_kernel(int n, const double* __restrict__ src, int
 
 // dst(j, i) = src(i, j) through 32 x 32 LDS tiles; src m x n, dst n x m.
 template <typename S, typename D>
-__global__ void transpose_kernel(int m, int n, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd) {
+__global__ void transpose_kernel(int m, int n, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd,
+                                 Zb zb) {
   __shared__ D tile[32][33];
+  src = zb.at(src);
+  dst = zb.at(dst);
   const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
   const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
   for (int r = ty; r < 32; r += 8) {
@@ -96,23 +105,24 @@ template <typename S, typename D>
 void transpose_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd) {
   if (m <= 0 || n <= 0) return;
   if (cdiv(n, 32) > 65535) throw std::runtime_error("transpose: matrix too wide");
-  hipLaunchKernelGGL((transpose_kernel<S, D>), dim3(cdiv(m, 32), cdiv(n, 32)), dim3(32, 8), 0, s, m, n, src, lds, dst,
-                     ldd);
+  hipLaunchKernelGGL((transpose_kernel<S, D>), dim3(cdiv(m, 32), cdiv(n, 32), zgrid()), dim3(32, 8), 0, s, m, n, src,
+                     lds, dst, ldd, zb());
   XTB_LAUNCH_CHECK();
 }
 
 template <typename S, typename D>
 void copy_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int ldd) {
   if (m <= 0 || n <= 0) return;
-  hipLaunchKernelGGL((copy_kernel<S, D>), dim3(cdiv(m, 256), n < 65535 ? n : 65535), dim3(256), 0, s, m, n, src,
-                     lds, dst, ldd);
+  hipLaunchKernelGGL((copy_kernel<S, D>), dim3(cdiv(m, 256), n < 65535 ? n : 65535, zgrid()), dim3(256), 0, s, m, n,
+                     src, lds, dst, ldd, zb());
   XTB_LAUNCH_CHECK();
 }
 
 template <typename T>
 void copy_upper(hipStream_t s, int n, const T* src, int lds, T* dst, int ldd) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(copy_upper_kernel<T>, dim3(cdiv(n, 256), n), dim3(256), 0, s, n, src, lds, dst, ldd);
+  hipLaunchKernelGGL(copy_upper_kernel<T>, dim3(cdiv(n, 256), n, zgrid()), dim3(256), 0, s, n, src, lds, dst, ldd,
+                     zb());
   XTB_LAUNCH_CHECK();
 }
 
@@ -120,21 +130,24 @@ template <typename T>
 void symmetrize(hipStream_t s, char uplo, int n, T* a, int lda) {
   if (n <= 1) return;
   const int nt = cdiv(n, 32);
-  hipLaunchKernelGGL(symmetrize_kernel<T>, dim3(nt, nt), dim3(32, 8), 0, s, n, a, lda, uplo == 'U' || uplo == 'u');
+  hipLaunchKernelGGL(symmetrize_kernel<T>, dim3(nt, nt, zgrid()), dim3(32, 8), 0, s, n, a, lda,
+                     uplo == 'U' || uplo == 'u', zb());
   XTB_LAUNCH_CHECK();
 }
 
 template <typename T>
 void set_zero(hipStream_t s, int m, int n, T* a, int lda) {
   if (m <= 0 || n <= 0) return;
-  hipLaunchKernelGGL(fill_kernel<T>, dim3(cdiv(m, 256), n < 65535 ? n : 65535), dim3(256), 0, s, m, n, a, lda, T(0));
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(cdiv(m, 256), n < 65535 ? n : 65535, zgrid()), dim3(256), 0, s, m, n, a,
+                     lda, T(0), zb());
   XTB_LAUNCH_CHECK();
 }
 
 template <typename T>
 void set_identity(hipStream_t s, int n, T* a, int lda) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(fill_kernel<T>, dim3(cdiv(n, 256), n < 65535 ? n : 65535), dim3(256), 0, s, n, n, a, lda, T(1));
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(cdiv(n, 256), n < 65535 ? n : 65535, zgrid()), dim3(256), 0, s, n, n, a,
+                     lda, T(1), zb());
   XTB_LAUNCH_CHECK();
 }
 

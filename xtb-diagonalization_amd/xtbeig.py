@@ -84,6 +84,9 @@ SIGNATURES = {
     "xtbeig_dev_sygvd_batched": (ci, [vp, ci, ci, c_int_p, ctypes.POINTER(vp), c_int_p, ctypes.POINTER(vp), c_int_p,
                                       ctypes.POINTER(vp), ctypes.POINTER(vp), c_int_p, ci, ci,
                                       ctypes.POINTER(ctypes.c_float)]),
+    "xtbeig_dev_sygvd_grouped": (ci, [vp, ci, ci, c_int_p, ctypes.POINTER(vp), c_int_p, ctypes.POINTER(vp), c_int_p,
+                                      ctypes.POINTER(vp), ctypes.POINTER(vp), c_int_p, ci, c_int_p,
+                                      ctypes.POINTER(ctypes.c_float)]),
     "xtbeig_autotune": (ci, [vp, ci, ci, ci, c_int_p, c_int_p, ctypes.POINTER(ctypes.c_float), c_int_p]),
     "xtbeig_get_band_table": (ci, [vp, ci, c_int_p, c_int_p]),
     "xtbeig_set_band_table": (ci, [vp, ci, c_int_p]),
@@ -529,6 +532,26 @@ class Handle:
         self._check(lib().xtbeig_dev_sygvd_batched(self._h, precision, cnt, ns, ph, ldh, ps, lds, pw, pc, ldc,
                                                    slots, 1 if use_graph else 0, lat))
         return [lat[q] for q in range(cnt)]
+
+    def sygvd_grouped(self, hs, ss, ws, cs, precision=FP64, granule=0):
+        """Lists of device tensors as sygvd_batched; every kernel runs once per group of equal padded size. Returns
+        (per-problem status words [count][4], per-problem latencies in ms)."""
+        cnt = len(hs)
+        arr_i = ctypes.c_int * max(cnt, 1)
+        arr_p = vp * max(cnt, 1)
+        ns = arr_i(*[t.shape[0] for t in hs])
+        ldh = arr_i(*[t.stride(0) for t in hs])
+        lds = arr_i(*[t.stride(0) for t in ss])
+        ldc = arr_i(*[t.stride(0) for t in cs])
+        ph = arr_p(*[t.data_ptr() for t in hs])
+        ps = arr_p(*[t.data_ptr() for t in ss])
+        pw = arr_p(*[t.data_ptr() for t in ws])
+        pc = arr_p(*[t.data_ptr() for t in cs])
+        st = (ctypes.c_int * (4 * max(cnt, 1)))()
+        lat = (ctypes.c_float * max(cnt, 1))()
+        self._check(lib().xtbeig_dev_sygvd_grouped(self._h, precision, cnt, ns, ph, ldh, ps, lds, pw, pc, ldc,
+                                                   granule, st, lat))
+        return [[st[4 * q + k] for k in range(4)] for q in range(cnt)], [lat[q] for q in range(cnt)]
 
     def autotune(self, precision, nmin, nmax):
         """Returns [(bucket_start_n, band, ms)] for the buckets tuned."""
