@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 namespace xtbeig {
 
@@ -57,13 +58,19 @@ __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 // Tables shared by every problem of the batch (the divide-and-conquer tree, the bulge back-transformation plan) are
 // not moved.
 struct Zb {
-  size_t stride;  // bytes between consecutive problems' slices
+  size_t stride;  // bytes between consecutive problems' slices (0: a single problem)
   int zin;        // the kernel's own z extent
-  __device__ inline int prob() const { return (int)blockIdx.z / zin; }
-  __device__ inline int z() const { return (int)blockIdx.z % zin; }
+  // Uniform branches keep the division off a single solve's path (these kernels are short and latency-bound).
+  __device__ inline int prob() const { return stride == 0 ? 0 : zin == 1 ? (int)blockIdx.z : (int)blockIdx.z / zin; }
+  __device__ inline int z() const { return (stride == 0 || zin == 1) ? (zin == 1 ? 0 : (int)blockIdx.z)
+                                                                      : (int)blockIdx.z % zin; }
+  // byte arithmetic on the pointer itself (no round trip through an integer), so that the compiler still knows a
+  // kernel argument points to global memory and emits global rather than flat loads and stores
   template <typename P>
   __device__ inline P* at(P* p) const {
-    return p ? (P*)((uintptr_t)p + (size_t)prob() * stride) : p;
+    using Byte = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+    if (stride == 0) return p;
+    return p ? reinterpret_cast<P*>(reinterpret_cast<Byte*>(p) + (size_t)prob() * stride) : p;
   }
 };
 // The batch that the calling host thread is enqueuing (count 1, stride 0 outside a BatchScope).
