@@ -249,15 +249,18 @@ int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, c
  * `granule` (0: 64; 1: no padding) with an exactly decoupled block (H: a diagonal inside the problem's spectral
  * range, S: the identity) that the results leave out, problems of one padded size solved together, every kernel of
  * the solve launched once over the whole group (the problem in the grid's z extent, workspaces in equal slices of
- * the handle's arena). Results as xtbeig_dev_sygvd's for each problem. Waits for completion. status (count x 4 ints,
- * may be NULL): each problem's status words as xtbeig_synchronize reports them (word 3 = 2: the padded block did
- * not separate; the problem is not solved), also merged into the handle's; latency_ms (count, may be NULL): each
- * problem's time from the call's start to the end of its group. The handle's workspace grows to the largest group's
- * slices (a group is split so that they fit 60 % of free device memory). For batches of independent problems
- * (256 of n in [124, 1204], say); no reference counterpart (the reference solves one problem per thread). */
+ * an arena). Groups (or chunks of a large group) run concurrently on up to `slots` sub-handle streams (0: 4).
+ * Results as xtbeig_dev_sygvd's for each problem (bit-identical to it with granule 1). Waits for completion. status
+ * (count x 4 ints, may be NULL): each problem's status words as xtbeig_synchronize reports them (word 3 = 2: the
+ * padded block did not separate; the problem is not solved), also merged into the handle's; latency_ms (count, may
+ * be NULL): each problem's time from the call's start to the end of its chunk. The sub-handles' workspaces grow to
+ * their largest chunks' slices (chunks are cut to fit 60 % of free device memory between the slots). For batches
+ * of independent problems (256 of n in [124, 1204], say); no reference counterpart (the reference solves one
+ * problem per thread). */
 int xtbeig_dev_sygvd_grouped(xtbeig_handle_t handle, int precision, int count, const int *n, const double *const *h,
                              const int *ldh, const double *const *s, const int *lds, double *const *w,
-                             double *const *c, const int *ldc, int granule, int *status, float *latency_ms);
+                             double *const *c, const int *ldc, int granule, int slots, int *status,
+                             float *latency_ms);
 /* Autotuner: first the GEMM tile widths over [nmin, nmax] (xtbeig_autotune_gemm, FP64, and FP32 too for precision
  * XTBEIG_FP32_REFINE), then for every n bucket of the handle's table that meets [nmin, nmax] (2 <= nmin <= nmax <=
  * 2048), the candidate band widths (also the panel width of the dense-to-band reduction) 8, 16 and 32 timed on a

@@ -3,12 +3,12 @@
 every solve gated against scipy.
 
 Two APIs: `grouped` (xtbeig_dev_sygvd_grouped: problems padded to a multiple of --granule and solved group by group,
-every kernel launched once per group) and `streams` (xtbeig_dev_sygvd_batched: whole solves on --slots concurrent
-sub-handle streams, band widths from the autotuner). --uniform N solves --count copies of the n = N workload of the
-single-solve configuration (seed 1001) instead.
+every kernel launched once per group, groups on --slots concurrent streams) and `streams` (xtbeig_dev_sygvd_batched:
+whole solves on --slots concurrent sub-handle streams, band widths from the autotuner). --uniform N solves --count
+copies of the n = N workload of the single-solve configuration (seed 1001) instead.
 
 Usage: python xtb-diagonalization_amd/batched_bench.py [--api grouped|streams] [--count 256] [--granule 64]
-                                                       [--slots 8] [--uniform N] [--check K] [--reps 2]
+                                                       [--slots S] [--uniform N] [--check K] [--reps 2]
 Prints one JSON line: solves/s, p50 / p99 per-solve latency and the nominal rate (7 n^3 flop per solve) per
 precision, and whether every checked problem passed the FP64 gates.
 """
@@ -26,7 +26,8 @@ def main():
     ap.add_argument("--api", choices=("grouped", "streams"), default="grouped")
     ap.add_argument("--count", type=int, default=256)
     ap.add_argument("--granule", type=int, default=64)
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--slots", type=int, default=0,
+                    help="concurrent streams (0: the API's default, 4 grouped / 8 streams)")
     ap.add_argument("--graph", action="store_true", help="streams API: each solve through its slot's graph")
     ap.add_argument("--uniform", type=int, default=0, help="every problem the n = N single-solve workload")
     ap.add_argument("--reps", type=int, default=2, help="timed passes after the warm-up pass (the fastest kept)")
@@ -67,7 +68,7 @@ def main():
     groups = len({((n + gran - 1) // gran) * gran if gran > 1 else n for n in ns})
     out = {"config": {"workload": workload, "api": args.api, "flops_nominal": flops,
                       "granule": gran if args.api == "grouped" else None, "groups": groups if args.api == "grouped"
-                      else None, "slots": args.slots if args.api == "streams" else None,
+                      else None, "slots": args.slots,
                       "timing": "host wall clock around the call, inputs resident in HBM, fastest of the timed passes"},
            "generate_s": gen_s}
     precs = {"fp64": xtbeig.FP64, "fp32_refine": xtbeig.FP32_REFINE}
@@ -77,11 +78,11 @@ def main():
 
         def run():
             if args.api == "grouped":
-                st, lat = hnd.sygvd_grouped(hs, ss, ws, cs, prec, granule=gran)
+                st, lat = hnd.sygvd_grouped(hs, ss, ws, cs, prec, granule=gran, slots=args.slots)
                 if any(x != [0, 0, 0, 0] for x in st):
                     raise RuntimeError(f"grouped status {[x for x in st if x != [0, 0, 0, 0]][:4]}")
                 return lat
-            lat = hnd.sygvd_batched(hs, ss, ws, cs, prec, slots=args.slots, use_graph=args.graph)
+            lat = hnd.sygvd_batched(hs, ss, ws, cs, prec, slots=args.slots or 8, use_graph=args.graph)
             return lat
 
         run()  # warm-up pass: workspaces, plans

@@ -110,22 +110,22 @@ void sygvd_graph(Handle& h, int n, const double* hm, int ldh, const double* sm, 
                            hipMemcpyDeviceToDevice, st));
 }
 
-void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, const int* ldh,
-                   const double* const* sm, const int* lds, double* const* w, double* const* cm, const int* ldc,
-                   bool mixed, int slots, bool use_graph, float* latency_ms) {
-  if (count <= 0) return;
-  slots = std::max(1, std::min(slots, count));
-  while ((int)h.subs.size() < slots) {
-    h.subs.push_back(std::make_unique<Handle>(h.device));
-    Handle& sub = *h.subs.back();
-    sub.refine_iters = h.refine_iters;
-  }
+void ensure_subs(Handle& h, int slots) {
+  while ((int)h.subs.size() < slots) h.subs.push_back(std::make_unique<Handle>(h.device));
   for (auto& sub : h.subs) {
     sub->band_override = h.band_override;
     sub->refine_iters = h.refine_iters;
     sub->test_secular_maxit = h.test_secular_maxit;
     std::copy(&h.tuned_band[0][0], &h.tuned_band[0][0] + 2 * Handle::NBUCKET, &sub->tuned_band[0][0]);
   }
+}
+
+void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, const int* ldh,
+                   const double* const* sm, const int* lds, double* const* w, double* const* cm, const int* ldc,
+                   bool mixed, int slots, bool use_graph, float* latency_ms) {
+  if (count <= 0) return;
+  slots = std::max(1, std::min(slots, count));
+  ensure_subs(h, slots);
   // longest first onto the least-loaded slot
   std::vector<int> order(count);
   std::iota(order.begin(), order.end(), 0);

@@ -1023,18 +1023,19 @@ int xtbeig_dev_sygvd_batched(xtbeig_handle_t handle, int precision, int count, c
 
 int xtbeig_dev_sygvd_grouped(xtbeig_handle_t handle, int precision, int count, const int* n, const double* const* hm,
                              const int* ldh, const double* const* sm, const int* lds, double* const* w,
-                             double* const* c, const int* ldc, int granule, int* status, float* latency_ms) {
+                             double* const* c, const int* ldc, int granule, int slots, int* status,
+                             float* latency_ms) {
   if (!handle) return set_error("xtbeig_dev_sygvd_grouped: null handle");
   if (precision != XTBEIG_FP64 && precision != XTBEIG_FP32_REFINE)
     return set_error("xtbeig_dev_sygvd_grouped: unknown precision");
-  if (count < 0 || granule < 0 || granule > 1024)
-    return set_error("xtbeig_dev_sygvd_grouped: count >= 0, 0 <= granule <= 1024");
+  if (count < 0 || granule < 0 || granule > 1024 || slots < 0 || slots > 64)
+    return set_error("xtbeig_dev_sygvd_grouped: count >= 0, 0 <= granule <= 1024, 0 <= slots <= 64");
   for (int q = 0; q < count; ++q)
     if (n[q] < 1 || ldh[q] < n[q] || lds[q] < n[q] || ldc[q] < n[q])
       return set_error("xtbeig_dev_sygvd_grouped: bad dimensions");
   return guarded([&] {
     xtbeig::sygvd_grouped(handle->h, count, n, hm, ldh, sm, lds, w, c, ldc, precision == XTBEIG_FP32_REFINE, granule,
-                          status, latency_ms);
+                          slots, status, latency_ms);
   });
 }
 

@@ -18,9 +18,14 @@
 // columns that vanish in the padded rows are the problem's own, kept in their (ascending) order; a count that is not
 // n is reported in status word 3 (value 2).
 //
-// Limits. A slice holds the padded H, S, w, C and a full solve's workspace; a chunk of a group is sized so that its
-// slices fit the memory budget (a fraction of the device's free memory) and so that no launch's grid z extent
-// (the problem count times the kernel's own z extent, at most twice the widest D&C level) exceeds 65535.
+// Concurrency. The stages of a small group leave most of the GPU idle (the chase of 5 problems runs on 5 CUs, a D&C
+// level's kernels are short), so groups run at once on up to `slots` sub-handles with their own streams and arenas:
+// chunks go most expensive first (cost count x N^3) to the least-loaded slot, and the host enqueues them round robin
+// so that every stream starts early.
+//
+// Limits. A slice holds the padded H, S, w, C and a full solve's workspace; a chunk is sized so that its slices fit
+// the slot's share of the memory budget (a fraction of the device's free memory) and so that no launch's grid z
+// extent (the problem count times the kernel's own z extent, at most twice the widest D&C level) exceeds 65535.
 #include "common.hpp"
 #include "kernels.hpp"
 #include "solver.hpp"
@@ -195,7 +200,7 @@ void pad_spectrum(hipStream_t s, int n, const int* nvec, double* hp, int ldhp, d
 
 void sygvd_grouped(Handle& h, int count, const int* n, const double* const* hm, const int* ldh,
                    const double* const* sm, const int* lds, double* const* w, double* const* cm, const int* ldc,
-                   bool mixed, int granule, int* status, float* latency_ms) {
+                   bool mixed, int granule, int slots, int* status, float* latency_ms) {
   if (count <= 0) return;
   if (granule <= 0) granule = 64;
   // padded size per problem: n rounded up to the granule, unless that passes the size limit
@@ -206,21 +211,26 @@ void sygvd_grouped(Handle& h, int count, const int* n, const double* const* hm, 
     if (N - bN > MAX_N) N = n[q];
     groups[N].push_back(q);
   }
+  slots = std::max(1, std::min(slots <= 0 ? 4 : slots, 64));
   size_t free_b = 0, total_b = 0;
   XTB_HIP(hipSetDevice(h.device));
   XTB_HIP(hipMemGetInfo(&free_b, &total_b));
-  const size_t budget = std::max<size_t>(std::min<size_t>((size_t)(0.6 * (double)(free_b + h.arena_bytes)),
-                                                          size_t(64) << 30),
+  size_t held = h.arena_bytes;
+  for (auto& sub : h.subs) held += sub->arena_bytes;
+  const size_t budget = std::max<size_t>(std::min<size_t>((size_t)(0.6 * (double)(free_b + held)), size_t(64) << 30),
                                          size_t(1) << 30);
   struct Chunk {
     int N, b, first, count;
     SliceLayout lay;
     bool padded;
+    double cost;
   };
   std::vector<Chunk> chunks;
   std::vector<ProbIO> tab;
   std::vector<int> nv, order;
-  size_t need = 0;
+  // a group is cut into equal chunks when its slices exceed a slot's share of the memory budget or its launches the
+  // grid's z limit
+  const size_t slot_budget = std::max<size_t>(budget / slots, size_t(256) << 20);
   for (auto& g : groups) {
     const int N = g.first;
     const std::vector<int>& list = g.second;
@@ -230,13 +240,14 @@ void sygvd_grouped(Handle& h, int count, const int* n, const double* const* hm, 
       throw std::runtime_error("grouped solve: n = " + std::to_string(N) + " exceeds the largest supported size");
     const SliceLayout lay(N, b, mixed);
     const DcPlan& plan = h.plan(N);
-    (void)h.backplan(N, b);
     int zmax = 64;  // the largest own z extent of a launch: the D&C merge products (two per merge of a level)
     for (const auto& lv : plan.lvls) zmax = std::max(zmax, 2 * lv.count);
-    const int cap = (int)std::max<size_t>(1, std::min<size_t>(budget / lay.bytes, (size_t)(65535 / zmax)));
-    for (size_t at = 0; at < list.size(); at += cap) {
-      const int cnt = (int)std::min<size_t>(cap, list.size() - at);
-      Chunk ch{N, b, (int)tab.size(), cnt, lay, false};
+    const int cap = (int)std::max<size_t>(1, std::min<size_t>(slot_budget / lay.bytes, (size_t)(65535 / zmax)));
+    const int pieces = std::max(1, cdiv((int)list.size(), cap));
+    const int per = cdiv((int)list.size(), pieces);
+    for (size_t at = 0; at < list.size(); at += per) {
+      const int cnt = (int)std::min<size_t>(per, list.size() - at);
+      Chunk ch{N, b, (int)tab.size(), cnt, lay, false, (double)cnt * N * N * N};
       for (int k = 0; k < cnt; ++k) {
         const int q = list[at + k];
         tab.push_back(ProbIO{hm[q], sm[q], w[q], cm[q], ldh[q], lds[q], ldc[q], n[q]});
@@ -245,10 +256,38 @@ void sygvd_grouped(Handle& h, int count, const int* n, const double* const* hm, 
         ch.padded |= n[q] < N;
       }
       chunks.push_back(ch);
-      need = std::max(need, (size_t)cnt * lay.bytes);
     }
   }
-  h.reserve(need);
+  // chunks onto slots, most expensive first onto the least-loaded slot; one slot: the handle itself
+  slots = std::min<int>(slots, (int)chunks.size());
+  std::vector<Handle*> hs;
+  if (slots == 1) {
+    hs.push_back(&h);
+  } else {
+    ensure_subs(h, slots);
+    for (int sl = 0; sl < slots; ++sl) hs.push_back(h.subs[sl].get());
+  }
+  std::vector<int> by_cost(chunks.size());
+  for (size_t ci = 0; ci < chunks.size(); ++ci) by_cost[ci] = (int)ci;
+  std::stable_sort(by_cost.begin(), by_cost.end(), [&](int a, int b) { return chunks[a].cost > chunks[b].cost; });
+  std::vector<std::vector<int>> lists(slots);
+  std::vector<double> load(slots, 0.0);
+  for (int ci : by_cost) {
+    const int sl = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lists[sl].push_back(ci);
+    load[sl] += chunks[ci].cost;
+  }
+  // every slot's arena and plans before anything is enqueued
+  for (int sl = 0; sl < slots; ++sl) {
+    size_t need = 0;
+    for (int ci : lists[sl]) {
+      const Chunk& ch = chunks[ci];
+      need = std::max(need, (size_t)ch.count * ch.lay.bytes);
+      (void)hs[sl]->plan(ch.N);
+      (void)hs[sl]->backplan(ch.N, ch.b);
+    }
+    hs[sl]->reserve(need);
+  }
   // device tables: the problems' buffers and sizes, then the status words (count x NSTATUS), in chunk order
   const size_t tab_bytes = (sizeof(ProbIO) * tab.size() + 255) & ~size_t(255);
   const size_t nv_bytes = (sizeof(int) * nv.size() + 255) & ~size_t(255);
@@ -263,44 +302,62 @@ void sygvd_grouped(Handle& h, int count, const int* n, const double* const* hm, 
   std::copy(nv.begin(), nv.end(), reinterpret_cast<int*>(stage + tab_bytes));
   hipStream_t st = h.stream;
   XTB_HIP(hipMemcpyAsync(aux, stage, tab_bytes + nv_bytes, hipMemcpyHostToDevice, st));
-  std::vector<hipEvent_t> ev(chunks.size() + 1, nullptr);
+  std::vector<hipEvent_t> ev(chunks.size() + 1 + slots, nullptr);
   for (auto& e : ev) XTB_HIP(hipEventCreate(&e));
+  hipEvent_t* done = ev.data() + chunks.size() + 1;
   XTB_HIP(hipEventRecord(ev[0], st));
-  char* base = static_cast<char*>(h.arena);
-  int* const saved_dinfo = h.dinfo;
-  try {
-    for (size_t ci = 0; ci < chunks.size(); ++ci) {
-      const Chunk& ch = chunks[ci];
-      const int N = ch.N;
-      const SliceLayout& L = ch.lay;
-      const ProbIO* io = tab_dev + ch.first;
-      hipLaunchKernelGGL(group_pack_kernel, dim3(cdiv(N, 256), N, ch.count), dim3(256), 0, st, N, io, base, L.bytes,
-                         L.h, L.s);
-      XTB_LAUNCH_CHECK();
-      {
-        BatchScope scope(ch.count, L.bytes);
-        h.dinfo = reinterpret_cast<int*>(base);
-        double* H = reinterpret_cast<double*>(base + L.h);
-        const PadSpec pad{nv_dev + ch.first, H, N};
-        sygvd_in(h, base + L.arena, N, H, N, reinterpret_cast<double*>(base + L.s), N,
+  for (int sl = 0; sl < slots; ++sl)
+    if (hs[sl] != &h) XTB_HIP(hipStreamWaitEvent(hs[sl]->stream, ev[0], 0));
+  auto enqueue = [&](Handle& hh, int ci) {
+    const Chunk& ch = chunks[ci];
+    const int N = ch.N;
+    const SliceLayout& L = ch.lay;
+    const ProbIO* io = tab_dev + ch.first;
+    hipStream_t s = hh.stream;
+    char* base = static_cast<char*>(hh.arena);
+    hipLaunchKernelGGL(group_pack_kernel, dim3(cdiv(N, 256), N, ch.count), dim3(256), 0, s, N, io, base, L.bytes, L.h,
+                       L.s);
+    XTB_LAUNCH_CHECK();
+    {
+      BatchScope scope(ch.count, L.bytes);
+      int* const keep = hh.dinfo;
+      hh.dinfo = reinterpret_cast<int*>(base);
+      double* H = reinterpret_cast<double*>(base + L.h);
+      const PadSpec pad{nv_dev + ch.first, H, N};
+      try {
+        sygvd_in(hh, base + L.arena, N, H, N, reinterpret_cast<double*>(base + L.s), N,
                  reinterpret_cast<double*>(base + L.w), reinterpret_cast<double*>(base + L.c), N, mixed, ch.b,
                  nullptr, ch.padded ? &pad : nullptr);
-        h.dinfo = saved_dinfo;
+      } catch (...) {
+        hh.dinfo = keep;
+        throw;
       }
-      hipLaunchKernelGGL(group_select_kernel, dim3(1, 1, ch.count), dim3(256), 0, st, N, io, base, L.bytes, L.c,
-                         L.pos);
-      XTB_LAUNCH_CHECK();
-      hipLaunchKernelGGL(group_unpack_kernel, dim3(cdiv(N, 256), N, ch.count), dim3(256), 0, st, N, io, base,
-                         L.bytes, L.w, L.c, L.pos);
-      XTB_LAUNCH_CHECK();
-      hipLaunchKernelGGL(group_status_kernel, dim3(cdiv(ch.count * NSTATUS, 256)), dim3(256), 0, st, ch.count, base,
-                         L.bytes, st_dev + (size_t)NSTATUS * ch.first);
-      XTB_LAUNCH_CHECK();
-      XTB_HIP(hipEventRecord(ev[ci + 1], st));
+      hh.dinfo = keep;
     }
+    hipLaunchKernelGGL(group_select_kernel, dim3(1, 1, ch.count), dim3(256), 0, s, N, io, base, L.bytes, L.c, L.pos);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(group_unpack_kernel, dim3(cdiv(N, 256), N, ch.count), dim3(256), 0, s, N, io, base, L.bytes,
+                       L.w, L.c, L.pos);
+    XTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(group_status_kernel, dim3(cdiv(ch.count * NSTATUS, 256)), dim3(256), 0, s, ch.count, base,
+                       L.bytes, st_dev + (size_t)NSTATUS * ch.first);
+    XTB_LAUNCH_CHECK();
+    XTB_HIP(hipEventRecord(ev[ci + 1], s));
+  };
+  try {
+    // round robin over the slots, so that every slot starts while the host is still enqueueing
+    size_t longest = 0;
+    for (auto& l : lists) longest = std::max(longest, l.size());
+    for (size_t pos = 0; pos < longest; ++pos)
+      for (int sl = 0; sl < slots; ++sl)
+        if (pos < lists[sl].size()) enqueue(*hs[sl], lists[sl][pos]);
+    for (int sl = 0; sl < slots; ++sl)
+      if (hs[sl] != &h) {
+        XTB_HIP(hipEventRecord(done[sl], hs[sl]->stream));
+        XTB_HIP(hipStreamWaitEvent(st, done[sl], 0));
+      }
   } catch (...) {
-    h.dinfo = saved_dinfo;
-    (void)hipStreamSynchronize(st);
+    for (int sl = 0; sl < slots; ++sl) (void)hipStreamSynchronize(hs[sl]->stream);
     for (auto& e : ev) (void)hipEventDestroy(e);
     throw;
   }

@@ -123,9 +123,10 @@ void sygvd_in(Handle& h, void* base, int n, const double* hm, int ldh, const dou
 // Grouped solves (grouped.hip): count problems padded to multiples of `granule` and solved group by group, every
 // kernel of a group's pipeline launched once over the whole group. status (count x NSTATUS ints, may be null): each
 // problem's status words; latency_ms (may be null): each problem's time to the end of its group.
+// slots: groups (chunks of a group) run concurrently on up to this many sub-handle streams.
 void sygvd_grouped(Handle& h, int count, const int* n, const double* const* hm, const int* ldh,
                    const double* const* sm, const int* lds, double* const* w, double* const* cm, const int* ldc,
-                   bool mixed, int granule, int* status, float* latency_ms);
+                   bool mixed, int granule, int slots, int* status, float* latency_ms);
 void stedc_device(Handle& h, int n, double* d, double* e, double* z, int ldz);
 // The cycle's generalized solve on the handle's geometry (set_geometry): all n eigenvalues, the first m eigenvectors.
 void sygvd_geometry(Handle& h, int n, int m, const double* hm, int ldh, double* w, double* cm, int ldc, bool mixed,
@@ -142,6 +143,8 @@ void sygvd_graph(Handle& h, int n, const double* hm, int ldh, const double* sm, 
 void sygvd_batched(Handle& h, int count, const int* n, const double* const* hm, const int* ldh,
                    const double* const* sm, const int* lds, double* const* w, double* const* cm, const int* ldc,
                    bool mixed, int slots, bool use_graph, float* latency_ms);
+// h.subs grown to at least `slots` sub-handles, each given h's band, refinement and test settings.
+void ensure_subs(Handle& h, int slots);
 void autotune_bands(Handle& h, bool mixed, int nmin, int nmax, int* buckets_out, int* bands_out, float* ms_out);
 
 }  // namespace xtbeig

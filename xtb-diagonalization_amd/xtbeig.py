@@ -85,7 +85,7 @@ SIGNATURES = {
                                       ctypes.POINTER(vp), ctypes.POINTER(vp), c_int_p, ci, ci,
                                       ctypes.POINTER(ctypes.c_float)]),
     "xtbeig_dev_sygvd_grouped": (ci, [vp, ci, ci, c_int_p, ctypes.POINTER(vp), c_int_p, ctypes.POINTER(vp), c_int_p,
-                                      ctypes.POINTER(vp), ctypes.POINTER(vp), c_int_p, ci, c_int_p,
+                                      ctypes.POINTER(vp), ctypes.POINTER(vp), c_int_p, ci, ci, c_int_p,
                                       ctypes.POINTER(ctypes.c_float)]),
     "xtbeig_autotune": (ci, [vp, ci, ci, ci, c_int_p, c_int_p, ctypes.POINTER(ctypes.c_float), c_int_p]),
     "xtbeig_get_band_table": (ci, [vp, ci, c_int_p, c_int_p]),
@@ -533,7 +533,7 @@ class Handle:
                                                    slots, 1 if use_graph else 0, lat))
         return [lat[q] for q in range(cnt)]
 
-    def sygvd_grouped(self, hs, ss, ws, cs, precision=FP64, granule=0):
+    def sygvd_grouped(self, hs, ss, ws, cs, precision=FP64, granule=0, slots=0):
         """Lists of device tensors as sygvd_batched; every kernel runs once per group of equal padded size. Returns
         (per-problem status words [count][4], per-problem latencies in ms)."""
         cnt = len(hs)
@@ -550,7 +550,7 @@ class Handle:
         st = (ctypes.c_int * (4 * max(cnt, 1)))()
         lat = (ctypes.c_float * max(cnt, 1))()
         self._check(lib().xtbeig_dev_sygvd_grouped(self._h, precision, cnt, ns, ph, ldh, ps, lds, pw, pc, ldc,
-                                                   granule, st, lat))
+                                                   granule, slots, st, lat))
         return [[st[4 * q + k] for k in range(4)] for q in range(cnt)], [lat[q] for q in range(cnt)]
 
     def autotune(self, precision, nmin, nmax):
