@@ -6,6 +6,9 @@ resident in HBM. The FP64 path and the FP32+refine path are each run W untimed +
 stream, both checked against the host-LAPACK FP64 oracle (eigenvalues within 1e-12 ||H||_2). `value` is the
 FP32+refine rate; the FP64 rate and their ratio are reported beside it.
 
+Throughput beside it (not `value`): 32 solves of the same workload in flight through the stream-batched call
+(throughput_concurrent) and through the grouped call, every kernel launched once for all 32 (throughput_grouped).
+
 roofline: the dominant kernel of the FP32+refine solve (largest share of its time in the kernel trace) timed alone
 with HIP events on the handle's stream, against its bound (HBM bytes or MFMA flops per launch).
 cpu_baseline: the oracle (C restatement of the reference's 1-stage reduction and gemm back-transformation, with the
@@ -88,6 +91,36 @@ def concurrent_throughput(torch, np, workloads, xtbeig, hnd, h, s, wref, count=3
         out[name] = {"solves_per_s": count / best, "wall_ms": best * 1e3, "checked_pass": ok}
         log(f"throughput {name}", out[name])
     out["config"] = {"solves_in_flight": count, "slots": slots, "api": "xtbeig_dev_sygvd_batched",
+                     "timing": "host wall clock around the call, inputs resident in HBM"}
+    return out
+
+
+def grouped_throughput(torch, np, workloads, xtbeig, hnd, h, s, wref, count=32):
+    """`count` solves of the workload through xtbeig_dev_sygvd_grouped (no padding: one group, every kernel of the
+    solve launched once for all of them), one warm-up pass then two timed, the rate of the faster; every solve's status
+    words checked and the first and last solve gated against the oracle. Returns {precision: {...}}."""
+    n = h.shape[0]
+    hd = torch.from_numpy(np.ascontiguousarray(h.T)).cuda()
+    sd = torch.from_numpy(np.ascontiguousarray(s.T)).cuda()
+    ws = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(count)]
+    cs = [torch.zeros((n, n), dtype=torch.float64, device="cuda") for _ in range(count)]
+    out = {}
+    for name, prec in (("fp64", xtbeig.FP64), ("fp32_refine", xtbeig.FP32_REFINE)):
+        best = None
+        for rep in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st, _ = hnd.sygvd_grouped([hd] * count, [sd] * count, ws, cs, prec, granule=1)
+            dt = time.perf_counter() - t0
+            if any(x != [0, 0, 0, 0] for x in st):
+                raise RuntimeError(f"grouped status {st}")
+            if rep:
+                best = dt if best is None else min(best, dt)
+        ok = all(workloads.passes(workloads.gates(h, s, ws[q].cpu().numpy(), cs[q].cpu().numpy().T, wref))
+                 for q in (0, count - 1))
+        out[name] = {"solves_per_s": count / best, "wall_ms": best * 1e3, "checked_pass": ok}
+        log(f"grouped throughput {name}", out[name])
+    out["config"] = {"solves_per_call": count, "api": "xtbeig_dev_sygvd_grouped", "granule": 1,
                      "timing": "host wall clock around the call, inputs resident in HBM"}
     return out
 
@@ -356,6 +389,22 @@ def rocprof_summary(path, top=40):
             if r.get("Start_Timestamp"):
                 dur[key] = float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
         names = sorted({c for v in per.values() for c in v})
+        # the GEMM-shaped kernels together (every product of the solve: plain, upper-tile, split-K, thin and the
+        # D&C merge products): MFMA flops over their summed durations
+        # (each precision over the kernels that run its MFMA instructions)
+        parts = []
+        for prec, peak in (("F64", MI355X_FP64_TFLOPS), ("F32", MI355X_FP32_TFLOPS)):
+            ns, fl = 0.0, 0.0
+            for key, v in per.items():
+                f = v.get(f"SQ_INSTS_VALU_MFMA_MOPS_{prec}", 0.0) * 512
+                if "gemm" in key[0] and dur.get(key) and f > 0:
+                    ns += dur[key]
+                    fl += f
+            if ns > 0:
+                parts.append(f"{prec}: {fl / 1e9:.1f} GFLOP in {ns / 1e6:.2f} ms, {fl / ns / 1e3:.1f} TF/s = "
+                             f"{fl / ns / 1e3 / peak:.1%} of peak")
+        if parts:
+            out.append("\nGEMM-shaped kernels together: " + "; ".join(parts))
         for by_grid, title, rows in ((False, "per kernel", top), (True, "per kernel and grid size", top // 2)):
             agg = collections.defaultdict(list)
             for key, v in per.items():
@@ -630,6 +679,9 @@ def main():
     log("concurrent throughput ...")
     thru = concurrent_throughput(torch, np, workloads, xtbeig, hnd, h, s, wref)
     thru["ratio_fp32_refine_vs_fp64"] = thru["fp32_refine"]["solves_per_s"] / thru["fp64"]["solves_per_s"]
+    log("grouped throughput ...")
+    grouped = grouped_throughput(torch, np, workloads, xtbeig, hnd, h, s, wref)
+    grouped["ratio_fp32_refine_vs_fp64"] = grouped["fp32_refine"]["solves_per_s"] / grouped["fp64"]["solves_per_s"]
 
     log("kernel roofline ...")
     roof = kernel_roofline(torch, xtbeig, hnd, h, s, band)
@@ -686,6 +738,7 @@ def main():
         "fp32_before_refinement": {"gates": unrefined, "pass": workloads.passes(unrefined)},
         "speedup_fp32_refine_vs_fp64": fp64["ms"] / mixed["ms"],
         "throughput_concurrent": thru,
+        "throughput_grouped": grouped,
         "nominal_flop_rate_TFLOPs": {"fp64": flops / (fp64["ms"] * 1e-3) / 1e12,
                                      "fp32_refine": flops / (mixed["ms"] * 1e-3) / 1e12},
         "roofline": {k: roof[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},

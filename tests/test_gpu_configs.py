@@ -3,8 +3,9 @@
   * configuration 2: the 20-step SCC-like sequence H_k = H_0 + D_k at n = 1788 (||D_k||_F from 1e-1 down to 1e-7,
     FP32 + refinement while ||D_k||_F >= 1e-3, then FP64), band 32 and the built-in band, every step's eigenpairs
     against scipy.linalg.eigh with the three gates;
-  * configuration 3: 256 generalized problems, n log-uniform in [124, 1204] (seed 1003), through the batched API,
-    once in FP64 and once in FP32 + refinement, every solve against scipy.linalg.eigh;
+  * configuration 3: 256 generalized problems, n log-uniform in [124, 1204] (seed 1003), through the stream-batched
+    API and through the grouped API (every kernel launched once per group of equal padded size), each once in FP64
+    and once in FP32 + refinement, every solve against scipy.linalg.eigh;
   * configuration 4: the density-matrix backward for 16 problems at n = 588 and 16 at n = 1204 (the spectrum with 8
     near-degenerate pairs, gaps 1e-10 .. 1e-4), kT = 9.5e-4 and 3.17e-3 Eh, mu bisected to trace(P S) = n / 2,
     upstream gradient random symmetric N(0, 1): every problem against an FP64 CPU reference gradient built on
@@ -94,6 +95,33 @@ def test_config3_batched_256(torch_cuda, prec):
             for k in worst:
                 worst[k] = max(worst[k], g[k])
         print("worst gates over 256 problems", worst, flush=True)
+    finally:
+        hnd.close()
+
+
+@pytest.mark.parametrize("prec", [xtbeig.FP64, xtbeig.FP32_REFINE])
+def test_config3_grouped_256(torch_cuda, prec):
+    """The same 256 problems through xtbeig_dev_sygvd_grouped: padded to multiples of 64 (18 groups), every kernel
+    launched once per group, groups on four streams; every problem's status words zero and its gates passed."""
+    torch = torch_cuda
+    ns, probs, refs = config3_problems()
+    hs = [dev(torch, h) for h, _ in probs]
+    ss = [dev(torch, s) for _, s in probs]
+    ws = [torch.zeros(n, dtype=torch.float64, device="cuda") for n in ns]
+    cs = [torch.zeros((n, n), dtype=torch.float64, device="cuda") for n in ns]
+    hnd = xtbeig.Handle(0)
+    try:
+        st, lat = hnd.sygvd_grouped(hs, ss, ws, cs, prec, granule=64, slots=4)
+        assert all(x == [0, 0, 0, 0] for x in st), [q for q, x in enumerate(st) if x != [0, 0, 0, 0]]
+        assert hnd.synchronize() == [0, 0, 0, 0]
+        assert len(lat) == 256 and all(x > 0 for x in lat)
+        worst = {"eig_gate": 0.0, "residual_gate": 0.0, "orth_gate": 0.0}
+        for q, (h, s) in enumerate(probs):
+            g = workloads.gates(h, s, ws[q].cpu().numpy(), cs[q].cpu().numpy().T, refs[q])
+            assert workloads.passes(g), (q, ns[q], g)
+            for k in worst:
+                worst[k] = max(worst[k], g[k])
+        print("worst gates over 256 grouped problems", worst, flush=True)
     finally:
         hnd.close()
 
