@@ -10,12 +10,15 @@
 // copied back, so the caller sees the dsytrd('U') result.
 //
 // One column step of a panel (column c, 0-based; W column wc; the panel's later columns c + 1 ..< nc already done)
-// is five launches, each over many workgroups with deterministic partial sums:
-//   trd_colupd   A(0:c+1, c) -= A(:, c+1:nc) W(c, wc+1:)^T + W(:, wc+1:) A(c, c+1:nc)^T; partial |x|^2
-//   trd_symv     larfg from the partials (every workgroup, redundantly), v in LDS; y = A(0:c, 0:c) v
-//   trd_dots     u = W(0:c, wc+1:)^T v, u2 = A(0:c, c+1:nc)^T v (one workgroup per dot)
+// is three launches, each over many workgroups with deterministic partial sums:
+//   trd_colupd   the previous column's w += -tau/2 (w^T v) v (its alpha from the partials, in every workgroup),
+//                then A(0:c+1, c) -= A(:, c+1:nc) W(c, wc+1:)^T + W(:, wc+1:) A(c, c+1:nc)^T; partial |x|^2. The
+//                one value of that w a workgroup needs outside its own rows, W(c, wc+1), reaches every workgroup
+//                as a scalar (before the update) and gets the same update there.
+//   trd_symv     larfg from the partials (every workgroup, redundantly), v in LDS; y = A(0:c, 0:c) v, and as
+//                2m more columns of the same product u = W(0:c, wc+1:)^T v and u2 = A(0:c, c+1:nc)^T v
 //   trd_wrow     w = tau (y - A(:, c+1:nc) u - W(:, wc+1:) u2); A(0:c, c) = v; partial w^T v
-//   trd_walpha   w += -tau/2 (w^T v) v
+// The last column of a panel gets its w update alone (trd_walpha) before the trailing update uses W.
 // Back-transformation (TridiagonalBackTransformation.swift:16-45): 32-reflector blocks in increasing order, V
 // explicit (unit, zeros below), T = larft('B', 'C') from the block's Gram, then W = V^T Z, W2 = T W, Z -= V W2.
 #include "common.hpp"
@@ -31,16 +34,32 @@ constexpr int NB = 32, NX = 32;
 constexpr int ROWS = 256;  // rows per workgroup of the row kernels
 constexpr int SYMV_COLS = 8;  // columns of y per workgroup (two per wavefront)
 
-// A(0:c+1, c) -= A(0:c+1, c+1:nc) W(c, wc+1:wc+1+m)^T + W(0:c+1, wc+1:) A(c, c+1:nc)^T (m = nc - 1 - c may be 0);
-// part[blockIdx.x] = sum of A(r, c)^2 over this block's rows r <= c - 2 (the larfg vector).
+// With prev: the previous column's (c + 1) w += alpha v first, alpha = -tau/2 (w^T v) from its partials pw (npw of
+// them, summed in order), its e and tau written by workgroup 0; W(c, wc+1) from the scalar wnext plus the same
+// update. Then A(0:c+1, c) -= A(0:c+1, c+1:nc) W(c, wc+1:wc+1+m)^T + W(0:c+1, wc+1:) A(c, c+1:nc)^T (m = nc - 1 - c
+// may be 0); part[blockIdx.x] = sum of A(r, c)^2 over this block's rows r <= c - 2 (the larfg vector).
 template <typename T>
 __global__ __launch_bounds__(ROWS) void trd_colupd_kernel(int c, int m, T* __restrict__ a, int lda,
-                                                          const T* __restrict__ w, int ldw, int wc,
-                                                          T* __restrict__ part) {
+                                                          T* __restrict__ w, int ldw, int wc, T* __restrict__ part,
+                                                          int prev, const T* __restrict__ vbuf,
+                                                          const T* __restrict__ scal3, const T* __restrict__ pw,
+                                                          int npw, T* __restrict__ e, T* __restrict__ tau) {
   __shared__ T wr[NB], ar[NB], red[ROWS / 64];
   const int t = threadIdx.x, r = blockIdx.x * ROWS + t;
+  T alpha = 0;
+  if (prev) {
+    T dot = 0;
+    for (int q = 0; q < npw; ++q) dot += pw[q];
+    const T tv = scal3[0];
+    alpha = T(-0.5) * tv * dot;
+    if (r <= c) w[idx(r, wc + 1, ldw)] += alpha * vbuf[r];
+    if (blockIdx.x == 0 && t == 0) {
+      e[c] = scal3[1];
+      tau[c] = tv;
+    }
+  }
   if (t < m) {
-    wr[t] = w[idx(c, wc + 1 + t, ldw)];
+    wr[t] = (t == 0 && prev) ? scal3[2] + alpha * vbuf[c] : w[idx(c, wc + 1 + t, ldw)];
     ar[t] = a[idx(c, c + 1 + t, lda)];
   }
   __syncthreads();
@@ -56,12 +75,14 @@ __global__ __launch_bounds__(ROWS) void trd_colupd_kernel(int c, int m, T* __res
 }
 
 // larfg of (alpha = A(c-1, c), x = A(0:c-1, c)) from the norm partials, computed by every workgroup; v in LDS
-// (v(c-1) = 1). Workgroup 0 publishes v, tau, beta. y(j) = sum_r A(r, j) v(r), j in this workgroup's columns, into
-// W(j, wc).
+// (v(c-1) = 1). Workgroup 0 publishes v, tau, beta. Columns j < c + 2m of the product: y(j) = A(0:c, j)^T v into
+// W(j, wc) for j < c; then u(q) = W(0:c, wc+1+q)^T v into uu[q] and u2(q) = A(0:c, c+1+q)^T v into uu[NB + q] and
+// W(c+1+q, wc) (where LAPACK's dlatrd leaves it), q < m.
 template <typename T>
-__global__ __launch_bounds__(256) void trd_symv_kernel(int c, const T* __restrict__ a, int lda,
+__global__ __launch_bounds__(256) void trd_symv_kernel(int c, int m, const T* __restrict__ a, int lda,
                                                        const T* __restrict__ part, int nparts, T* __restrict__ w,
-                                                       int ldw, int wc, T* __restrict__ vbuf, T* __restrict__ scal3) {
+                                                       int ldw, int wc, T* __restrict__ vbuf, T* __restrict__ scal3,
+                                                       T* __restrict__ uu) {
   extern __shared__ char smem_raw[];
   T* v = reinterpret_cast<T*>(smem_raw);  // c entries
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -85,8 +106,9 @@ __global__ __launch_bounds__(256) void trd_symv_kernel(int c, const T* __restric
   }
   for (int u = 0; u < SYMV_COLS / 4; ++u) {
     const int j = blockIdx.x * SYMV_COLS + wid * (SYMV_COLS / 4) + u;
-    if (j >= c) break;
-    const T* col = a + idx(0, j, lda);
+    if (j >= c + 2 * m) break;
+    const T* col = j < c ? a + idx(0, j, lda) : j < c + m ? w + idx(0, wc + 1 + (j - c), ldw)
+                                                          : a + idx(0, c + 1 + (j - c - m), lda);
     T s0 = 0, s1 = 0;
     int r = lane;
     for (; r + 64 < c; r += 128) {
@@ -95,33 +117,25 @@ __global__ __launch_bounds__(256) void trd_symv_kernel(int c, const T* __restric
     }
     if (r < c) s0 += col[r] * v[r];
     const T s = wave_sum(s0 + s1);
-    if (lane == 0) w[idx(j, wc, ldw)] = s;
+    if (lane == 0) {
+      if (j < c) {
+        w[idx(j, wc, ldw)] = s;
+      } else if (j < c + m) {
+        uu[j - c] = s;
+      } else {
+        uu[NB + (j - c - m)] = s;
+        w[idx(c + 1 + (j - c - m), wc, ldw)] = s;
+      }
+    }
   }
 }
 
-// Dot q < m: u(q) = W(0:c, wc+1+q)^T v; dot m + q: u2(q) = A(0:c, c+1+q)^T v, stored in uu[q] / uu[NB + q] and u2
-// also in W(c+1+q, wc) (where LAPACK's dlatrd leaves it).
-template <typename T>
-__global__ __launch_bounds__(256) void trd_dots_kernel(int c, int m, const T* __restrict__ a, int lda,
-                                                       T* __restrict__ w, int ldw, int wc,
-                                                       const T* __restrict__ vbuf, T* __restrict__ uu) {
-  __shared__ T red[4];
-  const int q = blockIdx.x % m, which = blockIdx.x / m;
-  const T* col = which == 0 ? w + idx(0, wc + 1 + q, ldw) : a + idx(0, c + 1 + q, lda);
-  T s = 0;
-  for (int r = threadIdx.x; r < c; r += blockDim.x) s += col[r] * vbuf[r];
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) {
-    uu[which * NB + q] = s;
-    if (which == 1) w[idx(c + 1 + q, wc, ldw)] = s;
-  }
-}
-
-// w(r) = tau (y(r) - A(r, c+1:nc) u - W(r, wc+1:) u2) for r < c, into W(r, wc); A(r, c) = v(r); part = partial w^T v.
+// w(r) = tau (y(r) - A(r, c+1:nc) u - W(r, wc+1:) u2) for r < c, into W(r, wc); A(r, c) = v(r); part = partial w^T v;
+// scal3[2] = w(c - 1) (the next column's trd_colupd reads it in every workgroup).
 template <typename T>
 __global__ __launch_bounds__(ROWS) void trd_wrow_kernel(int c, int m, T* __restrict__ a, int lda, T* __restrict__ w,
                                                         int ldw, int wc, const T* __restrict__ vbuf,
-                                                        const T* __restrict__ uu, const T* __restrict__ scal3,
+                                                        const T* __restrict__ uu, T* __restrict__ scal3,
                                                         T* __restrict__ part) {
   __shared__ T us[2 * NB], red[ROWS / 64];
   const int t = threadIdx.x, r = blockIdx.x * ROWS + t;
@@ -134,6 +148,7 @@ __global__ __launch_bounds__(ROWS) void trd_wrow_kernel(int c, int m, T* __restr
     for (int q = 0; q < m; ++q) y -= a[idx(r, c + 1 + q, lda)] * us[q] + w[idx(r, wc + 1 + q, ldw)] * us[NB + q];
     y *= tau;
     w[idx(r, wc, ldw)] = y;
+    if (r == c - 1) scal3[2] = y;
     const T vr = vbuf[r];
     a[idx(r, c, lda)] = vr;
     x = y * vr;
@@ -294,7 +309,7 @@ __global__ __launch_bounds__(256) void ormtr_blocks_kernel(int n, const T* __res
 
 template <typename T>
 size_t sytrd_workspace(int n) {
-  // W (n x NB) | vbuf (n) | uu (2 NB) | scal3 (4) | partials (2 cdiv(n, 8) + 64)
+  // W (n x NB) | vbuf (n) | uu (2 NB) | scal3 (4) | norm partials, 16, w^T v partials (cdiv(n, ROWS) each)
   return (size_t)n * NB + n + 2 * NB + 4 + 2 * (size_t)cdiv(n, SYMV_COLS) + 2 * (size_t)cdiv(n, ROWS) + 256;
 }
 
@@ -307,36 +322,37 @@ void sytrd_upper(hipStream_t s, int n, T* a, int lda, T* d, T* e, T* tau, T* wor
   T* vbuf = w + (size_t)n * NB;
   T* uu = vbuf + n;
   T* scal3 = uu + 2 * NB;
-  T* part = scal3 + 4;
+  T* partn = scal3 + 4;                       // norm partials of the larfg vector
+  T* partw = partn + cdiv(n, ROWS) + 16;      // partials of w^T v
   int kk = n;
   if (n > NX) {
     kk = n - ((n - NX + NB - 1) / NB) * NB;
     for (int i1 = n - NB + 1; i1 >= kk + 1; i1 -= NB) {
       const int nc = i1 + NB - 1;  // leading block of the panel
       const int L = nc - NB;       // columns L ..< nc, leading block of the update L x L
+      int npw = 0;                 // partials of the previous column's w^T v
       for (int c = nc - 1; c >= L; --c) {
         const int wc = c - L, m = nc - 1 - c;
         const int nrow = cdiv(c + 1, ROWS);
-        hipLaunchKernelGGL(trd_colupd_kernel<T>, dim3(nrow), dim3(ROWS), 0, s, c, m, a, lda, w, ldw, wc, part);
+        hipLaunchKernelGGL(trd_colupd_kernel<T>, dim3(nrow), dim3(ROWS), 0, s, c, m, a, lda, w, ldw, wc, partn,
+                           c < nc - 1 ? 1 : 0, vbuf, scal3, partw, npw, e, tau);
         XTB_LAUNCH_CHECK();
         // c >= L >= 1 here (kk >= 1), so the column has a reflector
         const size_t lds = sizeof(T) * (size_t)c;
         allow_lds(trd_symv_kernel<T>, lds);
-        hipLaunchKernelGGL(trd_symv_kernel<T>, dim3(cdiv(c, SYMV_COLS)), dim3(256), lds, s, c, a, lda, part, nrow, w,
-                           ldw, wc, vbuf, scal3);
+        hipLaunchKernelGGL(trd_symv_kernel<T>, dim3(cdiv(c + 2 * m, SYMV_COLS)), dim3(256), lds, s, c, m, a, lda,
+                           partn, nrow, w, ldw, wc, vbuf, scal3, uu);
         XTB_LAUNCH_CHECK();
-        if (m > 0) {
-          hipLaunchKernelGGL(trd_dots_kernel<T>, dim3(2 * m), dim3(256), 0, s, c, m, a, lda, w, ldw, wc, vbuf, uu);
-          XTB_LAUNCH_CHECK();
-        }
         const int nr2 = cdiv(c, ROWS);
         hipLaunchKernelGGL(trd_wrow_kernel<T>, dim3(nr2), dim3(ROWS), 0, s, c, m, a, lda, w, ldw, wc, vbuf, uu, scal3,
-                           part);
+                           partw);
         XTB_LAUNCH_CHECK();
-        hipLaunchKernelGGL(trd_walpha_kernel<T>, dim3(nr2), dim3(ROWS), 0, s, c, w, ldw, wc, vbuf, scal3, part, nr2,
-                           e, tau);
-        XTB_LAUNCH_CHECK();
+        npw = nr2;
       }
+      // the panel's last column: its w update before the trailing update reads W
+      hipLaunchKernelGGL(trd_walpha_kernel<T>, dim3(cdiv(L, ROWS)), dim3(ROWS), 0, s, L, w, ldw, 0, vbuf, scal3, partw,
+                         npw, e, tau);
+      XTB_LAUNCH_CHECK();
       // A(0:L, 0:L) -= V W^T + W V^T, V = A(0:L, L:L+NB), both triangles
       gemm<T>(s, 'N', 'T', L, L, NB, T(-1), a + idx(0, L, lda), lda, w, ldw, T(1), a, lda);
       gemm<T>(s, 'N', 'T', L, L, NB, T(-1), w, ldw, a + idx(0, L, lda), lda, T(1), a, lda);
