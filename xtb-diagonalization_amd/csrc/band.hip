@@ -12,6 +12,12 @@
 //      W = X - 1/2 V Y, again over row blocks;
 //      A2 -= [V W] [W V]^T, one rank-2k GEMM on the matrix cores (full storage: both triangles of the trailing block
 //      stay current).
+// The fused path for band widths 8 and 16 (panel_factor_kernel, panel_xpart / xfin, panel_update_factor_kernel)
+// keeps only A2's lower triangle current: its updates write the 64 x 64 tiles on and below the diagonal, and its
+// product X = A2 V reads an element above the diagonal from its mirror, so a panel moves half the update bytes. (The
+// panels' tile grids shift by b from one panel to the next, so it is the elementwise lower triangle that is current,
+// not a fixed set of tiles.) The general path, which runs first when a panel is too tall for the fused one, updates
+// both triangles.
 // After the last panel the lower band is copied into the chase's band storage (element (i, j), 0 <= i - j <= 2b - 1,
 // at i + (2b - 1) j; see chase.hip).
 #include "common.hpp"
@@ -378,11 +384,28 @@ __global__ __launch_bounds__(256) void panel_xpart_kernel(int m, int k, const T*
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int i0 = blockIdx.x * 64, ks = blockIdx.y, l0 = ks * KC, kl = min(KC, m - l0);
   constexpr int NA = 64 * KC / 256, NV = 16 * KC / 256;
+  // Only A2's lower triangle is current (the updates write the tiles on and below the diagonal): an element above it
+  // is read from its mirror. A 64-column sub-chunk entirely above the diagonal is read with the lanes along the
+  // mirror's rows (kk fastest), so that those loads stay coalesced; element (i, kk) of a 64 x 64 sub-chunk is the
+  // thread's f = t + 256 (q mod 16) with (i, kk) = (f mod 64, f / 64), or (f / 64, f mod 64) when mirrored.
+  auto mirrored = [&](int q) { return i0 + 63 < l0 + 64 * (q / 16); };
+  auto coords = [&](int q, int& i, int& kk) {
+    const int f = t + 256 * (q % 16);
+    if (mirrored(q)) {
+      i = f / 64;
+      kk = 64 * (q / 16) + f % 64;
+    } else {
+      i = f % 64;
+      kk = 64 * (q / 16) + f / 64;
+    }
+  };
   T ra[NA], rv[NV];  // every load in flight before the first LDS store
 #pragma unroll
   for (int q = 0; q < NA; ++q) {
-    const int e = t + 256 * q, i = e % 64, kk = e / 64;
-    ra[q] = (i0 + i < m && kk < kl) ? a2[idx(i0 + i, l0 + kk, lda)] : T(0);
+    int i, kk;
+    coords(q, i, kk);
+    const int gi = i0 + i, gk = l0 + kk;
+    ra[q] = (gi < m && kk < kl) ? (gi >= gk ? a2[idx(gi, gk, lda)] : a2[idx(gk, gi, lda)]) : T(0);
   }
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
@@ -391,8 +414,9 @@ __global__ __launch_bounds__(256) void panel_xpart_kernel(int m, int k, const T*
   }
 #pragma unroll
   for (int q = 0; q < NA; ++q) {
-    const int e = t + 256 * q;
-    As[e / 64][e % 64] = ra[q];
+    int i, kk;
+    coords(q, i, kk);
+    As[kk][i] = ra[q];
   }
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
@@ -534,6 +558,15 @@ __device__ __forceinline__ void form_y(int k, const T* __restrict__ ypart, int n
   __syncthreads();
 }
 
+// Tile q of the lower-triangle enumeration (I >= J, row tile I of the q-th row I (I + 1) / 2 + J): A2 is symmetric and
+// only its tiles on and below the diagonal are updated (and read, by the next panel's product and factorization).
+__device__ inline void lower_tile(int q, int& I, int& J) {
+  I = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+  while (I * (I + 1) / 2 > q) --I;
+  while ((I + 1) * (I + 2) / 2 <= q) ++I;
+  J = q - I * (I + 1) / 2;
+}
+
 // One 64 x 64 tile (rows I0.., columns J0..) of the update. Elements (i, j) with i >= skip and j < skip are left as
 // they are (the fused next-panel factorization takes those).
 template <typename T>
@@ -637,7 +670,9 @@ __global__ __launch_bounds__(256) void panel_update_kernel(int m, int k, T* __re
   ypart = zb.at(ypart);
   tt = zb.at(tt);
   __shared__ UpdateLds<T> L;
-  update_tile<T>(m, k, a2, lda, v, ldv, x, ypart, nrb, tt, ldt, blockIdx.x * 64, blockIdx.y * 64, 0, L);
+  int I, J;
+  lower_tile(blockIdx.x, I, J);
+  update_tile<T>(m, k, a2, lda, v, ldv, x, ypart, nrb, tt, ldt, I * 64, J * 64, 0, L);
 }
 
 // The update of this panel (as panel_update_kernel, workgroups 1 ..) fused with the factorization of the next panel
@@ -666,8 +701,9 @@ __global__ __launch_bounds__(256) void panel_update_factor_kernel(int m, T* __re
   __shared__ UpdateLds<T> L;
   const int t = threadIdx.x;
   if (blockIdx.x > 0) {
-    const int q = blockIdx.x - 1;
-    update_tile<T>(m, NB, a2, lda, v, ldv, x, ypart, nrb, tt, ldt, (q % nrb) * 64, (q / nrb) * 64, NB, L);
+    int I, J;
+    lower_tile(blockIdx.x - 1, I, J);
+    update_tile<T>(m, NB, a2, lda, v, ldv, x, ypart, nrb, tt, ldt, I * 64, J * 64, NB, L);
     return;
   }
   const int mn = m - NB;  // the next panel's rows
@@ -996,7 +1032,7 @@ bool panel_factor(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, 
 template <typename T, int NB, int R>
 void launch_update_factor(hipStream_t s, int m, T* a2, int lda, const T* v, int ldv, const T* x, const T* kpart,
                           int nrb, const T* tt, int ldt, T* v1n, int ldv1, T* tau, T* ttn) {
-  hipLaunchKernelGGL((panel_update_factor_kernel<T, NB, R>), dim3(1 + nrb * nrb, 1, zgrid()), dim3(256), 0, s, m, a2, lda,
+  hipLaunchKernelGGL((panel_update_factor_kernel<T, NB, R>), dim3(1 + nrb * (nrb + 1) / 2, 1, zgrid()), dim3(256), 0, s, m, a2, lda,
                      v, ldv, x, kpart, nrb, tt, ldt, v1n, ldv1, tau, ttn, NB, zb());
   XTB_LAUNCH_CHECK();
 }
@@ -1079,7 +1115,7 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
                  update_factor<T>(s, m, b, a2, lda, v, n, xb, kpart, nrb, tt, v1 + idx(jn + b, jn, n), n, tau,
                                   t1 + idx(0, jn, b));
       if (!factored) {
-        hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb, nrb, zgrid()), dim3(256), 0, s, m, k, a2, lda, v, n, xb,
+        hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb * (nrb + 1) / 2, 1, zgrid()), dim3(256), 0, s, m, k, a2, lda, v, n, xb,
                            kpart, nrb, tt, b, zb());
         XTB_LAUNCH_CHECK();
       }
