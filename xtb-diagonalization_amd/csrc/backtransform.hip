@@ -410,9 +410,11 @@ bool launch_strips(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldz
     // ahead. A batch fills the GPU with strips: eight wavefronts and loads two blocks ahead hide more latency there
     // (measured: 32 solves of n = 1204, FP64 505 -> 531 solves/s).
     if (current_batch().count > 1 &&
-        (launch_strip<T, B, REV, 8, 8, 2>(s, p, v2, zt, ldzt, m) || launch_strip<T, B, REV, 8, 4, 2>(s, p, v2, zt, ldzt, m)))
+        (launch_strip<T, B, REV, 8, 8, 2>(s, p, v2, zt, ldzt, m) ||
+         launch_strip<T, B, REV, 8, 4, 2>(s, p, v2, zt, ldzt, m)))
       return true;
-    return launch_strip<T, B, REV, 8, 4, 1>(s, p, v2, zt, ldzt, m) || launch_strip<T, B, REV, 8, 2, 1>(s, p, v2, zt, ldzt, m);
+    return launch_strip<T, B, REV, 8, 4, 1>(s, p, v2, zt, ldzt, m) ||
+           launch_strip<T, B, REV, 8, 2, 1>(s, p, v2, zt, ldzt, m);
   }
 }
 
@@ -421,8 +423,8 @@ void launch_bulges(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldz
   if (transposed ? launch_strips<T, B, true>(s, p, v2, zt, ldzt, m) : launch_strips<T, B, false>(s, p, v2, zt, ldzt, m))
     return;
   if (transposed)
-    hipLaunchKernelGGL((bulge_apply_kernel<T, B, true>), dim3(cdiv(m, 64), 1, zgrid()), dim3(64 * BULGE_WAVES), 0, s, p.n,
-                       p.b, m, p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt, zb());
+    hipLaunchKernelGGL((bulge_apply_kernel<T, B, true>), dim3(cdiv(m, 64), 1, zgrid()), dim3(64 * BULGE_WAVES), 0, s,
+                       p.n, p.b, m, p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt, zb());
   else
     hipLaunchKernelGGL((bulge_apply_kernel<T, B, false>), dim3(cdiv(m, 64), 1, zgrid()), dim3(64 * BULGE_WAVES), 0, s,
                        p.n, p.b, m, p.blocks_dev, p.stage_off_dev, p.order_dev, p.nstages, v2, p.n, zt, ldzt, zb());

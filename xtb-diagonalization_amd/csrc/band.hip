@@ -1,5 +1,6 @@
 // Stage 1 of the two-stage reduction: dense symmetric -> lower band of width b (the reference's reduceToBand with the
-// combined sy2sb update, Stages/BandForm.swift:28-142); the back-transformation through its panels is in backtransform.hip.
+// combined sy2sb update, Stages/BandForm.swift:28-142); the back-transformation through its panels is in
+// backtransform.hip.
 //
 // Per panel j (columns j ..< j + b, rows r = j + b ..< n, m = n - r, k = min(b, m)):
 //   1. panel_qr_kernel: Householder QR of the m x b panel by one workgroup of 256 threads. The panel is processed
@@ -983,8 +984,8 @@ size_t band_workspace(int n, int b) {
 
 template <typename T, int NB, int R>
 void launch_factor(hipStream_t s, int m, T* p, int lda, T* v1, int ldv1, T* tau, T* tt, int ldt) {
-  hipLaunchKernelGGL((panel_factor_kernel<T, NB, R>), dim3(1, 1, zgrid()), dim3(256), 0, s, m, p, lda, v1, ldv1, tau, tt,
-                     ldt, zb());
+  hipLaunchKernelGGL((panel_factor_kernel<T, NB, R>), dim3(1, 1, zgrid()), dim3(256), 0, s, m, p, lda, v1, ldv1, tau,
+                     tt, ldt, zb());
   XTB_LAUNCH_CHECK();
 }
 
@@ -1032,8 +1033,8 @@ bool panel_factor(hipStream_t s, int m, int nb, T* p, int lda, T* v1, int ldv1, 
 template <typename T, int NB, int R>
 void launch_update_factor(hipStream_t s, int m, T* a2, int lda, const T* v, int ldv, const T* x, const T* kpart,
                           int nrb, const T* tt, int ldt, T* v1n, int ldv1, T* tau, T* ttn) {
-  hipLaunchKernelGGL((panel_update_factor_kernel<T, NB, R>), dim3(1 + nrb * (nrb + 1) / 2, 1, zgrid()), dim3(256), 0, s, m, a2, lda,
-                     v, ldv, x, kpart, nrb, tt, ldt, v1n, ldv1, tau, ttn, NB, zb());
+  hipLaunchKernelGGL((panel_update_factor_kernel<T, NB, R>), dim3(1 + nrb * (nrb + 1) / 2, 1, zgrid()), dim3(256), 0,
+                     s, m, a2, lda, v, ldv, x, kpart, nrb, tt, ldt, v1n, ldv1, tau, ttn, NB, zb());
   XTB_LAUNCH_CHECK();
 }
 
@@ -1115,8 +1116,8 @@ void reduce_to_band(hipStream_t s, int n, int b, T* a, int lda, T* v1, T* t1, T*
                  update_factor<T>(s, m, b, a2, lda, v, n, xb, kpart, nrb, tt, v1 + idx(jn + b, jn, n), n, tau,
                                   t1 + idx(0, jn, b));
       if (!factored) {
-        hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb * (nrb + 1) / 2, 1, zgrid()), dim3(256), 0, s, m, k, a2, lda, v, n, xb,
-                           kpart, nrb, tt, b, zb());
+        hipLaunchKernelGGL(panel_update_kernel<T>, dim3(nrb * (nrb + 1) / 2, 1, zgrid()), dim3(256), 0, s, m, k, a2,
+                           lda, v, n, xb, kpart, nrb, tt, b, zb());
         XTB_LAUNCH_CHECK();
       }
       continue;

@@ -27,8 +27,12 @@ namespace xtbeig {
 
 namespace {
 
-__device__ inline int ld_acquire(const int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
-__device__ inline void st_release(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ inline int ld_acquire(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void st_release(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 template <typename T>
 struct Band {
@@ -262,9 +266,9 @@ __device__ inline T part_sum(T v) {
 // Lane-per-column form for b = B (8 or 16): H * B lanes carry one bulge. Lane (h, cl) owns column rs - b + cl of the
 // left block, row re + cl of the block below and row rs + cl of the diagonal block, over the U = B / H rows (columns)
 // u = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms followed, for H > 1, by a sum
-// over the group's H parts. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is a sum
-// over the B lanes of one half, and q is exchanged through LDS. One wavefront advances 64 / (H B) sweeps in lockstep,
-// each group on its own sweep and at its own bulge. The band is read and written as if the matrix continued past n
+// over the group's H parts. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is
+// a sum over the B lanes of one half, and q is exchanged through LDS. One wavefront advances 64 / (H B) sweeps in
+// lockstep, each group on its own sweep and at its own bulge. The band is read and written as if the matrix continued past n
 // with zeros (band_elems(n, b) covers b extra columns, and the kernel clears every stored element outside the matrix
 // before the chase), so a bulge at the end of the band takes the same branch-free path as any other: the rows past n
 // give x = 0, w = 0 there and leave the zeros in place.

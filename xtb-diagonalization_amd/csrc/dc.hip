@@ -15,8 +15,8 @@
 // Deflation bookkeeping (K, the index arrays, column types) follows dlaed2 step for step and is compared with the host
 // LAPACK's dlaed2 on the same inputs by the tests.
 //
-// The deflation (deflate_kernel) and the secular solver follow LAPACK's dlaed2 / dlaed4 / dlaed5 / dlaed6 closely (their variable
-// names and branch structure), so LAPACK's license applies to those parts:
+// The deflation (deflate_kernel) and the secular solver follow LAPACK's dlaed2 / dlaed4 / dlaed5 / dlaed6 closely
+// (their variable names and branch structure), so LAPACK's license applies to those parts:
 //   Copyright (c) 1992-2017 The University of Tennessee and The University of Tennessee Research Foundation.
 //   Copyright (c) 2000-2017 The University of California Berkeley. Copyright (c) 2006-2017 The University of
 //   Colorado Denver. All rights reserved.
@@ -524,8 +524,9 @@ constexpr int SEC_G = 32;  // lanes per secular root: each sums every SEC_G-th p
 
 __global__ __launch_bounds__(64) void secular_kernel(const Node* nodes, const NodeOut* out, const double* rho_lvl,
                                                      const double* __restrict__ dlamda_all,
-                                                     const double* __restrict__ w_all, const int* __restrict__ indxc_all,
-                                                     double* __restrict__ d_all, int* __restrict__ orig_all,
+                                                     const double* __restrict__ w_all,
+                                                     const int* __restrict__ indxc_all, double* __restrict__ d_all,
+                                                     int* __restrict__ orig_all,
                                                      double* __restrict__ tau_all, double* __restrict__ s_all,
                                                      int* info, int maxit, Zb zb) {
   out = zb.at(out);
@@ -630,7 +631,8 @@ __global__ __launch_bounds__(256) void vectors_kernel(const Node* nodes, const N
                                                       const int* __restrict__ orig_all,
                                                       const double* __restrict__ tau_all,
                                                       const double* __restrict__ wnew_all,
-                                                      const int* __restrict__ indxc_all, double* __restrict__ s_all, Zb zb) {
+                                                      const int* __restrict__ indxc_all, double* __restrict__ s_all,
+                                                      Zb zb) {
   out = zb.at(out);
   dlamda_all = zb.at(dlamda_all);
   orig_all = zb.at(orig_all);
@@ -694,7 +696,8 @@ __global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, cons
 // stored descending (read backwards), merged by ranks; on ties the first list's element comes first, as dlamrg.
 // K = 0 (every column deflated at once) leaves them ascending: the identity.
 __global__ __launch_bounds__(256) void lamrg_kernel(int count, const Node* nodes, const NodeOut* out,
-                                                    const double* __restrict__ d_all, int* __restrict__ perm_all, Zb zb) {
+                                                    const double* __restrict__ d_all, int* __restrict__ perm_all,
+                                                    Zb zb) {
   out = zb.at(out);
   d_all = zb.at(d_all);
   perm_all = zb.at(perm_all);

@@ -1,7 +1,8 @@
 // The MFMA GEMM tile shared by every matrix-product kernel (gemm.hip, and the divide-and-conquer merge products
 // whose sizes are only known on the device). A 64 x TBN tile of C per workgroup (TBN 64 or 128), its wavefronts
 // in a 2 x (threads / 128) grid of 16 x 16 MFMA blocks; K staged through two LDS buffers 32 at a time, with the next
-// two slabs prefetched into registers (a slab is 2048 FP64 MFMA cycles per wavefront, less than the latency of its loads).
+// two slabs prefetched into registers (a slab is 2048 FP64 MFMA cycles per wavefront, less than the latency of its
+// loads).
 // MFMA operand layout (16x16x4): lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16); the accumulator layout
 // differs between FP32 and FP64 (Mfma<T>::row).
 #pragma once

@@ -389,7 +389,8 @@ __global__ __launch_bounds__(256) void cluster_apply_kernel(int n, double* __res
 
 // rank of lambda_i in ascending order (ties by index): RANK_G lanes per i, each counting every RANK_G-th j
 constexpr int RANK_G = 16;
-__global__ __launch_bounds__(256) void rank_kernel(int n, const double* __restrict__ lam, int* __restrict__ rank, Zb zb) {
+__global__ __launch_bounds__(256) void rank_kernel(int n, const double* __restrict__ lam, int* __restrict__ rank,
+                                                   Zb zb) {
   lam = zb.at(lam);
   rank = zb.at(rank);
   const int g = (blockIdx.x * blockDim.x + threadIdx.x) / RANK_G, gl = threadIdx.x % RANK_G;
