@@ -2,12 +2,12 @@
 // batches of uneven-n problems spread over concurrent streams, and the per-bucket band-width autotuner. (Batches
 // solved inside the kernels, a group at a time, are in grouped.hip.)
 //
-// Graphs. A solve enqueues several hundred kernels; replaying them from a graph removes the host's launch work and
-// the gaps between launches. The graph for (n, precision, band, refinement steps) is captured after one uncaptured
-// warm-up solve (the D&C plan, the back-transformation plan and the arena exist then, so the capture records
-// kernels only: no allocation, no memset, no host synchronization). It reads H and S from its own staging buffers and
-// writes w and C there; a call copies in, launches, copies out. A graph remembers the arena generation it was captured against:
-// when a later, larger solve reallocates the arena, the graph is captured again.
+// Graphs. A solve enqueues several hundred kernels; replaying them from a graph removes the host's launch work and the
+// gaps between launches. The graph for (n, precision, band, refinement steps) is captured after one uncaptured warm-up
+// solve (the D&C plan, the back-transformation plan and the arena exist then, so the capture records kernels only: no
+// allocation, no memset, no host synchronization). It reads H and S from its own staging buffers and writes w and C
+// there; a call copies in, launches, copies out. A graph remembers the arena generation it was captured against: when a
+// later, larger solve reallocates the arena, the graph is captured again.
 //
 // Batches. Each generalized solve at these n is latency-bound (the chase runs on one CU, the D&C levels and panel
 // factorizations on few), so a batch runs up to `slots` solves at once on sub-handles with their own streams and

@@ -264,17 +264,16 @@ __device__ inline T part_sum(T v) {
 }
 
 // Lane-per-column form for b = B (8 or 16): H * B lanes carry one bulge. Lane (h, cl) owns column rs - b + cl of the
-// left block, row re + cl of the block below and row rs + cl of the diagonal block, over the U = B / H rows (columns)
-// u = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms followed, for H > 1, by a sum
-// over the group's H parts. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is
-// a sum over the B lanes of one half, and q is exchanged through LDS. One wavefront advances 64 / (H B) sweeps in
-// lockstep, each group on its own sweep and at its own bulge. The band is read and written as if the matrix continued past n
-// with zeros (band_elems(n, b) covers b extra columns, and the kernel clears every stored element outside the matrix
-// before the chase), so a bulge at the end of the band takes the same branch-free path as any other: the rows past n
-// give x = 0, w = 0 there and leave the zeros in place.
-// In band storage A(rs + a, rs + c) sits at rs (ld + 1) + a + ld c = rs 2b + a + ld c, so every element a bulge
-// touches is at rs 2b plus a constant of the lane: one set for the first bulge of a sweep (whose left block is the
-// single column s = rs - 1) and one for the others. The kernel forms them once.
+// left block, row re + cl of the block below and row rs + cl of the diagonal block, over the U = B / H rows (columns) u
+// = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms followed, for H > 1, by a sum over
+// the group's H parts. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is a sum
+// over the B lanes of one half, and q is exchanged through LDS. One wavefront advances 64 / (H B) sweeps in lockstep,
+// each group on its own sweep and at its own bulge. The band is read and written as if the matrix continued past n with
+// zeros (band_elems(n, b) covers b extra columns, and the kernel clears every stored element outside the matrix before
+// the chase), so a bulge at the end of the band takes the same branch-free path as any other: the rows past n give x =
+// 0, w = 0 there and leave the zeros in place. In band storage A(rs + a, rs + c) sits at rs (ld + 1) + a + ld c = rs 2b
+// + a + ld c, so every element a bulge touches is at rs 2b plus a constant of the lane: one set for the first bulge of
+// a sweep (whose left block is the single column s = rs - 1) and one for the others. The kernel forms them once.
 struct LaneOffsets {
   int x0, x1;  // x = A[R, c]: first bulge, others
   int l0, l1;  // the lane's left block column
