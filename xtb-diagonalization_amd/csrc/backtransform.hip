@@ -410,7 +410,8 @@ bool launch_strips(hipStream_t s, const BackPlan& p, const T* v2, T* zt, int ldz
     // ahead. A batch fills the GPU with strips: eight wavefronts and loads two blocks ahead hide more latency there
     // (measured: 32 solves of n = 1204, FP64 505 -> 531 solves/s).
     if (current_batch().count > 1 &&
-        (launch_strip<T, B, REV, 8, 8, 2>(s, p, v2, zt, ldzt, m) ||
+        (launch_strip<T, B, REV, 16, 8, 2>(s, p, v2, zt, ldzt, m) ||
+         launch_strip<T, B, REV, 8, 8, 2>(s, p, v2, zt, ldzt, m) ||
          launch_strip<T, B, REV, 8, 4, 2>(s, p, v2, zt, ldzt, m)))
       return true;
     return launch_strip<T, B, REV, 8, 4, 1>(s, p, v2, zt, ldzt, m) ||
