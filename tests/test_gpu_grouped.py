@@ -60,7 +60,7 @@ def grouped(torch, hnd, probs, prec, granule, ld_extra=0):
 @pytest.mark.parametrize("prec", [xtbeig.FP64, xtbeig.FP32_REFINE])
 def test_grouped_unpadded_bit_identical(torch_cuda, prec):
     torch = torch_cuda
-    ns = [200, 200, 200, 131, 131, 200, 64]
+    ns = [200, 200, 200, 131, 131, 200, 64, 1, 2, 3, 3]
     probs = [workloads.generalized_pair(n, 700 + q) for q, n in enumerate(ns)]
     hnd = xtbeig.Handle(0)
     st, lat, out = grouped(torch, hnd, probs, prec, granule=1, ld_extra=3)
