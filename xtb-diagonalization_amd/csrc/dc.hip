@@ -691,6 +691,40 @@ __global__ __launch_bounds__(256) void merge_gemm_kernel(const Node* nodes, cons
   gemm_tile<double, false, false>(rows, k, inner, 1.0, A, rows, B, nd.sz, 0.0, C, ldq, m0, c0, lds);
 }
 
+// The same products for a level whose merges are all at most SMALL_MERGE wide: direct dot products in registers
+// (lane = row, one wavefront per column, the sum over the inner index in order), no LDS, so that these small
+// workgroups find room beside other kernels' LDS-heavy ones (in a batch, the bulge back-transformation's strips run
+// beside the divide and conquer). Grid: (cdiv(maxsz, 4), 1, 2 * merges).
+constexpr int SMALL_MERGE = 128;
+__global__ __launch_bounds__(256) void merge_small_kernel(const Node* nodes, const NodeOut* out,
+                                                          const double* __restrict__ q2_all,
+                                                          const double* __restrict__ s_all,
+                                                          double* __restrict__ q_all, int ldq, Zb zb) {
+  out = zb.at(out);
+  q2_all = zb.at(q2_all);
+  s_all = zb.at(s_all);
+  q_all = zb.at(q_all);
+  const int node = zb.z() >> 1, half = zb.z() & 1;
+  const Node nd = nodes[node];
+  const NodeOut o = out[node];
+  const int k = o.k;
+  const int n1 = nd.n1, n2 = nd.sz - nd.n1;
+  const int n12 = o.ctot[0] + o.ctot[1], n23 = o.ctot[1] + o.ctot[2];
+  const int rows = half ? n2 : n1, inner = half ? n23 : n12;
+  const double* A = q2_all + nd.q2off + (half ? (size_t)n1 * n12 : 0);
+  const double* B = s_all + nd.q2off + (half ? o.ctot[0] : 0);
+  double* C = q_all + idx(nd.off + (half ? n1 : 0), nd.off, ldq);
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= k) return;
+  const double* Bj = B + (size_t)j * nd.sz;
+  for (int i = threadIdx.x & 63; i < rows; i += 64) {
+    double acc = 0;
+#pragma unroll 8
+    for (int l = 0; l < inner; ++l) acc += A[i + (size_t)l * rows] * Bj[l];
+    C[idx(i, j, ldq)] = acc;
+  }
+}
+
 // dlamrg of the K merged (ascending) and n - K deflated (descending) eigenvalues.
 // dlamrg(k, sz - k, d, 1, -1) per merge (one workgroup each): the K new eigenvalues ascending, the deflated ones
 // stored descending (read backwards), merged by ranks; on ties the first list's element comes first, as dlamrg.
@@ -915,8 +949,12 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
   hipLaunchKernelGGL(vectors_kernel, dim3(cdiv(maxsz, 4), cnt, zgrid()), dim3(256), 0, s, lvl, W.outs, W.dlamda,
                      W.orig, W.tau, W.wnew, W.indxc, W.sbuf, zb());
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), zgrid(2 * cnt)), dim3(256), 0, s, lvl,
-                     W.outs, W.q2, W.sbuf, z, ldz, zb(2 * cnt));
+  if (maxsz <= SMALL_MERGE)
+    hipLaunchKernelGGL(merge_small_kernel, dim3(cdiv(maxsz, 4), 1, zgrid(2 * cnt)), dim3(256), 0, s, lvl, W.outs, W.q2, W.sbuf, z,
+                       ldz, zb(2 * cnt));
+  else
+    hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), zgrid(2 * cnt)), dim3(256), 0, s,
+                       lvl, W.outs, W.q2, W.sbuf, z, ldz, zb(2 * cnt));
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(lamrg_kernel, dim3(cnt, 1, zgrid()), dim3(256), 0, s, cnt, lvl, W.outs, d, W.perm, zb());
   XTB_LAUNCH_CHECK();
