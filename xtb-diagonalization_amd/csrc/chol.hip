@@ -50,18 +50,16 @@ __device__ inline gemm_detail::Mfma<double>::acc_t mfma_block(const double* pa, 
 // - Y = L^{-1}: each wavefront inverts one 16 x 16 diagonal block by forward substitution (lane = column), then the
 //   sibling blocks at sizes 16 and 32, Y_BA = -Y_BB (L_BA Y_AA), as MFMA block products.
 // Writes U = L^T and X = U^{-1} = Y^T at (k0, k0) (zero below the diagonal); info = k0 + k + 1 at a non-positive
-// pivot k (then nothing is written).
-__global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* __restrict__ a, int lda,
-                                                         double* __restrict__ u, int ldu, double* __restrict__ x,
-                                                         int ldx, int* info, Zb zb) {
-  a = zb.at(a);
-  u = zb.at(u);
-  x = zb.at(x);
-  info = zb.at(info);
-  constexpr int LD = NB + 1;
-  __shared__ double Am[NB][LD];
-  __shared__ double Ym[NB][LD];
-  __shared__ double Tm[32][33];
+// pivot k (then nothing is written). sm: DIAG_LDS doubles of LDS.
+constexpr int DIAG_LD = NB + 1;
+constexpr int DIAG_LDS = 2 * NB * DIAG_LD + 32 * 33;
+__device__ __forceinline__ void diag_block(int n, int k0, const double* __restrict__ a, int lda,
+                                           double* __restrict__ u, int ldu, double* __restrict__ x, int ldx, int* info,
+                                           double* sm) {
+  constexpr int LD = DIAG_LD;
+  double(*Am)[LD] = reinterpret_cast<double(*)[LD]>(sm);
+  double(*Ym)[LD] = reinterpret_cast<double(*)[LD]>(sm + NB * LD);
+  double(*Tm)[33] = reinterpret_cast<double(*)[33]>(sm + 2 * NB * LD);
   __shared__ int bad_s;
   if (*info != 0) return;
   const int kb = min(NB, n - k0), t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -183,6 +181,42 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* 
   }
 }
 
+__global__ __launch_bounds__(256) void potrf_diag_kernel(int n, int k0, double* __restrict__ a, int lda,
+                                                         double* __restrict__ u, int ldu, double* __restrict__ x,
+                                                         int ldx, int* info, Zb zb) {
+  __shared__ double sm[DIAG_LDS];
+  diag_block(n, k0, zb.at(a), lda, zb.at(u), ldu, zb.at(x), ldx, zb.at(info), sm);
+}
+
+// One step's trailing update fused with the next step's diagonal block: A(rest, rest) -= P^T P on the tiles on and
+// above the diagonal (P = U(k0 block, rest), the row panel; tile q = J (J + 1) / 2 + I as in the plain upper-tile
+// product, so every tile sums the same products in the same order), and workgroup 0, after its tile (0, 0), which is
+// the whole of the next diagonal block, factors and inverts that block (diag_block at k1 = k0 + kb). The next
+// diagonal block is thus ready when the update is, instead of one launch later.
+__global__ __launch_bounds__(256) void potrf_update_diag_kernel(int n, int k1, int kb, const double* __restrict__ p,
+                                                                int ldu, double* __restrict__ a, int lda,
+                                                                double* __restrict__ u, double* __restrict__ x,
+                                                                int ldx, int* info, Zb zb) {
+  constexpr int LDS_ELEMS = gemm_detail::tile_lds_elems<double>() > DIAG_LDS ? gemm_detail::tile_lds_elems<double>()
+                                                                              : DIAG_LDS;
+  __shared__ __attribute__((aligned(16))) double lds[LDS_ELEMS];
+  p = zb.at(p);
+  a = zb.at(a);
+  u = zb.at(u);
+  x = zb.at(x);
+  info = zb.at(info);
+  const int r = n - k1, q = blockIdx.x;
+  int J = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+  while (J * (J + 1) / 2 > q) --J;
+  while ((J + 1) * (J + 2) / 2 <= q) ++J;
+  const int I = q - J * (J + 1) / 2;
+  gemm_detail::gemm_tile<double, true, false>(r, r, kb, -1.0, p, ldu, p, ldu, 1.0, a + idx(k1, k1, lda), lda,
+                                              I * gemm_detail::BM, J * gemm_detail::BN, lds);
+  if (q != 0) return;
+  __syncthreads();  // the tile's stores of A(k1 block, k1 block) before the block's loads, and its LDS reads
+  diag_block(n, k1, a, lda, u, ldu, x, ldx, info, lds);
+}
+
 }  // namespace
 
 namespace {
@@ -215,11 +249,12 @@ void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu,
   set_zero(s, n, n, u, ldu);
   set_zero(s, n, n, x, ldx);
   const bool beside = s2 != nullptr && ev != nullptr && ev_join != nullptr;
+  // the first diagonal block alone; every later one inside the previous step's trailing update
+  hipLaunchKernelGGL(potrf_diag_kernel, dim3(1, 1, zgrid()), dim3(256), 0, s, n, 0, a, lda, u, ldu, x, ldx, info,
+                     zb());
+  XTB_LAUNCH_CHECK();
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int kb = std::min(NB, n - k0), r = n - k0 - kb;
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1, 1, zgrid()), dim3(256), 0, s, n, k0, a, lda, u, ldu, x, ldx, info,
-                       zb());
-    XTB_LAUNCH_CHECK();
     if (beside && k0 > 0) {
       // X(0:k0, block) = -(X(0:k0, 0:k0) U(0:k0, block)) X_kk on s2: U's block column came from the row panels of
       // the earlier steps, X_kk from this step's diagonal kernel, X(0:k0, 0:k0) from s2's earlier steps
@@ -233,10 +268,12 @@ void potrf_inverse(hipStream_t s, int n, double* a, int lda, double* u, int ldu,
       gemm<double>(s, 'T', 'N', kb, r, kb, 1.0, x + idx(k0, k0, ldx), ldx, a + idx(k0, k0 + kb, lda), lda, 0.0,
                    u + idx(k0, k0 + kb, ldu), ldu);
       // A(rest, rest) -= U(k0 block, rest)^T U(k0 block, rest), on the tiles the later steps read: the diagonal
-      // blocks and everything above them (the diagonal kernel reads its whole 64 x 64 block, the row panels the part
-      // right of it)
-      gemm_upper_tn<double>(s, r, kb, -1.0, u + idx(k0, k0 + kb, ldu), ldu, u + idx(k0, k0 + kb, ldu), ldu, 1.0,
-                            a + idx(k0 + kb, k0 + kb, lda), lda);
+      // blocks and everything above them (the diagonal block reads its whole 64 x 64 block, the row panels the part
+      // right of it); then the next diagonal block
+      const int nt = cdiv(r, NB);
+      hipLaunchKernelGGL(potrf_update_diag_kernel, dim3(nt * (nt + 1) / 2, 1, zgrid()), dim3(256), 0, s, n, k0 + kb,
+                         kb, u + idx(k0, k0 + kb, ldu), ldu, a, lda, u, x, ldx, info, zb());
+      XTB_LAUNCH_CHECK();
     }
   }
   if (beside) {
