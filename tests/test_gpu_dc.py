@@ -86,3 +86,80 @@ def test_secular_roots_match_dlaed4(handle, torch_cuda, n):
                           torch.from_numpy(np.ascontiguousarray(q.T)).cuda(), n // 2, rho)
     allw = np.sort(np.concatenate([roots, ref["d"][k:]]))
     assert np.max(np.abs(out["d"] - allw)) <= 2e-15 * max(1.0, np.max(np.abs(allw)))
+
+
+def degenerate_tridiagonal(n, mult, seed):
+    """The tridiagonal of a dense matrix whose eigenvalues come in groups of mult equal values (its reduction leaves
+    off-diagonals of all sizes; the D&C merges then meet exactly equal poles, Givens deflations and secular roots
+    at the deflated values)."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(seed)
+    base = np.sort(rng.uniform(-1, 1, n // mult + 1))
+    lam = np.sort(np.tile(base, mult)[:n])
+    qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    a = (qm * lam) @ qm.T
+    t = sl.hessenberg((a + a.T) / 2)
+    return np.diag(t).copy(), np.diag(t, -1).copy()
+
+
+@pytest.mark.parametrize("mult,n", [(3, 600), (4, 600), (3, 1204), (2, 1204)])
+def test_stedc_degenerate_spectra(handle, torch_cuda, mult, n):
+    """Regression: with degenerate spectra the deflated eigenvalues of a merge can be a few ulps out of order (LAPACK's
+    dlaed2 leaves them so too); the merges' orderings must still be permutations (a collision there once returned a
+    duplicated eigenpair). Eigenvalues against host LAPACK, orthogonality and residual of the vectors."""
+    import scipy.linalg as sl
+    torch = torch_cuda
+    d, e = degenerate_tridiagonal(n, mult, 11 + n)
+    dd, ee = torch.from_numpy(d.copy()).cuda(), torch.from_numpy(e.copy()).cuda()
+    z = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    handle.dstedc(dd, ee, z)
+    assert handle.synchronize() == [0, 0, 0, 0]
+    w, zz = dd.cpu().numpy(), z.cpu().numpy().T
+    ref = sl.eigh_tridiagonal(d, e, eigvals_only=True)
+    t = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    # ascending, up to the rounding that separates the members of a degenerate group (dlamrg keeps such inversions)
+    assert np.all(np.diff(w) >= -4 * np.finfo(float).eps * np.max(np.abs(w)))
+    assert np.max(np.abs(w - ref)) < 1e-13
+    assert np.max(np.abs(zz.T @ zz - np.eye(n))) < 1e-13
+    assert np.max(np.abs(t @ zz - zz * w)) < 1e-13
+
+
+def test_merge_tree_degenerate(handle, torch_cuda):
+    """The same tree merge by merge through the one-merge probe (each merge's inputs the probe's own outputs of its
+    halves): every merge is an orthonormal eigendecomposition of its torn matrix."""
+    torch = torch_cuda
+    n = 600
+    d, e = degenerate_tridiagonal(n, 3, 11 + n)
+    nrm = max(np.abs(d).max(), np.abs(e).max())
+
+    def solve(d, e):
+        m = d.size
+        if m == 1:
+            return d.copy(), np.ones((1, 1))
+        n1 = m // 2
+        rho = e[n1 - 1]
+        d = d.copy()
+        d[n1 - 1] -= abs(rho)
+        d[n1] -= abs(rho)
+        w1, q1 = solve(d[:n1], e[:n1 - 1])
+        w2, q2 = solve(d[n1:], e[n1:])
+        dd = np.concatenate([w1, w2])
+        q = np.zeros((m, m))
+        q[:n1, :n1] = q1
+        q[n1:, n1:] = q2
+        out = xtbeig.dc_merge(handle, torch.from_numpy(dd.copy()).cuda(),
+                              torch.from_numpy(np.ascontiguousarray(q.T)).cuda(), n1, rho)
+        assert out["status"] == [0, 0, 0, 0], m
+        v = np.zeros(m)
+        v[n1 - 1] = 1
+        v[n1] = np.sign(rho)
+        mm = (q * dd) @ q.T + abs(rho) * np.outer(v, v)
+        w, qm = out["d"], out["q"]
+        assert np.all(np.diff(w) >= -4 * np.finfo(float).eps * np.max(np.abs(w))), m
+        assert np.max(np.abs(qm.T @ qm - np.eye(m))) < 1e-13, m
+        assert np.max(np.abs(mm @ qm - qm * w)) < 1e-13, m
+        return w, qm
+
+    w, _ = solve(d / nrm, e / nrm)
+    import scipy.linalg as sl
+    assert np.max(np.abs(w * nrm - sl.eigh_tridiagonal(d, e, eigvals_only=True))) < 1e-13

@@ -213,30 +213,58 @@ __global__ __launch_bounds__(256) void deflate_kernel(const Node* nodes, const i
   }
   __syncthreads();
   const double rho = fabs(2 * rho_in);
-  // dlamrg of the two ascending halves by ranks: on ties the first half's element comes first
-  for (int i = t; i < n; i += nt) {
-    int r;
-    const double v = DL[i];
-    if (i < n1) {
-      int lo = 0, hi = n2;  // # second-half elements < v
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (DL[n1 + mid] < v) lo = mid + 1;
-        else hi = mid;
+  // dlamrg of the two ascending halves. By ranks when both halves are in order (on ties the first half's element
+  // comes first, so each rank is the position dlamrg's sequential merge gives the element); a half that rounding left
+  // a few ulps out of order (or a NaN) would make ranks collide, and then thread 0 runs dlamrg's sequential merge,
+  // which yields a permutation whatever the order.
+  if (t == 0) sh_serial = 0;
+  __syncthreads();
+  for (int i = t; i + 1 < n; i += nt)
+    if (i + 1 != n1 && !(DL[i] <= DL[i + 1])) atomicOr(&sh_serial, 1);
+  __syncthreads();
+  if (sh_serial) {
+    if (t == 0) {
+      int i1 = 0, i2 = n1, o = 0;
+      while (i1 < n1 && i2 < n) {
+        const int src = (DL[i1] <= DL[i2]) ? i1++ : i2++;
+        INDXC[o] = src + 1;
+        INDX[o++] = src + 1;
       }
-      r = i + lo;
-    } else {
-      int lo = 0, hi = n1;  // # first-half elements <= v
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (DL[mid] <= v) lo = mid + 1;
-        else hi = mid;
+      while (i1 < n1) {
+        INDXC[o] = i1 + 1;
+        INDX[o++] = ++i1;
       }
-      r = (i - n1) + lo;
+      while (i2 < n) {
+        INDXC[o] = i2 + 1;
+        INDX[o++] = ++i2;
+      }
     }
-    INDXC[r] = i + 1;
-    INDX[r] = i + 1;
+  } else {
+    for (int i = t; i < n; i += nt) {
+      int r;
+      const double v = DL[i];
+      if (i < n1) {
+        int lo = 0, hi = n2;  // # second-half elements < v
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (DL[n1 + mid] < v) lo = mid + 1;
+          else hi = mid;
+        }
+        r = i + lo;
+      } else {
+        int lo = 0, hi = n1;  // # first-half elements <= v
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (DL[mid] <= v) lo = mid + 1;
+          else hi = mid;
+        }
+        r = (i - n1) + lo;
+      }
+      INDXC[r] = i + 1;
+      INDX[r] = i + 1;
+    }
   }
+  __syncthreads();
   // idamax of z and of d (first index of the largest magnitude)
   {
     double bz = -1, bd = -1;
@@ -729,12 +757,18 @@ __global__ __launch_bounds__(256) void merge_small_kernel(const Node* nodes, con
 // dlamrg(k, sz - k, d, 1, -1) per merge (one workgroup each): the K new eigenvalues ascending, the deflated ones
 // stored descending (read backwards), merged by ranks; on ties the first list's element comes first, as dlamrg.
 // K = 0 (every column deflated at once) leaves them ascending: the identity.
+// The deflated values are not always in exact order: a Givens deflation of close poles recombines two values, and
+// LAPACK's dlaed2 itself can leave neighbours a few ulps out of order (with a secular root between them, as
+// degenerate spectra produce, ranks would collide). Then thread 0 runs dlamrg's sequential merge on an LDS copy,
+// which yields a permutation whatever the order. Dynamic LDS: sz doubles.
 __global__ __launch_bounds__(256) void lamrg_kernel(int count, const Node* nodes, const NodeOut* out,
                                                     const double* __restrict__ d_all, int* __restrict__ perm_all,
                                                     Zb zb) {
   out = zb.at(out);
   d_all = zb.at(d_all);
   perm_all = zb.at(perm_all);
+  extern __shared__ double s_a[];
+  __shared__ int unsorted;
   const Node nd = nodes[blockIdx.x];
   const int k = out[blockIdx.x].k, sz = nd.sz, n2 = sz - k;
   const double* a = d_all + nd.off;
@@ -743,6 +777,24 @@ __global__ __launch_bounds__(256) void lamrg_kernel(int count, const Node* nodes
   if (k == 0) {
     // every column deflated at once: they are already ascending
     for (int i = threadIdx.x; i < sz; i += blockDim.x) perm[i] = i + 1;
+    return;
+  }
+  if (threadIdx.x == 0) unsorted = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i + 1 < sz; i += blockDim.x) {
+    const bool ok = (i + 1 < k) ? (a[i] <= a[i + 1]) : (i < k || a[i] >= a[i + 1]);
+    if (!ok) atomicOr(&unsorted, 1);
+  }
+  __syncthreads();
+  if (unsorted) {
+    for (int i = threadIdx.x; i < sz; i += blockDim.x) s_a[i] = a[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int i1 = 0, i2 = sz - 1, o = 0;
+      while (i1 < k && i2 >= k) perm[o++] = (s_a[i1] <= s_a[i2]) ? ++i1 : 1 + i2--;
+      while (i1 < k) perm[o++] = ++i1;
+      while (i2 >= k) perm[o++] = 1 + i2--;
+    }
     return;
   }
   for (int i = threadIdx.x; i < sz; i += blockDim.x) {
@@ -770,11 +822,13 @@ __global__ __launch_bounds__(256) void lamrg_kernel(int count, const Node* nodes
 }
 
 // temp(:, i) = column perm[i] of the merged system: the merge products (Q(:, 0 ..< K)) or a deflated eigenvector
-// (Q2's last group); dtemp(i) = d(perm[i]).
+// (Q2's last group); dtemp(i) = d(perm[i]). A source outside the merge (a permutation that is not one) reads
+// nothing and sets info.
 __global__ void gather_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ perm_all,
                               const double* __restrict__ q_all, int ldq, const double* __restrict__ q2_all,
                               const double* __restrict__ d_all, double* __restrict__ tmp_all,
-                              double* __restrict__ dtmp_all, Zb zb) {
+                              double* __restrict__ dtmp_all, int* info, Zb zb) {
+  info = zb.at(info);
   out = zb.at(out);
   perm_all = zb.at(perm_all);
   q_all = zb.at(q_all);
@@ -788,6 +842,10 @@ __global__ void gather_kernel(const Node* nodes, const NodeOut* out, const int* 
   const NodeOut o = out[zb.z()];
   const int n = nd.sz, n1 = nd.n1, n2 = n - n1, k = o.k;
   const int src = perm_all[nd.off + col] - 1;
+  if (src < 0 || src >= n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicCAS(info, 0, nd.off + col + 1);
+    return;
+  }
   const double* from;
   if (src < k) {
     from = q_all + idx(nd.off, nd.off + src, ldq);
@@ -956,10 +1014,11 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
     hipLaunchKernelGGL(merge_gemm_kernel, dim3(cdiv(maxsz, 64), cdiv(maxsz, 64), zgrid(2 * cnt)), dim3(256), 0, s,
                        lvl, W.outs, W.q2, W.sbuf, z, ldz, zb(2 * cnt));
   XTB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(lamrg_kernel, dim3(cnt, 1, zgrid()), dim3(256), 0, s, cnt, lvl, W.outs, d, W.perm, zb());
+  hipLaunchKernelGGL(lamrg_kernel, dim3(cnt, 1, zgrid()), dim3(256), sizeof(double) * maxsz, s, cnt, lvl, W.outs, d,
+                     W.perm, zb());
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(gather_kernel, dim3(cdiv(maxsz, 256), maxsz, zgrid(cnt)), dim3(256), 0, s, lvl, W.outs, W.perm,
-                     z, ldz, W.q2, d, W.sbuf, W.dtmp, zb(cnt));
+                     z, ldz, W.q2, d, W.sbuf, W.dtmp, info, zb(cnt));
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(scatter_kernel, dim3(cdiv(maxsz, 256), maxsz, zgrid(cnt)), dim3(256), 0, s, lvl, W.sbuf, W.dtmp,
                      z, ldz, d, zb(cnt));
