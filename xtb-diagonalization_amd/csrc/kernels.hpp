@@ -75,6 +75,14 @@ template <typename T>
 void set_zero(hipStream_t s, int m, int n, T* a, int lda);
 template <typename T>
 void set_identity(hipStream_t s, int n, T* a, int lda);
+// The FP32 copy of an FP64 matrix for the mixed path, kept inside FP32's range: *amax (one 64-bit word, cleared here)
+// receives max |src|; while that lies in [2^-60, 2^60] (squares and sums of squares stay normal FP32 numbers) dst =
+// src rounded, else dst = 2^-e src with 2^e the power of two just above max |src| (|e| <= 1000; a power of two
+// commutes with every rounding of the FP32 reduction, chase and products, so the solve is that of src up to the
+// factor). A non-finite src gives dst = 0 and *status = 1. unscale_values multiplies w (n) by the same 2^e.
+void copy_to_fp32_ranged(hipStream_t s, int n, const double* src, int lds, float* dst, int ldd,
+                         unsigned long long* amax, int* status);
+void unscale_values(hipStream_t s, int n, double* w, const unsigned long long* amax);
 // dst = the symmetric matrix of src's upper triangle (xtb_dsymmetrize_pad).
 void symmetrize_pad(hipStream_t s, int n, const double* src, int lds, double* dst, int ldd);
 

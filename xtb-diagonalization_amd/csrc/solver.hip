@@ -298,6 +298,7 @@ static void carve_core_bytes(Carver& c, int n, int b, bool mixed, int m) {
     c.take<double>(refine_workspace(n));  // also H X and H' in FP64 before the refinement
     c.take<int>(refine_iwork(n));
     if (m < n) c.take<double>(nn);  // all n refined eigenvectors, of which m are returned
+    c.take<double>(16);              // max |H'| for the FP32 copy's range
   }
 }
 
@@ -342,6 +343,7 @@ static void sygvd_core(Handle& h, Carver& c, int n, const double* hm, int ldh, c
     int* riwork = c.take<int>(refine_iwork(n));
     double* cfull = m < n ? c.take<double>(nn) : cm;
     const int ldfull = m < n ? n : ldc;
+    unsigned long long* amax = reinterpret_cast<unsigned long long*>(c.take<double>(16));
     // H' = X^T H X in FP64, then rounded: formed in FP32, its rounding error grows with cond(S) (about
     // eps32 cond(S) ||H||), which at cond(S) = 1e4 leaves the FP32 eigenvectors too far off for the refinement to
     // converge; in FP64 the FP32 solve starts from eps32 ||H'|| whatever S. The two products are a small part of
@@ -353,9 +355,12 @@ static void sygvd_core(Handle& h, Carver& c, int n, const double* hm, int ldh, c
     gemm_upper_tn<double>(st, n, n, 1.0, x, ldx, hx64, n, 0.0, hp64, n);
     symmetrize<double>(st, 'U', n, hp64, n);
     if (pad) pad_spectrum(st, n, pad->n, hp64, n, pad->h, pad->ldh);
-    copy_matrix<double, float>(st, n, n, hp64, n, hp, n);
+    // rounded to FP32 inside its range (a power-of-two scale for an H' near or past it, undone on the eigenvalues;
+    // a non-finite H' is not solved in FP32 and fails the check)
+    copy_to_fp32_ranged(st, n, hp64, n, hp, n, amax, h.dinfo + 3);
     // the refinement needs the whole basis: all n vectors are formed and refined, the first m returned
     syevd_core<float>(h, s, n, b, hp, n, w, c32, n, x32, n, pm);
+    unscale_values(st, n, w, amax);
     copy_matrix<float, double>(st, n, n, c32, n, cfull, ldfull);
     refine_oa(st, n, hm, ldh, sm, lds, cfull, ldfull, w, h.refine_iters, rwork, riwork, h.dinfo + 3,
               h.check_ratios());

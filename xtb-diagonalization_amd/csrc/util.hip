@@ -50,6 +50,58 @@ __global__ void symmetrize_kernel(int n, T* a, int lda, bool upper, Zb zb) {
   }
 }
 
+// max |a(i, j)| into *amax by atomicMax on the bit pattern of |a(i, j)| (for non-negative doubles the patterns
+// order as the values; a NaN lies above every other).
+__global__ __launch_bounds__(256) void absmax_kernel(int m, int n, const double* __restrict__ a, int lda,
+                                                     unsigned long long* amax, Zb zb) {
+  a = zb.at(a);
+  amax = zb.at(amax);
+  __shared__ unsigned long long red[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  for (int j = blockIdx.y; j < n; j += gridDim.y)
+    if (i < m) v = max(v, (unsigned long long)__double_as_longlong(fabs(a[idx(i, j, lda)])));
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned long long)__shfl_xor((long long)v, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = max(max(red[0], red[1]), max(red[2], red[3]));
+    if (v) atomicMax(amax, v);
+  }
+}
+
+// 2^-e for the FP32 copy (1 while max |src| lies in [2^-60, 2^60], or is zero or not finite)
+__device__ inline double fp32_scale(unsigned long long bits) {
+  const double m = __longlong_as_double((long long)bits);
+  if (!(m > 0.0) || !isfinite(m) || (m >= 0x1p-60 && m <= 0x1p60)) return 1.0;
+  int e;
+  (void)frexp(m, &e);  // m = f 2^e, f in [0.5, 1)
+  return ldexp(1.0, min(1000, max(-1000, -e)));  // a normal double either way (subnormal m: e down to -1073)
+}
+
+__global__ void copy_ranged_kernel(int m, int n, const double* __restrict__ src, int lds, float* __restrict__ dst,
+                                   int ldd, const unsigned long long* amax, int* status, Zb zb) {
+  src = zb.at(src);
+  dst = zb.at(dst);
+  amax = zb.at(amax);
+  status = zb.at(status);
+  const unsigned long long bits = *amax;
+  const bool finite = isfinite(__longlong_as_double((long long)bits));
+  const double sc = fp32_scale(bits);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!finite && i == 0 && blockIdx.y == 0) atomicExch(status, 1);
+  for (int j = blockIdx.y; j < n; j += gridDim.y)
+    if (i < m) dst[idx(i, j, ldd)] = finite ? float(sc * src[idx(i, j, lds)]) : 0.0f;
+}
+
+__global__ void unscale_values_kernel(int n, double* w, const unsigned long long* amax, Zb zb) {
+  w = zb.at(w);
+  amax = zb.at(amax);
+  const double sc = fp32_scale(*amax);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && sc != 1.0) w[i] /= sc;  // a power of two: exact
+}
+
 template <typename T>
 __global__ void fill_kernel(int m, int n, T* a, int lda, T diag, Zb zb) {
   a = zb.at(a);
@@ -115,6 +167,24 @@ void copy_matrix(hipStream_t s, int m, int n, const S* src, int lds, D* dst, int
   if (m <= 0 || n <= 0) return;
   hipLaunchKernelGGL((copy_kernel<S, D>), dim3(cdiv(m, 256), n < 65535 ? n : 65535, zgrid()), dim3(256), 0, s, m, n,
                      src, lds, dst, ldd, zb());
+  XTB_LAUNCH_CHECK();
+}
+
+void copy_to_fp32_ranged(hipStream_t s, int n, const double* src, int lds, float* dst, int ldd,
+                         unsigned long long* amax, int* status) {
+  if (n <= 0) return;
+  set_zero<double>(s, 1, 1, reinterpret_cast<double*>(amax), 1);
+  hipLaunchKernelGGL(absmax_kernel, dim3(cdiv(n, 256), std::min(n, 64), zgrid()), dim3(256), 0, s, n, n, src, lds,
+                     amax, zb());
+  XTB_LAUNCH_CHECK();
+  hipLaunchKernelGGL(copy_ranged_kernel, dim3(cdiv(n, 256), n < 65535 ? n : 65535, zgrid()), dim3(256), 0, s, n, n,
+                     src, lds, dst, ldd, amax, status, zb());
+  XTB_LAUNCH_CHECK();
+}
+
+void unscale_values(hipStream_t s, int n, double* w, const unsigned long long* amax) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(unscale_values_kernel, dim3(cdiv(n, 256), 1, zgrid()), dim3(256), 0, s, n, w, amax, zb());
   XTB_LAUNCH_CHECK();
 }
 
