@@ -163,3 +163,57 @@ def test_merge_tree_degenerate(handle, torch_cuda):
     w, _ = solve(d / nrm, e / nrm)
     import scipy.linalg as sl
     assert np.max(np.abs(w * nrm - sl.eigh_tridiagonal(d, e, eigvals_only=True))) < 1e-13
+
+
+def classic_tridiagonals():
+    """Standard hard cases for tridiagonal eigensolvers (LAPACK's test matrices among them): Wilkinson W+ (pairs of
+    eigenvalues that agree to many digits), glued Wilkinson blocks (clusters), Clement (integer spectrum), Legendre,
+    a split matrix (30 % of the couplings exactly zero), a diagonal, equal diagonal entries with couplings near
+    underflow, scales 1e+-300 and a graded matrix."""
+    rng = np.random.default_rng(3)
+    n = 300
+    out = {}
+
+    def wilk(m):
+        return np.abs(m - np.arange(2 * m + 1)).astype(float), np.ones(2 * m)
+
+    out["wilkinson201"] = wilk(100)
+    d0, e0 = wilk(10)
+    for k, delta in ((14, 1e-14), (57, 1e-8)):
+        out[f"glued_w21x{k}_{delta:g}"] = (np.tile(d0, k),
+                                          np.concatenate([np.concatenate([e0, [delta]]) for _ in range(k)])[:-1])
+    out["clement"] = (np.zeros(n), np.sqrt(np.arange(1, n) * (n - np.arange(1, n))))
+    i = np.arange(1, n)
+    out["legendre"] = (np.zeros(n), i / np.sqrt(4.0 * i * i - 1))
+    e = rng.uniform(-1, 1, n - 1)
+    e[rng.uniform(size=n - 1) < 0.3] = 0.0
+    out["split30"] = (rng.uniform(-1, 1, n), e)
+    out["diagonal"] = (rng.uniform(-1, 1, n), np.zeros(n - 1))
+    out["const_d_tiny_e"] = (np.ones(n), np.full(n - 1, 1e-300))
+    out["scaled1e300"] = (1e300 * rng.uniform(-1, 1, n), 1e300 * rng.uniform(-1, 1, n - 1))
+    out["scaled1e-300"] = (1e-300 * rng.uniform(-1, 1, n), 1e-300 * rng.uniform(-1, 1, n - 1))
+    out["graded"] = (10.0 ** (-np.arange(n) / 25.0), 0.5 * 10.0 ** (-np.arange(n - 1) / 25.0))
+    return out
+
+
+CLASSIC = classic_tridiagonals()
+
+
+@pytest.mark.parametrize("name", list(CLASSIC))
+def test_stedc_classic_tridiagonals(handle, torch_cuda, name):
+    """Host LAPACK's eigenvalues, orthogonality and residual, each relative to the spectrum's largest magnitude."""
+    import scipy.linalg as sl
+    torch = torch_cuda
+    d, e = CLASSIC[name]
+    n = d.size
+    dd, ee = torch.from_numpy(d.copy()).cuda(), torch.from_numpy(e.copy()).cuda()
+    z = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    handle.dstedc(dd, ee, z)
+    assert handle.synchronize() == [0, 0, 0, 0]
+    w, zz = dd.cpu().numpy(), z.cpu().numpy().T
+    ref = sl.eigh_tridiagonal(d, e, eigvals_only=True)
+    scale = max(np.abs(ref).max(), np.finfo(float).tiny)
+    t = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    assert np.max(np.abs(np.sort(w) - ref)) < 1e-13 * scale
+    assert np.max(np.abs(t @ zz - zz * w)) < 1e-13 * scale
+    assert np.max(np.abs(zz.T @ zz - np.eye(n))) < 1e-13

@@ -516,7 +516,9 @@ __global__ void rotate_kernel(const Node* nodes, const NodeOut* out, const int* 
 // Q2 = [ n1 x n12 | n2 x n23 | n x ctot3 ]: column pos of the grouped order from Q's column INDX(pos) (dlaed2's
 // copies). One workgroup per column.
 __global__ void q2_gather_kernel(const Node* nodes, const NodeOut* out, const int* __restrict__ indx_all,
-                                 const double* __restrict__ q_all, int ldq, double* __restrict__ q2_all, Zb zb) {
+                                 const double* __restrict__ q_all, int ldq, double* __restrict__ q2_all, int* info,
+                                 Zb zb) {
+  info = zb.at(info);
   out = zb.at(out);
   indx_all = zb.at(indx_all);
   q_all = zb.at(q_all);
@@ -533,6 +535,10 @@ __global__ void q2_gather_kernel(const Node* nodes, const NodeOut* out, const in
   double* q2b = q2a + (size_t)n1 * n12;
   double* q2c = q2b + (size_t)n2 * n23;
   const int js = indx_all[nd.off + pos] - 1;
+  if (js < 0 || js >= n) {  // not a column of this merge (a broken permutation): read nothing, report
+    if (threadIdx.x == 0) atomicCAS(info, 0, nd.off + pos + 1);
+    return;
+  }
   const double* src = q + (int64_t)js * ldq;
   const int t = threadIdx.x;
   if (pos < c0) {
@@ -986,7 +992,7 @@ void run_level(hipStream_t s, int n, const Node* lvl, const int* lvl_pre, int cn
                      W.rotcs, z, ldz, zb());
   XTB_LAUNCH_CHECK();
   hipLaunchKernelGGL(q2_gather_kernel, dim3(maxsz, cnt, zgrid()), dim3(256), 0, s, lvl, W.outs, W.indx, z, ldz, W.q2,
-                     zb());
+                     info, zb());
   XTB_LAUNCH_CHECK();
   if (probe) {
     XTB_HIP(hipMemcpyAsync(probe->d_defl, d, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
