@@ -156,6 +156,7 @@ int xtbeig_dev_sygvd(xtbeig_handle_t handle, int precision, int n, const double 
                      int lds, double *w, double *c, int ldc);
 /* Standard solve of a device-resident symmetric A (full storage, not modified). */
 int xtbeig_dev_dsyevd(xtbeig_handle_t handle, int n, const double *a, int lda, double *w, double *z, int ldz);
+/* FP32 a and z; the eigenvalues w are FP64 (n doubles: the divide and conquer runs in FP64). */
 int xtbeig_dev_ssyevd(xtbeig_handle_t handle, int n, const float *a, int lda, double *w, float *z, int ldz);
 /* All n eigenvalues and the first m eigenvectors (1 <= m <= n) of the symmetric a: the last product of the
  * back-transformation costs 2 n^2 m instead of 2 n^3 (xTB needs the occupied columns only, LOG.md:310-312). */

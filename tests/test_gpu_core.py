@@ -233,6 +233,28 @@ def test_syevd(handle, torch_cuda, n, prec):
         assert g["eig_err"] < 1e-5 and g["orth"] < 1e-5 * n, g
 
 
+def test_device_calls_refuse_mistyped_buffers(handle, torch_cuda):
+    # the FP32 standard solve writes FP64 eigenvalues: an FP32 w (half the bytes) is refused before any launch, as
+    # are host tensors and buffers too small for the result
+    torch = torch_cuda
+    n = 64
+    A = torch.eye(n, dtype=torch.float32, device="cuda")
+    Z = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+    with pytest.raises(ValueError, match="float64"):
+        handle.ssyevd(A, torch.zeros(n, dtype=torch.float32, device="cuda"), Z)
+    with pytest.raises(ValueError, match="device tensor"):
+        handle.ssyevd(A, torch.zeros(n, dtype=torch.float64), Z)
+    with pytest.raises(ValueError, match="at least"):
+        handle.dsyevd(A.double(), torch.zeros(n - 1, dtype=torch.float64, device="cuda"), Z.double())
+    W = torch.zeros(n, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="float64"):
+        handle.sygvd(A, A, W, Z, xtbeig.FP64)
+    assert handle.synchronize() == [0, 0, 0, 0]
+    handle.ssyevd(A, W, Z)
+    assert handle.synchronize() == [0, 0, 0, 0]
+    assert float((W - 1).abs().max()) < 1e-6
+
+
 @pytest.mark.parametrize("n", [17, 131, 412])
 @pytest.mark.parametrize("mixed", [False, True])
 def test_sygvd_device(handle, torch_cuda, n, mixed):

@@ -77,3 +77,43 @@ def test_adversarial(handle, torch_cuda, name):
             assert st[3] == 0 and ok, (name, eig, res, orth)
         else:
             assert ok or st[3] != 0, (name, eig, res, orth, st)
+
+
+def fp32_near_underflow_cases():
+    rng = np.random.default_rng(21)
+    n = N
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    out = {"rank1": np.ones((n, n))}  # a 299-fold zero eigenvalue: the chase's bulges decay below FP32's squares
+    h = n // 2
+    a = np.zeros((n, n))
+    qa, _ = np.linalg.qr(rng.standard_normal((h, h)))
+    qb, _ = np.linalg.qr(rng.standard_normal((n - h, n - h)))
+    a[:h, :h] = (qa * rng.uniform(-1, 1, h)) @ qa.T
+    a[h:, h:] = 1e-21 * (qb * rng.uniform(-1, 1, n - h)) @ qb.T  # panels whose columns' squares underflow in FP32
+    out["block_1e-21"] = (a + a.T) / 2
+    return out
+
+
+FP32_CASES = fp32_near_underflow_cases()
+
+
+@pytest.mark.parametrize("name", list(FP32_CASES))
+def test_fp32_reflectors_stay_orthogonal(handle, torch_cuda, name):
+    """The FP32 standard solve where Householder vectors get so small that their squares underflow in FP32 (the
+    bulges of a massively degenerate spectrum, a block 1e-21 below the rest): the reflectors are formed in FP64
+    there, so the eigenvectors stay orthogonal to FP32 rounding (a sum of squares that lost terms to underflow once
+    gave an orthogonality error of 0.24). Eigenvalues to FP32 rounding of ||A||."""
+    torch = torch_cuda
+    a = FP32_CASES[name]
+    n = a.shape[0]
+    A = torch.from_numpy(np.ascontiguousarray(a.T)).cuda().float()
+    w = torch.zeros(n, dtype=torch.float64, device="cuda")  # the eigenvalues come back in FP64
+    z = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+    handle.ssyevd(A, w, z)
+    assert handle.synchronize() == [0, 0, 0, 0]
+    ww, zz = w.double().cpu().numpy(), z.double().cpu().numpy().T
+    ref = np.linalg.eigvalsh(a)
+    nrm = np.abs(ref).max()
+    print(name, np.max(np.abs(ww - ref)) / nrm, np.max(np.abs(zz.T @ zz - np.eye(n))), flush=True)
+    assert np.max(np.abs(ww - ref)) < 1e-5 * nrm
+    assert np.max(np.abs(zz.T @ zz - np.eye(n))) < 1e-5 * n ** 0.5

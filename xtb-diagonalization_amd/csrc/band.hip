@@ -62,6 +62,7 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
   __shared__ T red[4 * SB];
   __shared__ T tot[SB];
   __shared__ T sh_alpha;
+  __shared__ double redd[4], totd[1];
   const int t = threadIdx.x;
   const int k = min(m, nb);
   T x[R][SB];
@@ -102,33 +103,34 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
     for (int c = 0; c < SB; ++c) {
       const int col = c0 + c;
       if (col < k) {
-        T part1[1];
-        part1[0] = 0;
+        // the column's tail norm and the reflector's scalars in FP64 (for FP32 panels too: a square of an FP32 entry
+        // never underflows in FP64, so that the reflector stays orthogonal however small the column is); the entries
+        // v_i = x_i / (alpha - beta) are rounded once to T
+        double part1[1];
+        part1[0] = 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int i = t + r * QR_THREADS;
           if (i == col) sh_alpha = x[r][c];
-          if (i > col && i < m) part1[0] += x[r][c] * x[r][c];
+          if (i > col && i < m) part1[0] += (double)x[r][c] * (double)x[r][c];
         }
-        block_sums<T, 1>(part1, red, tot);
-        const T alpha = sh_alpha, xn2 = tot[0];
-        T beta, tq, scal;
-        if (xn2 == T(0)) {
-          tq = 0; beta = alpha; scal = 0;
-        } else {
+        block_sums<double, 1>(part1, redd, totd);
+        const double alpha = sh_alpha, xn2 = totd[0];
+        double beta = alpha, tq = 0.0, scal = 0.0;
+        if (xn2 != 0.0) {
           beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
           tq = (beta - alpha) / beta;
-          scal = T(1) / (alpha - beta);
+          scal = 1.0 / (alpha - beta);
         }
         T v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int i = t + r * QR_THREADS;
-          v[r] = (i == col) ? T(1) : ((i > col && i < m) ? x[r][c] * scal : T(0));
-          if (i == col) x[r][c] = beta;
+          v[r] = (i == col) ? T(1) : ((i > col && i < m) ? T((double)x[r][c] * scal) : T(0));
+          if (i == col) x[r][c] = T(beta);
           else if (i > col) x[r][c] = 0;
         }
-        if (t == 0) tau[col] = tq;
+        if (t == 0) tau[col] = T(tq);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int i = t + r * QR_THREADS;
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(QR_THREADS) void panel_qr_kernel(int m, int nb, T* 
 #pragma unroll
           for (int cc = 0; cc < SB; ++cc) {
             if (cc > c) {
-              const T f = tq * tot[cc];
+              const T f = T(tq) * tot[cc];
 #pragma unroll
               for (int r = 0; r < R; ++r) x[r][cc] -= v[r] * f;
             }
@@ -235,6 +237,7 @@ __device__ __forceinline__ void factor_panel(int m, T (&x)[R][NB], T* __restrict
   __shared__ T red[2][WAVES][NB];
   __shared__ T rowb[2][NB];
   __shared__ T Ts[NB][NB + 1];
+  __shared__ double redd[WAVES * NB], totd[NB];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   const int k = min(m, NB);
   // T's strictly lower part is read (times a zero mask) by the branch-free T update: it must hold zeros, not whatever
@@ -265,17 +268,57 @@ __device__ __forceinline__ void factor_panel(int m, T (&x)[R][NB], T* __restrict
       T tot[NB];
 #pragma unroll
       for (int j = 0; j < NB; ++j) tot[j] = (red[par][0][j] + red[par][1][j]) + (red[par][2][j] + red[par][3][j]);
+      // FP32: row min(t, NB - 1) of T's finished columns read with the sums, before the branch below, so that the
+      // LDS reads still overlap (the entries past column c - 1 are masked below)
+      T trow[NB];
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) trow[j] = Ts[min(t, NB - 1)][max(j - NB + c, 0)];
+      }
       const T alpha = rowb[par][0], xn2 = tot[0];
       T beta = alpha, tq = 0, scal = 0;
-      if (xn2 != T(0)) {
-        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-        tq = (beta - alpha) / beta;
-        scal = T(1) / (alpha - beta);
-      }
       // g[j] = v_l^T v_c at a finished column's position j = NB - c + l, v^T x_j / ... at an unfinished one's
       T g[NB];
+      // FP32 sum of squares near underflow (below 2^-100, where squares and products that underflow are no longer
+      // negligible): every sum of the step again in FP64 (the same products), so that the reflector and its column
+      // of T stay orthogonal however small the column is. The test is uniform (every thread read the same sum).
+      bool wide = false;
+      if constexpr (sizeof(T) == 4) wide = __builtin_amdgcn_readfirstlane(xn2 >= T(0x1p-100) ? 0 : 1) != 0;
+      if (wide) {
+        double partd[NB];
 #pragma unroll
-      for (int j = 0; j < NB; ++j) g[j] = rowb[par][j] + scal * tot[j];
+        for (int j = 0; j < NB; ++j) partd[j] = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double xc = (r > 0 || t > c) ? (double)x[r][0] : 0.0;
+#pragma unroll
+          for (int j = 0; j < NB; ++j) partd[j] += xc * (double)x[r][j];
+        }
+        block_sums<double, NB>(partd, redd, totd);
+        const double xd = totd[0];
+        double scald = 0.0;
+        if (xd != 0.0) {
+          const double a = alpha, bd = -copysign(sqrt(a * a + xd), a);
+          beta = T(bd);
+          tq = T((bd - a) / bd);
+          scald = 1.0 / (a - bd);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) g[j] = T((double)rowb[par][j] + scald * totd[j]);
+        // the reflector's entries, each rounded once from FP64, replace the column: the update below scales by 1
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r > 0 || t > c) x[r][0] = T((double)x[r][0] * scald);
+        scal = T(1);
+      } else {
+        if (xn2 != T(0)) {
+          beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+          tq = (beta - alpha) / beta;
+          scal = T(1) / (alpha - beta);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) g[j] = rowb[par][j] + scal * tot[j];
+      }
       {
         // T(t, c) = -tau_c sum_{l = t}^{c - 1} T(t, l) G(l, c), branch-free so that the LDS reads overlap
         const int tr = min(t, NB - 1);
@@ -284,7 +327,7 @@ __device__ __forceinline__ void factor_panel(int m, T (&x)[R][NB], T* __restrict
         for (int j = 0; j < NB; ++j) {
           const int l = j - NB + c;
           const T gm = (l >= t) ? g[j] : T(0);  // the mask on g keeps the LDS read unconditional
-          sum += Ts[tr][max(l, 0)] * gm;
+          sum += (sizeof(T) == 4 ? trow[j] : Ts[tr][max(l, 0)]) * gm;
         }
         if (t < c) Ts[t][c] = -tq * sum;
       }

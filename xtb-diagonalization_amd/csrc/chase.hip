@@ -54,14 +54,29 @@ __device__ inline bool chase_bulge(int n, int b, const Band<T>& A, T* __restrict
   const T xi = (lane < len) ? A(rs + lane, c) : T(0);
   const T alpha = __shfl(xi, 0, 64);
   const T xn2 = wave_sum((lane >= 1 && lane < len) ? xi * xi : T(0));
-  T tau = 0, beta = alpha, scal = 0;
-  if (xn2 != T(0)) {
-    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-    tau = (beta - alpha) / beta;
-    scal = T(1) / (alpha - beta);
+  T tau = 0, beta = alpha, wi;
+  if (sizeof(T) == 4 && !(xn2 >= T(0x1p-100))) {
+    // FP32 sum of squares near underflow: the reflector from the entries in FP64 (see lane_reflector)
+    const double xd = wave_sum((lane >= 1 && lane < len) ? (double)xi * (double)xi : 0.0);
+    double sqd = 0.0, scald = 0.0;
+    if (xd != 0.0) {
+      const double a = alpha, bd = -copysign(sqrt(a * a + xd), a), taud = (bd - a) / bd;
+      beta = T(bd);
+      tau = T(taud);
+      sqd = sqrt(taud);
+      scald = 1.0 / (a - bd);
+    }
+    wi = (lane == 0) ? T(sqd) : ((lane < len) ? T(sqd * ((double)xi * scald)) : T(0));
+  } else {
+    T scal = 0;
+    if (xn2 != T(0)) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tau = (beta - alpha) / beta;
+      scal = T(1) / (alpha - beta);
+    }
+    const T sq = sqrt(tau);
+    wi = (lane == 0) ? sq : ((lane < len) ? sq * (xi * scal) : T(0));
   }
-  const T sq = sqrt(tau);
-  const T wi = (lane == 0) ? sq : ((lane < len) ? sq * (xi * scal) : T(0));
   if (lane < len) {
     v2[(int64_t)(rs + lane) + (int64_t)ldv2 * s] = wi;
     wsh[lane] = wi;
@@ -150,20 +165,42 @@ __device__ inline bool chase_bulge_reg(int n, int b, const Band<T>& A, T* __rest
   }
   const T xn2 = wave_sum(part);
   const T alpha = wave_sum(x0);
-  T tau = 0, beta = alpha, scal = 0;
-  if (xn2 != T(0)) {
-    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-    tau = (beta - alpha) / beta;
-    scal = T(1) / (alpha - beta);
+  T tau = 0, beta = alpha;
+  // FP32 sum of squares near underflow: the reflector from the entries in FP64 (see lane_reflector)
+  const bool wide = sizeof(T) == 4 && !(xn2 >= T(0x1p-100));
+  double sqd = 0.0, scald = 0.0;
+  T sq = 0, scal = 0;
+  if (wide) {
+    double pd = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const int jj = q + LPC * u;
+      if (j == jc && jj >= 1 && jj < len) pd += (double)L[u] * (double)L[u];
+    }
+    const double xd = wave_sum(pd);
+    if (xd != 0.0) {
+      const double a = alpha, bd = -copysign(sqrt(a * a + xd), a), taud = (bd - a) / bd;
+      beta = T(bd);
+      tau = T(taud);
+      sqd = sqrt(taud);
+      scald = 1.0 / (a - bd);
+    }
+  } else {
+    if (xn2 != T(0)) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tau = (beta - alpha) / beta;
+      scal = T(1) / (alpha - beta);
+    }
+    sq = sqrt(tau);
   }
-  const T sq = sqrt(tau);
   // w_i for i < len: lane (jc, q) owns x_i for i = q + LPC u; publish w through LDS
   if (j == jc) {
 #pragma unroll
     for (int u = 0; u < RPL; ++u) {
       const int jj = q + LPC * u;
       if (jj < len) {
-        const T wv = (jj == 0) ? sq : sq * (L[u] * scal);
+        const T wv = wide ? ((jj == 0) ? T(sqd) : T(sqd * ((double)L[u] * scald)))
+                          : ((jj == 0) ? sq : sq * (L[u] * scal));
         wsh[jj] = wv;
         v2[(int64_t)(rs + jj) + (int64_t)ldv2 * s] = wv;
       }
@@ -263,6 +300,40 @@ __device__ inline T part_sum(T v) {
   return v;
 }
 
+// The reflector of x = (alpha, x_1 ..) for one lane group: beta, and w = sqrt(tau) v over the lane's U entries (in
+// place of the x values w holds on entry) and its own element (wc, from xc). In FP32 the sum of squares is used while
+// it is at least 2^-100: every square that underflow could drop (below 2^-126) is then under the sum's rounding.
+// Below that the sum is taken again in FP64 from the entries and the reflector formed in FP64, so that ||w||^2 = 2
+// (an orthogonal reflector) however small the entries are; a reflector formed from a sum that lost squares to
+// underflow would not be orthogonal, and the back-transformation would lose orthogonality with it.
+template <typename T, int B, int H, int U>
+__device__ __forceinline__ void lane_reflector(T alpha, T xn2, int u0, int cl, T xc, T (&w)[U], T& beta, T& wc) {
+  if (sizeof(T) == 4 && __builtin_expect(!(xn2 >= T(0x1p-100)), 0)) {
+    double xd = (u0 == 0) ? 0.0 : (double)w[0] * (double)w[0];
+#pragma unroll
+    for (int j = 1; j < U; ++j) xd += (double)w[j] * (double)w[j];
+    if (H > 1) xd = part_sum<B, H>(xd);
+    double sqd = 0.0, s2d = 0.0;
+    beta = alpha;
+    if (xd != 0.0) {
+      const double a = alpha, bd = -copysign(sqrt(a * a + xd), a), dd = bd - a;
+      const double r = 1.0 / sqrt(dd * bd);
+      beta = T(bd);
+      sqd = fabs(dd) * r;
+      s2d = -copysign(r, dd);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j) w[j] = (j == 0 && u0 == 0) ? T(sqd) : T(s2d * (double)w[j]);
+    wc = (cl == 0) ? T(sqd) : T(s2d * (double)xc);
+    return;
+  }
+  T sq, s2;
+  larfg_scalars(alpha, xn2, beta, sq, s2);  // tau = 0 gives w = 0: every update below leaves its block unchanged
+#pragma unroll
+  for (int j = 0; j < U; ++j) w[j] = (j == 0 && u0 == 0) ? sq : s2 * w[j];
+  wc = (cl == 0) ? sq : s2 * xc;
+}
+
 // Lane-per-column form for b = B (8 or 16): H * B lanes carry one bulge. Lane (h, cl) owns column rs - b + cl of the
 // left block, row re + cl of the block below and row rs + cl of the diagonal block, over the U = B / H rows (columns) u
 // = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms followed, for H > 1, by a sum over
@@ -318,11 +389,8 @@ __device__ __forceinline__ bool chase_bulge_lanes(int n, T* __restrict__ ab, T* 
 #pragma unroll
   for (int j = 1; j < U; ++j) xn2 += w[j] * w[j];
   if (H > 1) xn2 = part_sum<B, H>(xn2);
-  T beta, sq, s2;
-  larfg_scalars(alpha, xn2, beta, sq, s2);  // tau = 0 gives w = 0: every update below leaves its block unchanged
-#pragma unroll
-  for (int j = 0; j < U; ++j) w[j] = (j == 0 && u0 == 0) ? sq : s2 * w[j];
-  const T wc = (cl == 0) ? sq : s2 * xc;
+  T beta, wc;
+  lane_reflector<T, B, H, U>(alpha, xn2, u0, cl, xc, w, beta, wc);
   if (h == 0 && rs + cl < n) v2s[rs + cl] = wc;
   T dl = 0, y = 0, p = 0;
 #pragma unroll
@@ -409,11 +477,8 @@ __device__ __forceinline__ bool chase_bulge_lanes_acc(int n, const Acc& ab, T* _
 #pragma unroll
   for (int j = 1; j < U; ++j) xn2 += w[j] * w[j];
   if (H > 1) xn2 = part_sum<B, H>(xn2);
-  T beta, sq, s2;
-  larfg_scalars(alpha, xn2, beta, sq, s2);  // tau = 0 gives w = 0: every update below leaves its block unchanged
-#pragma unroll
-  for (int j = 0; j < U; ++j) w[j] = (j == 0 && u0 == 0) ? sq : s2 * w[j];
-  const T wc = (cl == 0) ? sq : s2 * xc;
+  T beta, wc;
+  lane_reflector<T, B, H, U>(alpha, xn2, u0, cl, xc, w, beta, wc);
   if (h == 0 && rs + cl < n) v2s[rs + cl] = wc;
   T dl = 0, y = 0, p = 0;
 #pragma unroll

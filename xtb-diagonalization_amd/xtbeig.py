@@ -382,6 +382,26 @@ def dsymmetrize_pad(src, ldd=None, n=None):
 
 # ---- device-resident calls ---------------------------------------------------------------------------------------
 
+def _dev(t, dtype, numel, what):
+    """Refuses a device argument the C entry point would misread: a host tensor, another element type (an FP32
+    eigenvalue buffer handed to an entry point that writes FP64 ones would be overrun by half its size) or fewer
+    elements than the call writes or reads."""
+    import torch
+    dt = {"f64": torch.float64, "f32": torch.float32}[dtype]
+    if not t.is_cuda:
+        raise ValueError(f"{what}: expected a device tensor")
+    if t.dtype != dt:
+        raise ValueError(f"{what}: expected {dt}, got {t.dtype}")
+    if t.numel() < numel:
+        raise ValueError(f"{what}: expected at least {numel} elements, got {t.numel()}")
+
+
+def _dev_square(t, dtype, n, what):
+    _dev(t, dtype, n * n, what)
+    if n > 1 and (t.dim() != 2 or t.stride(1) != 1 or t.stride(0) < n):
+        raise ValueError(f"{what}: expected a row-major (n, ld >= n) tensor holding the column-major matrix")
+
+
 class Handle:
     """A device handle; run on the given torch stream (default: the handle's own stream)."""
 
@@ -406,6 +426,13 @@ class Handle:
     def _check(self, rc):
         if rc != 0:
             raise RuntimeError(last_error())
+
+    @staticmethod
+    def _args_sygvd(n, h, s, w, c):
+        _dev_square(h, "f64", n, "sygvd h")
+        _dev_square(s, "f64", n, "sygvd s")
+        _dev(w, "f64", n, "sygvd w")
+        _dev_square(c, "f64", n, "sygvd c")
 
     def set_stream(self, stream):
         raw = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
@@ -436,12 +463,16 @@ class Handle:
 
     def sygvd(self, h, s, w, c, precision=FP64):
         n = h.shape[0]
+        self._args_sygvd(n, h, s, w, c)
         self._check(lib().xtbeig_dev_sygvd(self._h, precision, n, vp(h.data_ptr()), h.stride(0),
                                            vp(s.data_ptr()), s.stride(0), vp(w.data_ptr()), vp(c.data_ptr()),
                                            c.stride(0)))
 
     def dsyevd_partial(self, a, w, z, m):
         n = a.shape[0]
+        _dev_square(a, "f64", n, "dsyevd_partial a")
+        _dev(w, "f64", n, "dsyevd_partial w")
+        _dev(z, "f64", n * m, "dsyevd_partial z")
         self._check(lib().xtbeig_dev_dsyevd_partial(self._h, n, m, vp(a.data_ptr()), n, vp(w.data_ptr()),
                                                     vp(z.data_ptr()), z.stride(0) if z.dim() == 2 else n))
 
@@ -466,14 +497,24 @@ class Handle:
 
     def dsyevd(self, a, w, z):
         n = a.shape[0]
+        _dev_square(a, "f64", n, "dsyevd a")
+        _dev(w, "f64", n, "dsyevd w")
+        _dev_square(z, "f64", n, "dsyevd z")
         self._check(lib().xtbeig_dev_dsyevd(self._h, n, vp(a.data_ptr()), n, vp(w.data_ptr()), vp(z.data_ptr()), n))
 
     def ssyevd(self, a, w, z):
+        """FP32 a and z; the eigenvalues w are FP64 (the divide and conquer runs in FP64)."""
         n = a.shape[0]
+        _dev_square(a, "f32", n, "ssyevd a")
+        _dev(w, "f64", n, "ssyevd w")
+        _dev_square(z, "f32", n, "ssyevd z")
         self._check(lib().xtbeig_dev_ssyevd(self._h, n, vp(a.data_ptr()), n, vp(w.data_ptr()), vp(z.data_ptr()), n))
 
     def dstedc(self, d, e, z):
         n = d.shape[0]
+        _dev(d, "f64", n, "dstedc d")
+        _dev(e, "f64", n - 1, "dstedc e")
+        _dev_square(z, "f64", n, "dstedc z")
         self._check(lib().xtbeig_dev_dstedc(self._h, n, vp(d.data_ptr()), vp(e.data_ptr()), vp(z.data_ptr()), n))
 
     def gemm(self, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
@@ -511,12 +552,17 @@ class Handle:
     # ---- graphs, batches, autotuning ----
     def sygvd_graph(self, h, s, w, c, precision=FP64):
         n = h.shape[0]
+        self._args_sygvd(n, h, s, w, c)
         self._check(lib().xtbeig_dev_sygvd_graph(self._h, precision, n, vp(h.data_ptr()), h.stride(0),
                                                  vp(s.data_ptr()), s.stride(0), vp(w.data_ptr()), vp(c.data_ptr()),
                                                  c.stride(0)))
 
     def sygvd_batched(self, hs, ss, ws, cs, precision=FP64, slots=8, use_graph=False):
         """Lists of device tensors (each h[q] n x n with ld = stride(0)); returns the per-solve latencies (ms)."""
+        if not (len(hs) == len(ss) == len(ws) == len(cs)):
+            raise ValueError("sygvd_batched: hs, ss, ws, cs must have one entry per problem")
+        for h, s_, w, c in zip(hs, ss, ws, cs):
+            self._args_sygvd(h.shape[0], h, s_, w, c)
         cnt = len(hs)
         arr_i = ctypes.c_int * max(cnt, 1)
         arr_p = vp * max(cnt, 1)
@@ -536,6 +582,10 @@ class Handle:
     def sygvd_grouped(self, hs, ss, ws, cs, precision=FP64, granule=0, slots=0):
         """Lists of device tensors as sygvd_batched; every kernel runs once per group of equal padded size. Returns
         (per-problem status words [count][4], per-problem latencies in ms)."""
+        if not (len(hs) == len(ss) == len(ws) == len(cs)):
+            raise ValueError("sygvd_grouped: hs, ss, ws, cs must have one entry per problem")
+        for h, s_, w, c in zip(hs, ss, ws, cs):
+            self._args_sygvd(h.shape[0], h, s_, w, c)
         cnt = len(hs)
         arr_i = ctypes.c_int * max(cnt, 1)
         arr_p = vp * max(cnt, 1)
