@@ -214,6 +214,10 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   const BackPlan& bp = h.backplan(n, b);
   reduce_to_band<T>(st, n, b, a, lda, s.v1, s.t1, s.ab, s.bwork);
   XTB_HIP(hipEventRecord(h.ev_band, st));
+  // first stream: the chase, then the tridiagonal's eigenpairs. The chase is enqueued before the second stream's
+  // work, so that it is queued behind the panels however long the host takes to enqueue the launches below.
+  chase_bulges<T>(st, n, b, s.ab, s.v2, s.dT, s.eT, h.dinfo + 1);
+  XTB_HIP(hipEventRecord(h.ev_chase, st));
   XTB_HIP(hipStreamWaitEvent(st2, h.ev_band, 0));
   // second stream: zt = (Q1 I)^T, qt = Q1 (both ldzt x rows, the rows past n zero)
   set_zero<T>(st2, ldzt, rows, s.zt, ldzt);
@@ -221,9 +225,6 @@ void syevd_core(Handle& h, const StdBuffers<T>& s, int n, int b, T* a, int lda, 
   back_transform_band<T>(st2, bp, s.v1, s.t1, s.zt, ldzt, n, s.btwork);
   set_zero<T>(st2, ldzt, rows - n, s.qt + (size_t)n * ldzt, ldzt);
   transpose_matrix<T, T>(st2, n, n, s.zt, ldzt, s.qt, ldzt);
-  // first stream: the chase, then the tridiagonal's eigenpairs
-  chase_bulges<T>(st, n, b, s.ab, s.v2, s.dT, s.eT, h.dinfo + 1);
-  XTB_HIP(hipEventRecord(h.ev_chase, st));
   // second stream: qt <- qt Q2 = Q1 Q2 (qt read as a column-major matrix with leading dimension ldzt), enqueued
   // before the divide and conquer so that it starts as soon as the chase ends
   XTB_HIP(hipStreamWaitEvent(st2, h.ev_chase, 0));
