@@ -121,7 +121,9 @@ def test_reduce_to_band(handle, torch_cuda, prec, b, n):
                                       # band widths past 32: the scalar chase and the 64-wide bulge window
                                       ("d", 64, 300), ("s", 48, 250), ("d", 48, 600), ("s", 64, 65),
                                       # the lane chase with its band split between device memory and LDS
-                                      ("d", 8, 1300), ("d", 8, 2000), ("s", 16, 1300)])
+                                      ("d", 8, 1300), ("d", 8, 2000), ("s", 16, 1300),
+                                      # b = 32: one wavefront per bulge, 16 rows per lane (whole band in LDS, split)
+                                      ("d", 32, 33), ("s", 32, 47), ("d", 32, 700), ("s", 32, 1300), ("d", 32, 1788)])
 def test_chase(handle, torch_cuda, prec, b, n):
     torch = torch_cuda
     rng = np.random.default_rng(5)
@@ -142,10 +144,11 @@ def test_chase(handle, torch_cuda, prec, b, n):
     handle.chase(n, b, AB, V2, D, E)
     assert handle.synchronize() == [0, 0, 0, 0]
     d, e = D.double().cpu().numpy(), E.double().cpu().numpy()
-    if prec == "d" and n <= (1000 if b <= 32 else 300):
+    if prec == "d" and n <= (1000 if b < 32 else 300):
         # same reflectors as the oracle's sequential chase: same tridiagonal to rounding (in FP32, and over the
-        # longer sweeps of large n, the tridiagonal form itself drifts with rounding, faster with the longer
-        # reflectors of b > 32: 1e-10 apart at b = 48, n = 600; only its spectrum is compared there)
+        # longer sweeps of large n, the tridiagonal form itself drifts with rounding, faster with longer reflectors:
+        # at b = 32, n = 700 the oracle's own d moves by 2e-9 at a few positions when its input is perturbed by one
+        # ulp, and b = 48, n = 600 gives 1e-10; only the spectrum is compared there)
         od, oe, _ = sbr.chase_bulges(full, b)
         assert np.max(np.abs(d - od)) < 1e-13 * n
         assert np.max(np.abs(np.abs(e[: n - 1]) - np.abs(oe))) < 1e-13 * n

@@ -334,7 +334,7 @@ __device__ __forceinline__ void lane_reflector(T alpha, T xn2, int u0, int cl, T
   wc = (cl == 0) ? sq : s2 * xc;
 }
 
-// Lane-per-column form for b = B (8 or 16): H * B lanes carry one bulge. Lane (h, cl) owns column rs - b + cl of the
+// Lane-per-column form for b = B (8, 16 or 32): H * B lanes carry one bulge. Lane (h, cl) owns column rs - b + cl of the
 // left block, row re + cl of the block below and row rs + cl of the diagonal block, over the U = B / H rows (columns) u
 // = h U ..< (h + 1) U, so the three block products are per-lane sums over U terms followed, for H > 1, by a sum over
 // the group's H parts. Every lane recomputes the reflector scalars from column c (read as a broadcast), w^T p is a sum
@@ -748,9 +748,12 @@ void launch_chase_lanes(hipStream_t s, int n, T* ab, T* v2, T* d, T* e, int* sta
 template <typename T>
 void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* status) {
   if (b > 64) throw std::runtime_error("chase_bulges: band width must be <= 64");
-  // lane-per-column bulges for b <= 16, register-resident wavefront bulges for b <= 32, the scalar form beyond
+  // lane-per-column bulges for b = 8, 16 and 32 (one wavefront per bulge at 32), register-resident wavefront bulges
+  // for the other b <= 32, the scalar form beyond
   if (b == 8 && n > 2) launch_chase_lanes<T, 8, 2>(s, n, ab, v2, d, e, status);
   else if (b == 16 && n > 2) launch_chase_lanes<T, 16, 2>(s, n, ab, v2, d, e, status);
+  // FP64 at 32: four wavefronts, so that each has the whole register file of its SIMD (eight spill the split form)
+  else if (b == 32 && n > 2) launch_chase_lanes<T, 32, 2, sizeof(T) == 8 ? 4 : 8>(s, n, ab, v2, d, e, status);
   else if (b <= 8) launch_chase<T, 8>(s, n, b, ab, v2, d, e, status);
   else if (b <= 16) launch_chase<T, 16>(s, n, b, ab, v2, d, e, status);
   else if (b <= 32) launch_chase<T, 32>(s, n, b, ab, v2, d, e, status);
@@ -759,13 +762,16 @@ void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* st
 
 bool chase_in_lds(int n, int b, bool fp32) {
   constexpr int WAVES = 8;
-  if (n > 2 && (b == 8 || b == 16)) {
+  if (n > 2 && (b == 8 || b == 16 || b == 32)) {
     size_t need;
     if (b == 8)
       need = fp32 ? chase_lanes_lds<8, 2, float>(n, WAVES, true) : chase_lanes_lds<8, 2, double>(n, WAVES, true);
-    else
+    else if (b == 16)
       need = fp32 ? chase_lanes_lds<16, 2, float>(n, WAVES, true) : chase_lanes_lds<16, 2, double>(n, WAVES, true);
-    return need + sizeof(int) * WAVES * (64 / (2 * b)) <= 160 * 1024;  // as launch_chase_lanes decides
+    else
+      need = fp32 ? chase_lanes_lds<32, 2, float>(n, WAVES, true) : chase_lanes_lds<32, 2, double>(n, 4, true);
+    const int waves = (b == 32 && !fp32) ? 4 : WAVES;
+    return need + sizeof(int) * waves * (64 / (2 * b)) <= 160 * 1024;  // as launch_chase_lanes decides
   }
   // as launch_chase decides
   const size_t el = fp32 ? sizeof(float) : sizeof(double);
