@@ -67,3 +67,46 @@ def test_sytrd_query_and_argument_checks_without_gpu():
     xtbeig.lib().xtbeig_dsytrd(b"U", xtbeig._i(n), None, xtbeig._i(n - 1), None, None, None, xtbeig._ptr(work),
                                xtbeig._i(-1), ctypes.byref(info))
     assert info.value == -4
+
+
+class _FakeDeviceTensor:
+    """Stands in for a device tensor (the checks read only these attributes), so that the host-side argument checks
+    of xtbeig.Handle run without a GPU."""
+
+    def __init__(self, dtype, shape):
+        self.is_cuda = True
+        self.dtype = dtype
+        self.shape = shape
+        self._numel = 1
+        for s in shape:
+            self._numel *= s
+
+    def numel(self):
+        return self._numel
+
+    def dim(self):
+        return len(self.shape)
+
+    def stride(self, k):
+        return self.shape[1] if (k == 0 and len(self.shape) == 2) else 1
+
+
+def test_device_call_argument_checks():
+    # the checks xtbeig.Handle runs before any launch: device memory, the entry point's element type, enough elements
+    import pytest
+    import torch
+
+    n = 8
+    with pytest.raises(ValueError, match="device tensor"):
+        xtbeig._dev(torch.zeros(n, dtype=torch.float64), "f64", n, "w")
+    with pytest.raises(ValueError, match="float64"):
+        xtbeig._dev(_FakeDeviceTensor(torch.float32, (n,)), "f64", n, "ssyevd w")
+    with pytest.raises(ValueError, match="at least 8"):
+        xtbeig._dev(_FakeDeviceTensor(torch.float64, (n - 1,)), "f64", n, "w")
+    xtbeig._dev(_FakeDeviceTensor(torch.float64, (n,)), "f64", n, "w")
+    with pytest.raises(ValueError, match="row-major"):
+        xtbeig._dev_square(_FakeDeviceTensor(torch.float64, (n * n,)), "f64", n, "a")
+    xtbeig._dev_square(_FakeDeviceTensor(torch.float64, (n, n)), "f64", n, "a")
+    with pytest.raises(ValueError, match="sygvd w"):
+        xtbeig.Handle._args_sygvd(n, _FakeDeviceTensor(torch.float64, (n, n)), _FakeDeviceTensor(torch.float64, (n, n)),
+                                  _FakeDeviceTensor(torch.float32, (n,)), _FakeDeviceTensor(torch.float64, (n, n)))
