@@ -41,16 +41,22 @@ struct Band {
   __device__ inline T& operator()(int i, int j) const { return ab[i + ld * j]; }  // band_elems < 2^31
 };
 
-// One bulge (s, k) by one wavefront. Returns false when the sweep has no such bulge.
+// One bulge (s, k) by one wavefront for 32 < b <= 64, the band in device memory or LDS. The reflector comes from the
+// column spread over the lanes (lane i holds x_i); every block product is a per-lane sum in ascending order. Each
+// block is held in registers, 64 elements per lane: the left block (lane = column), then the block below (lane = row),
+// then the diagonal block (lane = row), each loaded, updated and stored before the next is loaded. The three blocks
+// are disjoint (columns < rs; rows >= re; rows and columns in R), so storing one before loading the next changes
+// nothing, and one block at a time fits a wavefront's registers. Returns false when the sweep has no such bulge.
 template <typename T>
-__device__ inline bool chase_bulge(int n, int b, const Band<T>& A, T* __restrict__ v2, int ldv2, int s, int k,
-                                   T* wsh, T* qsh) {
+__device__ inline bool chase_bulge_blocks(int n, int b, const Band<T>& A, T* __restrict__ v2, int ldv2, int s, int k,
+                                          T* wsh, T* qsh) {
+  constexpr int MB = 64;  // widest band
   const int lane = threadIdx.x & 63;
   const int rs = s + 1 + k * b;
   if (rs >= n) return false;
   const int re = min(rs + b, n), len = re - rs;
   const int c = (k == 0) ? s : rs - b;
-  // larfg on x = A[R, c]
+  // larfg on x = A[R, c], lane i holding x_i
   const T xi = (lane < len) ? A(rs + lane, c) : T(0);
   const T alpha = __shfl(xi, 0, 64);
   const T xn2 = wave_sum((lane >= 1 && lane < len) ? xi * xi : T(0));
@@ -84,28 +90,44 @@ __device__ inline bool chase_bulge(int n, int b, const Band<T>& A, T* __restrict
   __builtin_amdgcn_wave_barrier();
   if (tau != T(0)) {
     const int lo = max(rs - b, 0), hi = min(re + b, n);
-    // left block: columns lo ..< rs, lane = column
+    // left block: columns lo ..< rs, lane = column (column c is rewritten below)
     if (lane < rs - lo) {
       const int j = lo + lane;
+      T L[MB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) L[i] = (i < len) ? A(rs + i, j) : T(0);
       T dot = 0;
-      for (int i = 0; i < len; ++i) dot += wsh[i] * A(rs + i, j);
-      for (int i = 0; i < len; ++i) A(rs + i, j) -= wsh[i] * dot;
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+        if (i < len) dot += wsh[i] * L[i];
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+        if (i < len) A(rs + i, j) = L[i] - wsh[i] * dot;
     }
     // block below: rows re ..< hi, lane = row
     if (lane < hi - re) {
       const int i = re + lane;
+      T Bl[MB];
+#pragma unroll
+      for (int jj = 0; jj < MB; ++jj) Bl[jj] = (jj < len) ? A(i, rs + jj) : T(0);
       T y = 0;
-      for (int jj = 0; jj < len; ++jj) y += A(i, rs + jj) * wsh[jj];
-      for (int jj = 0; jj < len; ++jj) A(i, rs + jj) -= y * wsh[jj];
+#pragma unroll
+      for (int jj = 0; jj < MB; ++jj)
+        if (jj < len) y += Bl[jj] * wsh[jj];
+#pragma unroll
+      for (int jj = 0; jj < MB; ++jj)
+        if (jj < len) A(i, rs + jj) = Bl[jj] - y * wsh[jj];
     }
-    // diagonal block R x R (lower triangle stored): p = A_RR w
+    // diagonal block R x R (lower triangle stored): p = A_RR w, then A_RR -= w q^T + q w^T
+    T D[MB];
     T p = 0;
     if (lane < len) {
       const int i = rs + lane;
-      for (int jj = 0; jj < len; ++jj) {
-        const T a = (jj <= lane) ? A(i, rs + jj) : A(rs + jj, i);
-        p += a * wsh[jj];
-      }
+#pragma unroll
+      for (int jj = 0; jj < MB; ++jj) D[jj] = (jj < len) ? ((jj <= lane) ? A(i, rs + jj) : A(rs + jj, i)) : T(0);
+#pragma unroll
+      for (int jj = 0; jj < MB; ++jj)
+        if (jj < len) p += D[jj] * wsh[jj];
     }
     const T half = wave_sum(lane < len ? wi * p : T(0)) * T(0.5);
     const T q = p - half * wi;
@@ -113,7 +135,9 @@ __device__ inline bool chase_bulge(int n, int b, const Band<T>& A, T* __restrict
     __builtin_amdgcn_wave_barrier();
     if (lane < len) {
       const int i = rs + lane;
-      for (int jj = 0; jj <= lane; ++jj) A(i, rs + jj) -= wi * qsh[jj] + q * wsh[jj];
+#pragma unroll
+      for (int jj = 0; jj < MB; ++jj)
+        if (jj <= lane) A(i, rs + jj) = D[jj] - (wi * qsh[jj] + q * wsh[jj]);
     }
   }
   __threadfence_block();
@@ -678,7 +702,7 @@ __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T
         }
         const bool more = (B > 0) ? chase_bulge_reg<T, (B > 0 ? B : 8)>(n, b, A, v2, ldv2, s, k, wbuf + wid * b,
                                                                           qbuf + wid * b)
-                                  : chase_bulge<T>(n, b, A, v2, ldv2, s, k, wbuf + wid * b, qbuf + wid * b);
+                                  : chase_bulge_blocks<T>(n, b, A, v2, ldv2, s, k, wbuf + wid * b, qbuf + wid * b);
         if (!more) break;
         if (lane == 0) st_release(&prog[wid], s * KB + (k + 1));
       }
@@ -702,7 +726,7 @@ __global__ __launch_bounds__(64 * CHASE_WAVES) void chase_kernel(int n, int b, T
 template <typename T, int B>
 void launch_chase(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* status) {
   // 16 wavefronts (sweeps in flight); 8 for b > 16, whose per-lane blocks need more than 128 registers
-  constexpr int CHASE_WAVES = (B == 32) ? 8 : 16;
+  constexpr int CHASE_WAVES = (B == 0 || B == 32) ? 8 : 16;
   const size_t vec_bytes = 2 * sizeof(T) * CHASE_WAVES * (size_t)b;
   const size_t band_bytes = sizeof(T) * (size_t)band_elems(n, b);
   if (vec_bytes + band_bytes + sizeof(int) * CHASE_WAVES <= 160 * 1024) {
@@ -775,7 +799,7 @@ bool chase_in_lds(int n, int b, bool fp32) {
   }
   // as launch_chase decides
   const size_t el = fp32 ? sizeof(float) : sizeof(double);
-  const int waves = (b > 16 && b <= 32) ? 8 : 16;
+  const int waves = b > 16 ? 8 : 16;
   return 2 * el * waves * (size_t)b + el * (size_t)band_elems(n, b) + sizeof(int) * waves <= 160 * 1024;
 }
 
