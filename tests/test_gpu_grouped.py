@@ -77,6 +77,32 @@ def test_grouped_unpadded_bit_identical(torch_cuda, prec):
     hnd.close()
 
 
+@pytest.mark.parametrize("prec", [xtbeig.FP64, xtbeig.FP32_REFINE])
+def test_grouped_band32_bit_identical(torch_cuda, prec):
+    # band width 32: the lane-per-column chase with one wavefront per bulge, over the group's z axis, with the band
+    # wholly in LDS (small n) and split between device memory and LDS (n = 300 in FP64)
+    torch = torch_cuda
+    ns = [300, 300, 150, 150, 100, 40]
+    probs = [workloads.generalized_pair(n, 900 + q) for q, n in enumerate(ns)]
+    hnd = xtbeig.Handle(0)
+    ref = xtbeig.Handle(0)
+    try:
+        hnd.set_tuning(32, 2)
+        ref.set_tuning(32, 2)
+        st, lat, out = grouped(torch, hnd, probs, prec, granule=1)
+        assert hnd.synchronize() == [0, 0, 0, 0]
+        assert all(s == [0, 0, 0, 0] for s in st), st
+        for q, (h, s) in enumerate(probs):
+            we, ce = solve_eager(torch, ref, h, s, prec)
+            w, c = out[q]
+            assert np.array_equal(w, we) and np.array_equal(c, ce), q
+            g = workloads.gates(h, s, w, c.T, sl.eigh(h, s, eigvals_only=True))
+            assert workloads.passes(g), (q, g)
+    finally:
+        ref.close()
+        hnd.close()
+
+
 @pytest.mark.parametrize("granule", [64, 16])
 @pytest.mark.parametrize("prec", [xtbeig.FP64, xtbeig.FP32_REFINE])
 def test_grouped_padded_uneven(torch_cuda, prec, granule):
