@@ -640,7 +640,8 @@ __global__ __launch_bounds__(64 * WAVES) void chase_lanes_kernel(int n, T* __res
       failed = true;
       s = sweeps;
     }
-    if (!__any(ready)) __builtin_amdgcn_s_sleep(1);
+    // a wavefront none of whose groups may go on polls again at once: a sleep between polls only delays the step
+    // that the flag releases (measured: 2.955 -> 2.93 ms at n = 1204, FP32); the spin count above still bounds the wait
   }
   if (failed && gl == 0) atomicExch(status, 1);
   __syncthreads();
