@@ -97,23 +97,30 @@ def fp32_near_underflow_cases():
 FP32_CASES = fp32_near_underflow_cases()
 
 
+@pytest.mark.parametrize("band", [0, 12, 32, 48])
 @pytest.mark.parametrize("name", list(FP32_CASES))
-def test_fp32_reflectors_stay_orthogonal(handle, torch_cuda, name):
+def test_fp32_reflectors_stay_orthogonal(handle, torch_cuda, name, band):
     """The FP32 standard solve where Householder vectors get so small that their squares underflow in FP32 (the
     bulges of a massively degenerate spectrum, a block 1e-21 below the rest): the reflectors are formed in FP64
     there, so the eigenvectors stay orthogonal to FP32 rounding (a sum of squares that lost terms to underflow once
-    gave an orthogonality error of 0.24). Eigenvalues to FP32 rounding of ||A||."""
+    gave an orthogonality error of 0.24). Eigenvalues to FP32 rounding of ||A||. Band widths: the built-in 8 (fused
+    panels, lane chase), 12 (general-path panels, register bulges), 32 (lane chase with one wavefront per bulge),
+    48 (block bulges)."""
     torch = torch_cuda
     a = FP32_CASES[name]
     n = a.shape[0]
     A = torch.from_numpy(np.ascontiguousarray(a.T)).cuda().float()
     w = torch.zeros(n, dtype=torch.float64, device="cuda")  # the eigenvalues come back in FP64
     z = torch.zeros((n, n), dtype=torch.float32, device="cuda")
-    handle.ssyevd(A, w, z)
-    assert handle.synchronize() == [0, 0, 0, 0]
+    handle.set_tuning(band, 2)
+    try:
+        handle.ssyevd(A, w, z)
+        assert handle.synchronize() == [0, 0, 0, 0]
+    finally:
+        handle.set_tuning(0, 2)
     ww, zz = w.double().cpu().numpy(), z.double().cpu().numpy().T
     ref = np.linalg.eigvalsh(a)
     nrm = np.abs(ref).max()
-    print(name, np.max(np.abs(ww - ref)) / nrm, np.max(np.abs(zz.T @ zz - np.eye(n))), flush=True)
+    print(name, band, np.max(np.abs(ww - ref)) / nrm, np.max(np.abs(zz.T @ zz - np.eye(n))), flush=True)
     assert np.max(np.abs(ww - ref)) < 1e-5 * nrm
     assert np.max(np.abs(zz.T @ zz - np.eye(n))) < 1e-5 * n ** 0.5
