@@ -774,7 +774,7 @@ template <typename T>
 void chase_bulges(hipStream_t s, int n, int b, T* ab, T* v2, T* d, T* e, int* status) {
   if (b > 64) throw std::runtime_error("chase_bulges: band width must be <= 64");
   // lane-per-column bulges for b = 8, 16 and 32 (one wavefront per bulge at 32), register-resident wavefront bulges
-  // for the other b <= 32, the scalar form beyond
+  // for the other b <= 32, wavefront bulges one register-staged block at a time beyond
   if (b == 8 && n > 2) launch_chase_lanes<T, 8, 2>(s, n, ab, v2, d, e, status);
   else if (b == 16 && n > 2) launch_chase_lanes<T, 16, 2>(s, n, ab, v2, d, e, status);
   // FP64 at 32: four wavefronts, so that each has the whole register file of its SIMD (eight spill the split form)
