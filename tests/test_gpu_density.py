@@ -12,6 +12,7 @@ Pins (no reference implementation exists for the backward):
 import numpy as np
 import pytest
 import scipy.linalg as sl
+from scipy.special import expit  # 1 / (1 + e^-x) without overflow
 
 import workloads
 import xtbeig
@@ -25,7 +26,7 @@ def np_mu(w, kT, nel):
     lo, hi = w.min() - 60 * kT - 1, w.max() + 60 * kT + 1
     for _ in range(200):
         mid = 0.5 * (lo + hi)
-        cnt = np.sum(1.0 / (1.0 + np.exp((w - mid) / kT)))
+        cnt = np.sum(expit(-(w - mid) / kT))
         lo, hi = (mid, hi) if cnt < nel else (lo, mid)
     return 0.5 * (lo + hi)
 
@@ -34,7 +35,7 @@ def np_density(h, s, kT, nel, mu=None):
     w, c = sl.eigh(h, s)
     if mu is None:
         mu = np_mu(w, kT, nel)
-    f = 1.0 / (1.0 + np.exp((w - mu) / kT))
+    f = expit(-(w - mu) / kT)
     return (c * f) @ c.T, mu, w, c
 
 
@@ -258,8 +259,8 @@ def test_autograd_function(handle, torch_cuda, recompute):
 def np_density_spin(h, s, kT, na, nb):
     w, c = sl.eigh(h, s)
     ma, mb = np_mu(w, kT, na), np_mu(w, kT, nb)
-    fa = 1.0 / (1.0 + np.exp((w - ma) / kT))
-    fb = 1.0 / (1.0 + np.exp((w - mb) / kT))
+    fa = expit(-(w - ma) / kT)
+    fb = expit(-(w - mb) / kT)
     return (c * (fa + fb)) @ c.T, (ma, mb), w, c, fa, fb
 
 
@@ -280,7 +281,7 @@ def test_density_spin_forward_and_closed_shell_limit(handle, torch_cuda, kT):
     # mu itself is not pinned where a count falls in a gap wider than kT (the count is flat there to rounding): the
     # counts it gives and P are
     for q, nq in ((0, na), (1, nb)):
-        cnt = np.sum(1.0 / (1.0 + np.exp((w - mu[0, q].item()) / kT)))
+        cnt = np.sum(expit(-(w - mu[0, q].item()) / kT))
         assert abs(cnt - nq) < 1e-10, (q, cnt, nq)
     pg = from_dev(p)[0]
     assert np.max(np.abs(pg - pn)) < 1e-12
