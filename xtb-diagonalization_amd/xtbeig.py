@@ -396,6 +396,16 @@ def _dev(t, dtype, numel, what):
         raise ValueError(f"{what}: expected at least {numel} elements, got {t.numel()}")
 
 
+def _prec(t, what):
+    """The element type of a stage call, from its first tensor: "f64" or "f32"."""
+    import torch
+    if t.dtype == torch.float64:
+        return "f64"
+    if t.dtype == torch.float32:
+        return "f32"
+    raise ValueError(f"{what}: expected torch.float64 or torch.float32, got {t.dtype}")
+
+
 def _dev_square(t, dtype, n, what):
     _dev(t, dtype, n * n, what)
     if n > 1 and (t.dim() != 2 or t.stride(1) != 1 or t.stride(0) < n):
@@ -519,33 +529,58 @@ class Handle:
 
     def gemm(self, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
         import torch
+        dt = _prec(c, "gemm c")
+        ra, ca = (m, k) if ta == "N" else (k, m)
+        rb, cb = (k, n) if tb == "N" else (n, k)
+        _dev(a, dt, lda * (ca - 1) + ra if ca > 0 else 0, "gemm a")
+        _dev(b, dt, ldb * (cb - 1) + rb if cb > 0 else 0, "gemm b")
+        _dev(c, dt, ldc * (n - 1) + m if n > 0 else 0, "gemm c")
         f = lib().xtbeig_dev_dgemm if c.dtype == torch.float64 else lib().xtbeig_dev_sgemm
         self._check(f(self._h, ta.encode(), tb.encode(), m, n, k, alpha, vp(a.data_ptr()), lda, vp(b.data_ptr()),
                       ldb, beta, vp(c.data_ptr()), ldc))
 
     def potrf_inverse(self, s, u, x):
         n = s.shape[0]
+        for t, nm in ((s, "s"), (u, "u"), (x, "x")):
+            _dev_square(t, "f64", n, "potrf_inverse " + nm)
         self._check(lib().xtbeig_dev_potrf_inverse(self._h, n, vp(s.data_ptr()), n, vp(u.data_ptr()), n,
                                                    vp(x.data_ptr()), n))
 
     def reduce_to_band(self, a, b, v1, t1, ab):
         import torch
         n = a.shape[0]
+        dt = _prec(a, "reduce_to_band a")
+        _dev_square(a, dt, n, "reduce_to_band a")
+        _dev(v1, dt, n * n, "reduce_to_band v1")
+        _dev(t1, dt, n * b, "reduce_to_band t1")
+        _dev(ab, dt, band_elems(n, b), "reduce_to_band ab")
         f = lib().xtbeig_dev_dreduce_to_band if a.dtype == torch.float64 else lib().xtbeig_dev_sreduce_to_band
         self._check(f(self._h, n, b, vp(a.data_ptr()), n, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(ab.data_ptr())))
 
     def chase(self, n, b, ab, v2, d, e):
         import torch
+        dt = _prec(ab, "chase ab")
+        _dev(ab, dt, band_elems(n, b), "chase ab")
+        _dev(v2, dt, n * n, "chase v2")
+        _dev(d, dt, n, "chase d")
+        _dev(e, dt, max(n - 1, 0), "chase e")
         f = lib().xtbeig_dev_dchase if ab.dtype == torch.float64 else lib().xtbeig_dev_schase
         self._check(f(self._h, n, b, vp(ab.data_ptr()), vp(v2.data_ptr()), vp(d.data_ptr()), vp(e.data_ptr())))
 
     def back_bulges(self, n, b, v2, z, m):
         import torch
+        dt = _prec(z, "back_bulges z")
+        _dev(v2, dt, n * n, "back_bulges v2")
+        _dev(z, dt, n * m, "back_bulges z")
         f = lib().xtbeig_dev_dback_bulges if z.dtype == torch.float64 else lib().xtbeig_dev_sback_bulges
         self._check(f(self._h, n, b, vp(v2.data_ptr()), vp(z.data_ptr()), n, m))
 
     def back_band(self, n, b, v1, t1, z, m):
         import torch
+        dt = _prec(z, "back_band z")
+        _dev(v1, dt, n * n, "back_band v1")
+        _dev(t1, dt, n * b, "back_band t1")
+        _dev(z, dt, n * m, "back_band z")
         f = lib().xtbeig_dev_dback_band if z.dtype == torch.float64 else lib().xtbeig_dev_sback_band
         self._check(f(self._h, n, b, vp(v1.data_ptr()), vp(t1.data_ptr()), vp(z.data_ptr()), n, m))
 
